@@ -1,0 +1,85 @@
+/*
+ * jhtx -- MI355X (gfx950) HEALPix on-grid spherical-harmonic synthesis and its exact
+ * adjoint, spin 0 and spin 2, behind a plain C ABI (no torch or HIP types beyond an
+ * opaque stream handle).
+ *
+ * Conventions are those of jrcheshire/jht v0.2.0:
+ *   - a_lm in healpy m-major packing, idx(l,m) = m(2 lmax + 1 - m)/2 + l,
+ *     K = (lmax+1)(lmax+2)/2 complex128 (interleaved re, im) per component;
+ *   - orthonormal Y_lm with the Condon-Shortley phase, map = sum a_lm Y_lm;
+ *   - spin 2: input (E, B), output (Q, U) in the HEALPix/COSMO convention;
+ *   - maps are RING ordered float64, npix = 12 nside^2 per component.
+ *
+ * Buffers are caller-owned DEVICE pointers.  A call never allocates: the caller passes a
+ * workspace of jhtx_plan_workspace_bytes(plan) bytes.  Calls are ordered on the given
+ * stream.  A plan is immutable after creation and may be used concurrently from several
+ * streams provided each call gets its own workspace.  Every function returns 0 on success
+ * or a negative JHTX_E* code; jhtx_last_error() then describes the failure (thread-local).
+ */
+#ifndef JHTX_H
+#define JHTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jhtx_plan jhtx_plan;
+
+enum {
+  JHTX_OK = 0,
+  JHTX_EINVAL = -1,       /* bad argument (nside, lmax, spin, batch, null pointer) */
+  JHTX_EUNSUPPORTED = -2, /* spin not in {0, 2} (reference raises NotImplementedError) */
+  JHTX_EDEVICE = -3,      /* a HIP runtime call failed */
+  JHTX_ENOMEM = -4        /* plan tables could not be allocated */
+};
+
+enum {
+  JHTX_PREC_F64 = 0,   /* FP64 recursion and FP64 contraction: <= 1e-12 vs float64 jht */
+  JHTX_PREC_MIXED = 1  /* FP64 recursion, FP32 contraction on MFMA (batched) */
+};
+
+/* Flags for jhtx_plan_create. */
+enum {
+  JHTX_FLAG_BLUESTEIN_ALL = 1 /* route every ring DFT through the chirp-z path */
+};
+
+/* Replaces jht's cached `_prepare(nside, lmax, spin, fft_mode)` closure pair
+ * (src/jht/healpix.py:184-357): builds the geometry, the triangle-packed recursion
+ * coefficients and the seed tables once, on the device.  max_batch bounds the `batch`
+ * argument of the calls (the workspace is sized for it). */
+int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch, int flags,
+                     jhtx_plan** plan);
+
+/* Bytes of device workspace a call on this plan needs (for max_batch maps). */
+size_t jhtx_plan_workspace_bytes(const jhtx_plan* plan);
+
+/* Replaces jht.synthesis(alm, nside, lmax, spin) (src/jht/healpix.py:363-387) and the
+ * jitted closure it dispatches to (healpix.py:280-291 spin 0, :312-330 spin 2).
+ * alm: batch x ncomp x K complex128 (interleaved); map: batch x ncomp x npix float64. */
+int jhtx_synth(jhtx_plan* plan, const void* alm, void* map, int batch, void* workspace,
+               void* stream);
+
+/* Replaces jht.adjoint_synthesis(m, nside, lmax, spin) (src/jht/healpix.py:390-412) and
+ * its closure (healpix.py:293-305 spin 0, :332-355 spin 2): the exact transpose of
+ * jhtx_synth.  map: batch x ncomp x npix float64; alm: batch x ncomp x K complex128. */
+int jhtx_adjoint(jhtx_plan* plan, const void* map, void* alm, int batch, void* workspace,
+                 void* stream);
+
+/* Releases the plan's device tables (the plan must not be in use on any stream). */
+void jhtx_plan_destroy(jhtx_plan* plan);
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char* jhtx_last_error(void);
+
+/* Plan introspection: 0 nside, 1 lmax, 2 spin, 3 precision, 4 max_batch, 5 ncomp,
+ * 6 nrings, 7 npix (low 31 bits), 8 number of ring-FFT launch classes. -1 if unknown. */
+int jhtx_plan_query(const jhtx_plan* plan, int what);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JHTX_H */

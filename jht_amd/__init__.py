@@ -1,0 +1,24 @@
+"""jht_amd -- MI355X-native HEALPix on-grid synthesis / adjoint (drop-in for jht's pair).
+
+The operators are hand-written HIP kernels for gfx950 in ``libjhtx.so`` (C ABI in
+``include/jhtx.h``) driven from torch tensors.  Importing this package does not touch
+the GPU; the first transform loads the library and builds a plan.
+"""
+
+from .healpix import (  # noqa: F401
+    RingInfo,
+    adjoint_synthesis,
+    adjoint_synthesis_batch,
+    alm_column_base,
+    alm_metric_weight,
+    alm_size,
+    clear_plans,
+    enable_looped_fft,
+    get_azimuth_fft_mode,
+    get_plan,
+    set_azimuth_fft_mode,
+    synthesis,
+    synthesis_batch,
+)
+
+__version__ = "0.1.0"

@@ -1,0 +1,80 @@
+"""ctypes binding of the jhtx C ABI (``include/jhtx.h``), built in-tree as ``libjhtx.so``.
+
+There is no fallback: if the library is missing or a call fails, an exception is raised.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjhtx.so")
+
+# Every symbol include/jhtx.h declares.
+EXPORTS = (
+    "jhtx_plan_create",
+    "jhtx_plan_workspace_bytes",
+    "jhtx_synth",
+    "jhtx_adjoint",
+    "jhtx_plan_destroy",
+    "jhtx_last_error",
+    "jhtx_plan_query",
+)
+
+JHTX_OK = 0
+JHTX_EINVAL = -1
+JHTX_EUNSUPPORTED = -2
+JHTX_EDEVICE = -3
+JHTX_ENOMEM = -4
+
+PREC_F64 = 0
+PREC_MIXED = 1
+FLAG_BLUESTEIN_ALL = 1
+
+
+class JhtxError(RuntimeError):
+    """A jhtx C call returned an error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"jhtx error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load() -> ctypes.CDLL:
+    """Load ``libjhtx.so`` once; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"jhtx HIP library not found at {LIB_PATH}; build it first "
+            "(python -m jht_amd.build, or __graft_entry__.build())"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.jhtx_plan_create.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(vp)]
+    lib.jhtx_plan_create.restype = ctypes.c_int
+    lib.jhtx_plan_workspace_bytes.argtypes = [vp]
+    lib.jhtx_plan_workspace_bytes.restype = ctypes.c_size_t
+    for name in ("jhtx_synth", "jhtx_adjoint"):
+        fn = getattr(lib, name)
+        fn.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp]
+        fn.restype = ctypes.c_int
+    lib.jhtx_plan_destroy.argtypes = [vp]
+    lib.jhtx_plan_destroy.restype = None
+    lib.jhtx_last_error.argtypes = []
+    lib.jhtx_last_error.restype = ctypes.c_char_p
+    lib.jhtx_plan_query.argtypes = [vp, ctypes.c_int]
+    lib.jhtx_plan_query.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != JHTX_OK:
+        msg = load().jhtx_last_error().decode(errors="replace")
+        raise JhtxError(code, msg)

@@ -1,0 +1,311 @@
+// C-ABI entry points and plan construction for the jhtx library (see include/jhtx.h).
+//
+// Plan creation replaces jht's cached `_prepare` (src/jht/healpix.py:184-357): it builds
+// the RING geometry of the north half + equator (healpix.py:107-140, same floating-point
+// expressions so the colatitudes are bit-identical), the per-m seeds
+// (_recursion.py:53-56, :166-188) and the triangle-packed, rescaled recursion
+// coefficients (_recursion.py:37-50, :136-143), then uploads them once.
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "jhtx_internal.h"
+
+namespace jhtx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+template <class T>
+int upload(T** dst, const std::vector<T>& src) {
+  if (src.empty()) {
+    *dst = nullptr;
+    return JHTX_OK;
+  }
+  if (hipMalloc(dst, sizeof(T) * src.size()) != hipSuccess) {
+    set_error("hipMalloc failed for a plan table");
+    return JHTX_ENOMEM;
+  }
+  if (hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("hipMemcpy failed for a plan table");
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+// z = cos(theta) of the 0-based north ring t < 2 nside, as RingInfo computes it.
+double ring_z(int nside, int t) {
+  const long long i = t + 1;
+  if (i < nside) {
+    const double num = (double)(i * i);
+    const double den = 3.0 * (double)((long long)nside * nside);
+    return 1.0 - num / den;
+  }
+  return 4.0 / 3.0 - (2.0 * (double)i) / (3.0 * (double)nside);
+}
+
+// Wigner-d single-term seed parameters of d^{lmin}_{M,M'} (_recursion.py:166-188).
+void wigner_seed_params(int M, int Mp, int lmin, double out[4]) {
+  const int k0 = std::max(0, M - Mp);
+  auto lg = [](double v) { return std::lgamma(v); };
+  const double log_fac =
+      0.5 * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) -
+      (lg(k0 + 1) + lg(lmin + M - k0 + 1) + lg(lmin - Mp - k0 + 1) + lg(k0 - M + Mp + 1));
+  out[0] = log_fac;
+  out[1] = (double)(2 * lmin - 2 * k0 + M - Mp);  // power of cos(theta/2)
+  out[2] = (double)(2 * k0 - M + Mp);              // power of sin(theta/2)
+  out[3] = (k0 % 2) ? -1.0 : 1.0;
+}
+
+int build_rec_tables(jhtx_plan* p) {
+  const int lmax = p->lmax, M = p->M, T = p->T, nside = p->nside;
+  const long long K = p->K;
+  std::vector<double> x(T), lsin(T), lcos;
+  for (int t = 0; t < T; ++t) {
+    const double z = ring_z(nside, t);
+    x[t] = z;
+    if (p->spin == 0) {
+      const double s2 = std::min(std::max(1.0 - z * z, 0.0), 1.0);
+      lsin[t] = std::log(std::sqrt(s2));
+    }
+  }
+  if (p->spin == 2) {
+    lcos.resize(T);
+    for (int t = 0; t < T; ++t) {
+      const double z = x[t];
+      const double ch = std::sqrt(std::min(std::max((1.0 + z) / 2.0, 0.0), 1.0));
+      const double sh = std::sqrt(std::min(std::max((1.0 - z) / 2.0, 0.0), 1.0));
+      lcos[t] = std::log(ch);
+      lsin[t] = std::log(sh);
+    }
+  }
+  std::vector<double> alpha(K, 0.0), G(K, 0.0), gamma, seed;
+  if (p->spin == 2) gamma.assign(K, 0.0);
+  const long double four_pi = 4.0L * 3.14159265358979323846264338327950288L;
+  if (p->spin == 0) {
+    seed.resize(M);
+    long double acc = 0.0L;  // sum_{k<=m} log((2k+1)/(2k))
+    for (int m = 0; m < M; ++m) {
+      if (m > 0) acc += logl((2.0L * m + 1.0L) / (2.0L * m));
+      seed[m] = (double)(-0.5L * logl(four_pi) + 0.5L * acc);
+    }
+    std::vector<long double> g(lmax + 2);
+    for (int m = 0; m < M; ++m) {
+      const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+      const long double mm = m;
+      g[m] = 1.0L;
+      if (m + 1 <= lmax) g[m + 1] = 1.0L;
+      for (int l = m + 2; l <= lmax; ++l) {
+        const long double L = l;
+        const long double b = sqrtl((2 * L + 1) * ((L - 1) * (L - 1) - mm * mm) /
+                                    ((2 * L - 3) * (L - mm) * (L + mm)));
+        g[l] = b * g[l - 2];
+      }
+      G[cb + m] = 1.0;
+      for (int l = m + 1; l <= lmax; ++l) {
+        const long double L = l;
+        const long double a = sqrtl((2 * L - 1) * (2 * L + 1) / ((L - mm) * (L + mm)));
+        alpha[cb + l] = (double)(a * g[l - 1] / g[l]);
+        G[cb + l] = (double)g[l];
+      }
+    }
+  } else {
+    seed.resize(8 * (size_t)M);
+    std::vector<long double> g(lmax + 2);
+    for (int m = 0; m < M; ++m) {
+      const int lmin = std::max(m, 2);
+      wigner_seed_params(-m, 2, lmin, &seed[8 * m]);
+      wigner_seed_params(-m, -2, lmin, &seed[8 * m + 4]);
+      if (lmin > lmax) continue;
+      const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+      const long double mm = m;
+      // d^{l'+1} = (A x + C) d^{l'} - B d^{l'-1}; coefficients of the produced degree l = l'+1
+      auto coefs = [&](int l, long double& A, long double& B, long double& C) {
+        const long double lp = l - 1;
+        const long double D = lp * sqrtl(((lp + 1) * (lp + 1) - mm * mm) * ((lp + 1) * (lp + 1) - 4.0L));
+        A = (2 * lp + 1) * lp * (lp + 1) / D;
+        C = 2.0L * mm * (2 * lp + 1) / D;  // -(2l'+1) M M' / D with M = -m, M' = +2
+        B = (lp + 1) * sqrtl((lp * lp - mm * mm) * (lp * lp - 4.0L)) / D;
+      };
+      g[lmin] = 1.0L;
+      if (lmin + 1 <= lmax) g[lmin + 1] = 1.0L;
+      for (int l = lmin + 2; l <= lmax; ++l) {
+        long double A, B, C;
+        coefs(l, A, B, C);
+        g[l] = B * g[l - 2];
+      }
+      const long double sg = (m & 1) ? -1.0L : 1.0L;
+      for (int l = lmin; l <= lmax; ++l) {
+        const long double pref = sg * sqrtl((2.0L * l + 1.0L) / four_pi);
+        G[cb + l] = (double)(g[l] * pref);
+        if (l > lmin) {
+          long double A, B, C;
+          coefs(l, A, B, C);
+          alpha[cb + l] = (double)(A * g[l - 1] / g[l]);
+          gamma[cb + l] = (double)(C * g[l - 1] / g[l]);
+        }
+      }
+    }
+  }
+  int rc;
+  if ((rc = upload(&p->rec.alpha, alpha))) return rc;
+  if ((rc = upload(&p->rec.G, G))) return rc;
+  if ((rc = upload(&p->rec.gamma, gamma))) return rc;
+  if ((rc = upload(&p->rec.x, x))) return rc;
+  if ((rc = upload(&p->rec.lsin, lsin))) return rc;
+  if ((rc = upload(&p->rec.lcos, lcos))) return rc;
+  if ((rc = upload(&p->rec.seed, seed))) return rc;
+  return JHTX_OK;
+}
+
+void free_rec_tables(jhtx_plan* p) {
+  double* ptrs[] = {p->rec.alpha, p->rec.gamma, p->rec.G, p->rec.x,
+                    p->rec.lsin,  p->rec.lcos,  p->rec.seed};
+  for (double* q : ptrs)
+    if (q) hipFree(q);
+  p->rec = RecTables();
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+int check_call(const jhtx_plan* p, const void* in, void* out, int batch, void* ws) {
+  if (!p) {
+    set_error("null plan");
+    return JHTX_EINVAL;
+  }
+  if (!in || !out || !ws) {
+    set_error("null buffer pointer");
+    return JHTX_EINVAL;
+  }
+  if (batch < 1 || batch > p->max_batch) {
+    set_error("batch " + std::to_string(batch) + " outside [1, " + std::to_string(p->max_batch) + "]");
+    return JHTX_EINVAL;
+  }
+  return JHTX_OK;
+}
+
+}  // namespace
+}  // namespace jhtx
+
+using namespace jhtx;
+
+extern "C" {
+
+const char* jhtx_last_error(void) { return g_err.c_str(); }
+
+int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch, int flags,
+                     jhtx_plan** out) {
+  g_err.clear();
+  if (!out) {
+    set_error("null plan output pointer");
+    return JHTX_EINVAL;
+  }
+  *out = nullptr;
+  if (spin != 0 && spin != 2) {
+    set_error("spin=" + std::to_string(spin) + " unsupported (only 0 and 2)");
+    return JHTX_EUNSUPPORTED;
+  }
+  if (nside < 1 || nside > 8192 || lmax < 0 || lmax > 32767 || max_batch < 1) {
+    set_error("invalid nside/lmax/max_batch");
+    return JHTX_EINVAL;
+  }
+  if (precision != JHTX_PREC_F64) {
+    set_error("precision " + std::to_string(precision) + " not available in this build");
+    return JHTX_EUNSUPPORTED;
+  }
+  jhtx_plan* p = new jhtx_plan();  // value-initialised: every field starts at zero / null
+  p->nside = nside;
+  p->lmax = lmax;
+  p->spin = spin;
+  p->precision = precision;
+  p->max_batch = max_batch;
+  p->flags = flags;
+  p->M = lmax + 1;
+  p->T = 2 * nside;
+  p->nrings = 4 * nside - 1;
+  p->ncomp = spin ? 2 : 1;
+  p->npix = 12LL * nside * nside;
+  p->K = (long long)(lmax + 1) * (lmax + 2) / 2;
+  // rings per lane and work-group size of the recursion kernels
+  if (spin == 0)
+    p->rec_R = (p->T >= 512) ? 4 : 1;
+  else
+    p->rec_R = (p->T >= 256) ? 2 : 1;
+  const int lanes = (p->T + p->rec_R - 1) / p->rec_R;
+  p->rec_block = std::min(256, std::max(64, ((lanes + 63) / 64) * 64));
+  p->nchunk = (p->T + p->rec_block * p->rec_R - 1) / (p->rec_block * p->rec_R);
+  p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
+  p->ws_part_bytes = align256(sizeof(double2) * (size_t)p->nchunk * max_batch * p->ncomp * p->K);
+  p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
+  int rc = build_rec_tables(p);
+  if (rc == JHTX_OK) rc = build_fft_tables(p);
+  if (rc != JHTX_OK) {
+    free_rec_tables(p);
+    free_fft_tables(p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return JHTX_OK;
+}
+
+size_t jhtx_plan_workspace_bytes(const jhtx_plan* p) { return p ? p->ws_bytes : 0; }
+
+int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, void* stream) {
+  int rc = check_call(p, alm, map, batch, ws);
+  if (rc) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  double2* F = reinterpret_cast<double2*>(ws);
+  hipError_t e = launch_synth_rec(p, reinterpret_cast<const double2*>(alm), F, batch, s);
+  if (e == hipSuccess) e = launch_ring_synth(p, F, reinterpret_cast<double*>(map), batch, s);
+  if (e != hipSuccess) {
+    set_error(std::string("synthesis launch failed: ") + hipGetErrorString(e));
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, void* stream) {
+  int rc = check_call(p, map, alm, batch, ws);
+  if (rc) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  double2* V = reinterpret_cast<double2*>(ws);
+  double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(ws) + p->ws_F_bytes);
+  hipError_t e = launch_ring_adjoint(p, reinterpret_cast<const double*>(map), V, batch, s);
+  if (e == hipSuccess)
+    e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
+  if (e != hipSuccess) {
+    set_error(std::string("adjoint launch failed: ") + hipGetErrorString(e));
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+void jhtx_plan_destroy(jhtx_plan* p) {
+  if (!p) return;
+  free_rec_tables(p);
+  free_fft_tables(p);
+  delete p;
+}
+
+int jhtx_plan_query(const jhtx_plan* p, int what) {
+  if (!p) return -1;
+  switch (what) {
+    case 0: return p->nside;
+    case 1: return p->lmax;
+    case 2: return p->spin;
+    case 3: return p->precision;
+    case 4: return p->max_batch;
+    case 5: return p->ncomp;
+    case 6: return p->nrings;
+    case 7: return (int)(p->npix & 0x7fffffff);
+    case 8: return p->fft.ncls;
+    default: return -1;
+  }
+}
+
+}  // extern "C"

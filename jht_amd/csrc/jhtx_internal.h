@@ -1,0 +1,122 @@
+// Internal definitions shared by the jhtx HIP sources (plan layout, kernel argument
+// blocks, launchers).  Not part of the public C ABI (include/jhtx.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <string>
+
+#include "../../include/jhtx.h"
+
+namespace jhtx {
+
+// Exponent-tracking constants of the on-the-fly recursion.  A recursion value is
+// stored as mu * 2^(kScaleStep * ex) with ex <= 0; while ex < 0 the true value is below
+// 2^-256 and its contribution is dropped (it is below double precision of any output).
+constexpr double kBig = 0x1p256;
+constexpr double kRescale = 0x1p-512;
+constexpr int kScaleStep = 512;
+
+// Per-plan device tables for the recursion kernels.
+struct RecTables {
+  // triangle-packed by the healpy index idx(l, m) = m(2 lmax + 1 - m)/2 + l
+  double* alpha = nullptr;  // mu_l = (alpha_l x + gamma_l) mu_{l-1} - mu_{l-2}
+  double* gamma = nullptr;  // spin 2 only (the -2 recursion uses -gamma)
+  double* G = nullptr;      // lambda_l = G_l mu_l
+  // north half + equator rings, t < T = 2 nside
+  double* x = nullptr;      // cos(theta_t)
+  double* lsin = nullptr;   // spin 0: log sin(theta); spin 2: log sin(theta/2)
+  double* lcos = nullptr;   // spin 2: log cos(theta/2)
+  // per-m seed parameters: spin 0 -> 1 double (theta-independent log|lambda_mm|);
+  // spin 2 -> 8 doubles (log_fac, pow_cos, pow_sin, sign) for s = +2 then s = -2
+  double* seed = nullptr;
+};
+
+// Argument block for the recursion kernels (passed by value).
+struct RecArgs {
+  const double* alpha;
+  const double* gamma;
+  const double* G;
+  const double* x;
+  const double* lsin;
+  const double* lcos;
+  const double* seed;
+  int lmax, M, T, nside, nrings, nchunk;
+  long long K;
+};
+
+// Ring-FFT launch class: ring pairs whose transform length fits one work-group shape.
+struct FftClass {
+  int block = 64;          // threads per work-group
+  int lds_doubles = 0;     // dynamic LDS (doubles)
+  int npairs = 0;
+  int* pairs = nullptr;    // device list of ring-pair indices p (0 <= p < 2 nside)
+};
+
+struct FftTables {
+  double2* tw = nullptr;  // exp(-2 pi i k / lmax_fft), k < lmax_fft
+  int lmax_fft = 0;       // largest power-of-two transform length in the plan
+  FftClass cls[3];
+  int ncls = 0;
+  long long* bofs = nullptr;  // device, per ring pair: offset of its Bluestein kernel spectrum or -1
+  double2* bspec = nullptr;   // device, concatenated kernel spectra
+  long long bspec_len = 0;
+};
+
+}  // namespace jhtx
+
+struct jhtx_plan {
+  int nside, lmax, spin, precision, max_batch, flags;
+  int M, T, nrings, ncomp, nchunk;
+  long long npix, K;
+  int rec_block;  // threads per recursion work-group
+  int rec_R;      // rings per lane
+  jhtx::RecTables rec;
+  jhtx::FftTables fft;
+  // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2][bpart: nchunk*batch*ncomp*K double2]
+  size_t ws_F_bytes, ws_part_bytes, ws_bytes;
+};
+
+namespace jhtx {
+
+void set_error(const std::string& msg);
+
+// launchers (return hipError_t)
+hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+                            hipStream_t s);
+hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
+                          int batch, hipStream_t s);
+hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
+                             hipStream_t s);
+hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,
+                               hipStream_t s);
+
+// FFT plan building (host): fills p->fft
+int build_fft_tables(jhtx_plan* p);
+void free_fft_tables(jhtx_plan* p);
+
+// ring geometry helpers (host and device)
+__host__ __device__ inline int ring_len(int nside, int r /*0-based*/) {
+  const int i = r + 1;
+  if (i < nside) return 4 * i;
+  if (i > 3 * nside) return 4 * (4 * nside - i);
+  return 4 * nside;
+}
+// true when the ring's first pixel sits at phi0 = pi / N (half-sample shift)
+__host__ __device__ inline bool ring_shifted(int nside, int r) {
+  const int i = r + 1;
+  if (i < nside || i > 3 * nside) return true;
+  return ((i - nside) % 2) == 0;
+}
+__host__ __device__ inline long long ring_start(int nside, int r) {
+  const long long i = r + 1;
+  const long long ns = nside;
+  if (i < ns) return 2 * i * (i - 1);
+  if (i <= 3 * ns) return 2 * ns * (ns - 1) + (i - ns) * 4 * ns;
+  const long long j = 4 * ns - i;  // mirror of north cap ring j
+  return 12 * ns * ns - 2 * j * (j + 1);
+}
+
+}  // namespace jhtx

@@ -1,0 +1,728 @@
+// FP64 recursion + contraction kernels: the on-the-fly scaled associated-Legendre (spin 0)
+// and Wigner-d (spin +-2) recursions over l for one order m and a tile of north-half
+// rings, contracted against a_lm (synthesis) or reduced over rings (adjoint).
+//
+// Follows the algorithm of src/jht/_recursion.py:441-572 (synth_contract_eo,
+// adjoint_contract_eo, synth_contract_spin2_ns, adjoint_contract_spin2_ns): one pass over
+// the north half + equator gives both hemispheres through
+//     lambda^s_lm(pi - theta) = (-1)^(l+m) lambda^-s_lm(theta).
+// Differences of method (same operator, same tolerance):
+//   * the three-term recursion runs on mu_l = lambda_l / G_l with the coefficient of
+//     mu_{l-2} scaled to exactly 1, so each step is one FMA pair; G_l (and the spin-2
+//     prefactor (-1)^m sqrt((2l+1)/4pi)) is folded into a_lm when it is staged;
+//   * instead of jht's per-step log-renormalisation (_recursion.py:358-368) the seed is
+//     carried as mantissa x 2^(512 ex) and rescaled by exact powers of two; while ex < 0
+//     the true value is below 2^-256 and its contribution is dropped;
+//   * only the active triangle l >= max(m, |s|) is iterated (no masked steps);
+//   * lanes own rings (R per lane); the even/odd (l - m) accumulators give the north
+//     value as their sum and the (-1)^m-reflected south value as their difference.
+#include "jhtx_internal.h"
+
+namespace jhtx {
+
+namespace {
+
+constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+constexpr double kLog2e = 1.44269504088896338700e+00;
+constexpr int CH = 64;  // l values staged per LDS chunk
+
+// exp(L) = sign * mu * 2^(512 ex), mu in [2^-256, 2^256), ex <= 0.
+__device__ __forceinline__ void seed_split(double L, double sign, double& mu, int& ex) {
+  if (!(L > -1.0e300)) {  // exact pole (never on the HEALPix grid)
+    mu = 0.0;
+    ex = 0;
+    return;
+  }
+  int e = (int)floor((L * kLog2e + 256.0) * (1.0 / 512.0));
+  if (e > 0) e = 0;
+  const double n = (double)(kScaleStep * e);
+  const double r = (L - n * kLn2Hi) - n * kLn2Lo;
+  mu = sign * exp(r);
+  ex = e;
+}
+
+// XCD-aware decode of the 1-D block index: consecutive physical blocks go round-robin
+// over the 8 XCDs, so all ring chunks of one m are placed on the same XCD (shared L2 for
+// the a_lm column and coefficients) and each XCD sweeps m in increasing order (longest
+// columns first).
+__device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
+  const int P = blockIdx.x;
+  const int xcd = P & 7;
+  const int slot = P >> 3;
+  chunk = slot % A.nchunk;
+  m = (slot / A.nchunk) * 8 + xcd;
+  return m < A.M;
+}
+
+enum { FULL = 0, MASKED = 1, RECONLY = 2 };
+
+__device__ __forceinline__ double2 operator+(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 operator-(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// ---------------------------------------------------------------------------------- //
+// spin 0 synthesis
+// ---------------------------------------------------------------------------------- //
+template <int R, int MODE>
+__device__ __forceinline__ void s0_synth_block(const double* s_al, const double2* s_a, int j0,
+                                               const double (&x)[R], double (&mu)[R],
+                                               double (&mup)[R], const double (&f)[R],
+                                               double2 (&c0)[R], double2 (&c1)[R]) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const double al = s_al[j0 + jj];
+    const double2 a = s_a[j0 + jj];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double c = al * x[r];
+      const double mn = fma(c, mu[r], -mup[r]);
+      mup[r] = mu[r];
+      mu[r] = mn;
+      if (MODE != RECONLY) {
+        const double v = (MODE == MASKED) ? mn * f[r] : mn;
+        double2& acc = (jj & 1) ? c1[r] : c0[r];
+        acc.x = fma(a.x, v, acc.x);
+        acc.y = fma(a.y, v, acc.y);
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void rescale(double (&mu)[R], double (&mup)[R], int (&ex)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ex[r] < 0 && fabs(mu[r]) > kBig) {
+      mu[r] *= kRescale;
+      mup[r] *= kRescale;
+      ex[r] += 1;
+    }
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_synth_s0(RecArgs A, const double2* __restrict__ alm,
+                                                  double2* __restrict__ F) {
+  __shared__ double s_al[CH];
+  __shared__ double2 s_a[CH];
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const double2* ab = alm + (long long)b * A.K;
+  double x[R], mu[R], mup[R], f[R];
+  int ex[R];
+  double2 c0[R], c1[R];
+  const int tb = (chunk * (int)blockDim.x + tid) * R;
+  const double sgn = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    c0[r] = make_double2(0.0, 0.0);
+    c1[r] = make_double2(0.0, 0.0);
+    x[r] = 0.0;
+    mu[r] = 0.0;
+    mup[r] = 0.0;
+    ex[r] = 0;
+    if (t < A.T) {
+      x[r] = A.x[t];
+      const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+      double s;
+      seed_split(L, sgn, s, ex[r]);
+      mup[r] = -s;  // the first step (alpha = 0) yields mu = seed at l = m
+    }
+  }
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      double al = 0.0;
+      double2 a = make_double2(0.0, 0.0);
+      if (l <= lmax) {
+        const long long idx = cb + l;
+        al = (l == m) ? 0.0 : A.alpha[idx];
+        const double g = A.G[idx];
+        const double2 v = ab[idx];
+        a = make_double2(v.x * g, v.y * g);
+      }
+      s_al[tid] = al;
+      s_a[tid] = a;
+    }
+    __syncthreads();
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    for (int j0 = 0; j0 < nst; j0 += 8) {
+      bool deep = false, alldeep = true;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        deep |= ex[r] < 0;
+        alldeep &= ex[r] < 0;
+      }
+      if (__ballot(deep) == 0ull) {
+        s0_synth_block<R, FULL>(s_al, s_a, j0, x, mu, mup, f, c0, c1);
+      } else if (__ballot(!alldeep) == 0ull) {
+        s0_synth_block<R, RECONLY>(s_al, s_a, j0, x, mu, mup, f, c0, c1);
+        rescale<R>(mu, mup, ex);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) f[r] = (ex[r] == 0) ? 1.0 : 0.0;
+        s0_synth_block<R, MASKED>(s_al, s_a, j0, x, mu, mup, f, c0, c1);
+        rescale<R>(mu, mup, ex);
+      }
+    }
+  }
+  double2* Fb = F + (long long)b * A.nrings * A.M;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    if (t < A.T) {
+      Fb[(long long)t * A.M + m] = c0[r] + c1[r];
+      if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * A.M + m] = c0[r] - c1[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// spin 2 synthesis
+// ---------------------------------------------------------------------------------- //
+struct S2Acc {
+  double2 pe, pb, me, mb;  // sum aE G mu+, aB G mu+, aE G mu-, aB G mu-
+};
+
+template <int R, int MODE>
+__device__ __forceinline__ void s2_synth_block(const double2* s_ag, const double2* s_e,
+                                               const double2* s_b, int j0, const double (&x)[R],
+                                               double (&pp)[R], double (&ppp)[R], double (&pm)[R],
+                                               double (&pmp)[R], const double (&fp)[R],
+                                               const double (&fm)[R], S2Acc (&c0)[R],
+                                               S2Acc (&c1)[R]) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const double2 ag = s_ag[j0 + jj];
+    const double2 e = s_e[j0 + jj];
+    const double2 bb = s_b[j0 + jj];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double cp = fma(ag.x, x[r], ag.y);
+      const double cm = fma(ag.x, x[r], -ag.y);
+      const double np = fma(cp, pp[r], -ppp[r]);
+      const double nm = fma(cm, pm[r], -pmp[r]);
+      ppp[r] = pp[r];
+      pp[r] = np;
+      pmp[r] = pm[r];
+      pm[r] = nm;
+      if (MODE != RECONLY) {
+        const double vp = (MODE == MASKED) ? np * fp[r] : np;
+        const double vm = (MODE == MASKED) ? nm * fm[r] : nm;
+        S2Acc& a = (jj & 1) ? c1[r] : c0[r];
+        a.pe.x = fma(e.x, vp, a.pe.x);
+        a.pe.y = fma(e.y, vp, a.pe.y);
+        a.pb.x = fma(bb.x, vp, a.pb.x);
+        a.pb.y = fma(bb.y, vp, a.pb.y);
+        a.me.x = fma(e.x, vm, a.me.x);
+        a.me.y = fma(e.y, vm, a.me.y);
+        a.mb.x = fma(bb.x, vm, a.mb.x);
+        a.mb.y = fma(bb.y, vm, a.mb.y);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, double ls) {
+  // sp = {log_fac, pow_cos, pow_sin, sign}; same operation order as the reference seed
+  double L = sp[0];
+  if (sp[1] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[1], lc));
+  if (sp[2] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[2], ls));
+  return L;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __restrict__ alm,
+                                                  double2* __restrict__ F) {
+  __shared__ double2 s_ag[CH];
+  __shared__ double2 s_e[CH];
+  __shared__ double2 s_b[CH];
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const double2* aE = alm + (long long)b * 2 * A.K;
+  const double2* aB = aE + A.K;
+  double x[R], pp[R], ppp[R], pm[R], pmp[R], fp[R], fm[R];
+  int ep[R], em[R];
+  S2Acc c0[R], c1[R];
+  const int tb = (chunk * (int)blockDim.x + tid) * R;
+  const double* sd = A.seed + 8 * m;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    c0[r].pe = c0[r].pb = c0[r].me = c0[r].mb = make_double2(0.0, 0.0);
+    c1[r] = c0[r];
+    x[r] = 0.0;
+    pp[r] = ppp[r] = pm[r] = pmp[r] = 0.0;
+    ep[r] = em[r] = 0;
+    if (t < A.T && lmin <= lmax) {
+      x[r] = A.x[t];
+      const double lc = A.lcos[t], ls = A.lsin[t];
+      double s;
+      seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep[r]);
+      ppp[r] = -s;
+      seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em[r]);
+      pmp[r] = -s;
+    }
+  }
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      double2 ag = make_double2(0.0, 0.0), e = ag, bb = ag;
+      if (l <= lmax) {
+        const long long idx = cb + l;
+        if (l != lmin) ag = make_double2(A.alpha[idx], A.gamma[idx]);
+        const double g = A.G[idx];
+        const double2 ve = aE[idx], vb = aB[idx];
+        e = make_double2(ve.x * g, ve.y * g);
+        bb = make_double2(vb.x * g, vb.y * g);
+      }
+      s_ag[tid] = ag;
+      s_e[tid] = e;
+      s_b[tid] = bb;
+    }
+    __syncthreads();
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    for (int j0 = 0; j0 < nst; j0 += 8) {
+      bool deep = false, alldeep = true;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        deep |= (ep[r] < 0) | (em[r] < 0);
+        alldeep &= (ep[r] < 0) & (em[r] < 0);
+      }
+      if (__ballot(deep) == 0ull) {
+        s2_synth_block<R, FULL>(s_ag, s_e, s_b, j0, x, pp, ppp, pm, pmp, fp, fm, c0, c1);
+      } else if (__ballot(!alldeep) == 0ull) {
+        s2_synth_block<R, RECONLY>(s_ag, s_e, s_b, j0, x, pp, ppp, pm, pmp, fp, fm, c0, c1);
+        rescale<R>(pp, ppp, ep);
+        rescale<R>(pm, pmp, em);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          fp[r] = (ep[r] == 0) ? 1.0 : 0.0;
+          fm[r] = (em[r] == 0) ? 1.0 : 0.0;
+        }
+        s2_synth_block<R, MASKED>(s_ag, s_e, s_b, j0, x, pp, ppp, pm, pmp, fp, fm, c0, c1);
+        rescale<R>(pp, ppp, ep);
+        rescale<R>(pm, pmp, em);
+      }
+    }
+  }
+  // bp = -(aE + i aB), bm = -(aE - i aB); south values carry (-1)^(l-m)
+  const long long ringM = (long long)A.M;
+  double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+  double2* Fm = Fp + (long long)A.nrings * ringM;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    if (t >= A.T) continue;
+    const double2 PE = c0[r].pe + c1[r].pe, PB = c0[r].pb + c1[r].pb;
+    const double2 ME = c0[r].me + c1[r].me, MB = c0[r].mb + c1[r].mb;
+    // -PE - i PB  and  -ME + i MB
+    Fp[(long long)t * ringM + m] = make_double2(-PE.x + PB.y, -PE.y - PB.x);
+    Fm[(long long)t * ringM + m] = make_double2(-ME.x - MB.y, -ME.y + MB.x);
+    if (t < A.T - 1) {
+      const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+      const double2 dE = c0[r].me - c1[r].me, dB = c0[r].mb - c1[r].mb;  // from mu-
+      const double2 qE = c0[r].pe - c1[r].pe, qB = c0[r].pb - c1[r].pb;  // from mu+
+      Fp[rs] = make_double2(-dE.x + dB.y, -dE.y - dB.x);
+      Fm[rs] = make_double2(-qE.x - qB.y, -qE.y + qB.x);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// adjoint: per-lane partial sums over the lane's rings for a block of l values, then a
+// work-group reduction through LDS into one partial per (ring chunk, l, m).
+// ---------------------------------------------------------------------------------- //
+constexpr int NV = 16;  // double2 partials per lane per block
+
+// LDS: red[(wave*NV + v)*64 + lane], fin[seg*NV + v]
+__device__ __forceinline__ void wg_reduce_store(double2 (&P)[NV], double2* red, double2* fin,
+                                                double2* out0, double2* out1, int ncomp, int nl,
+                                                int tid, int nthr) {
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) red[(wave * NV + v) * 64 + lane] = P[v];
+  __syncthreads();
+  {
+    const int v = tid & (NV - 1);
+    const int seg = tid >> 4;  // 16 lanes per segment
+    const int w = seg >> 2, l0 = (seg & 3) * 16;
+    double2 s = make_double2(0.0, 0.0);
+    const double2* src = red + (w * NV + v) * 64 + l0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s = s + src[i];
+    fin[seg * NV + v] = s;
+  }
+  __syncthreads();
+  if (tid < NV) {
+    const int nseg = nthr >> 4;
+    double2 s = make_double2(0.0, 0.0);
+    for (int g = 0; g < nseg; ++g) s = s + fin[g * NV + tid];
+    // value v = jl * ncomp + comp
+    const int jl = tid / ncomp, comp = tid - jl * ncomp;
+    if (jl < nl) (comp == 0 ? out0 : out1)[jl] = s;
+  }
+}
+
+template <int R, int MODE>
+__device__ __forceinline__ void s0_adj_block(const double* s_al, int j0, const double (&x)[R],
+                                             double (&mu)[R], double (&mup)[R],
+                                             const double (&f)[R], const double2 (&U0)[R],
+                                             const double2 (&U1)[R], double2 (&P)[NV]) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const double al = s_al[j0 + jj];
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double c = al * x[r];
+      const double mn = fma(c, mu[r], -mup[r]);
+      mup[r] = mu[r];
+      mu[r] = mn;
+      if (MODE != RECONLY) {
+        const double v = (MODE == MASKED) ? mn * f[r] : mn;
+        const double2 U = (jj & 1) ? U1[r] : U0[r];
+        acc.x = fma(v, U.x, acc.x);
+        acc.y = fma(v, U.y, acc.y);
+      }
+    }
+    P[(j0 & 8) + jj] = acc;
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __restrict__ V,
+                                                double2* __restrict__ part) {
+  __shared__ double s_al[CH];
+  __shared__ double2 red[4 * NV * 64];
+  __shared__ double2 fin[16 * NV];
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int b = blockIdx.y;
+  const int batch = gridDim.y;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const double2* Vb = V + (long long)b * A.nrings * ringM;
+  double2* pb = part + ((long long)chunk * batch + b) * A.K + cb;
+  double x[R], mu[R], mup[R], f[R];
+  int ex[R];
+  double2 U0[R], U1[R];
+  const int tb = (chunk * nthr + tid) * R;
+  const double sgn = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    x[r] = mu[r] = mup[r] = 0.0;
+    ex[r] = 0;
+    U0[r] = U1[r] = make_double2(0.0, 0.0);
+    if (t < A.T) {
+      x[r] = A.x[t];
+      const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+      double s;
+      seed_split(L, sgn, s, ex[r]);
+      mup[r] = -s;
+      const double2 vn = Vb[(long long)t * ringM + m];
+      double2 vs = make_double2(0.0, 0.0);
+      if (t < A.T - 1) vs = Vb[(long long)(4 * A.nside - 2 - t) * ringM + m];
+      U0[r] = vn + vs;  // (l - m) even
+      U1[r] = vn - vs;  // (l - m) odd
+    }
+  }
+  double2 P[NV];
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    __syncthreads();
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += NV) {
+      const int nl = min(NV, lmax - (l0 + jb) + 1);
+      bool act = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) act |= (ex[r] == 0) && (tb + r < A.T);
+      if (!__syncthreads_or(act)) {
+        s0_adj_block<R, RECONLY>(s_al, jb, x, mu, mup, f, U0, U1, P);
+        rescale<R>(mu, mup, ex);
+        s0_adj_block<R, RECONLY>(s_al, jb + 8, x, mu, mup, f, U0, U1, P);
+        rescale<R>(mu, mup, ex);
+        if (tid < nl) pb[l0 + jb + tid] = make_double2(0.0, 0.0);
+        continue;
+      }
+      bool deep = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) deep |= ex[r] < 0;
+      if (__ballot(deep) == 0ull) {
+        s0_adj_block<R, FULL>(s_al, jb, x, mu, mup, f, U0, U1, P);
+        s0_adj_block<R, FULL>(s_al, jb + 8, x, mu, mup, f, U0, U1, P);
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) f[r] = (ex[r] == 0) ? 1.0 : 0.0;
+          s0_adj_block<R, MASKED>(s_al, jb + 8 * h, x, mu, mup, f, U0, U1, P);
+          rescale<R>(mu, mup, ex);
+        }
+      }
+      wg_reduce_store(P, red, fin, pb + l0 + jb, nullptr, 1, nl, tid, nthr);
+    }
+  }
+}
+
+template <int R, int MODE>
+__device__ __forceinline__ void s2_adj_block(const double2* s_ag, int j0, const double (&x)[R],
+                                             double (&pp)[R], double (&ppp)[R], double (&pm)[R],
+                                             double (&pmp)[R], const double (&fp)[R],
+                                             const double (&fm)[R], const double2 (&Vpn)[R],
+                                             const double2 (&Vps)[R], const double2 (&Vmn)[R],
+                                             const double2 (&Vms)[R], double2 (&P)[NV]) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const double2 ag = s_ag[j0 + jj];
+    double2 a2 = make_double2(0.0, 0.0), b2 = a2;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double cp = fma(ag.x, x[r], ag.y);
+      const double cm = fma(ag.x, x[r], -ag.y);
+      const double np = fma(cp, pp[r], -ppp[r]);
+      const double nm = fma(cm, pm[r], -pmp[r]);
+      ppp[r] = pp[r];
+      pp[r] = np;
+      pmp[r] = pm[r];
+      pm[r] = nm;
+      if (MODE != RECONLY) {
+        const double vp = (MODE == MASKED) ? np * fp[r] : np;
+        const double vm = (MODE == MASKED) ? nm * fm[r] : nm;
+        // p2 += mu+ Vpn + s mu- Vps ; m2 += mu- Vmn + s mu+ Vms ; s = (-1)^(l-m)
+        const double svm = (jj & 1) ? -vm : vm;
+        const double svp = (jj & 1) ? -vp : vp;
+        a2.x = fma(vp, Vpn[r].x, a2.x);
+        a2.y = fma(vp, Vpn[r].y, a2.y);
+        a2.x = fma(svm, Vps[r].x, a2.x);
+        a2.y = fma(svm, Vps[r].y, a2.y);
+        b2.x = fma(vm, Vmn[r].x, b2.x);
+        b2.y = fma(vm, Vmn[r].y, b2.y);
+        b2.x = fma(svp, Vms[r].x, b2.x);
+        b2.y = fma(svp, Vms[r].y, b2.y);
+      }
+    }
+    P[2 * jj] = a2;
+    P[2 * jj + 1] = b2;
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __restrict__ V,
+                                                double2* __restrict__ part) {
+  __shared__ double2 s_ag[CH];
+  __shared__ double2 red[4 * NV * 64];
+  __shared__ double2 fin[16 * NV];
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int b = blockIdx.y;
+  const int batch = gridDim.y;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  if (lmin > lmax) return;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const double2* Vp = V + (long long)b * 2 * A.nrings * ringM;
+  const double2* Vm = Vp + (long long)A.nrings * ringM;
+  double2* p_p2 = part + (((long long)chunk * batch + b) * 2 + 0) * A.K + cb;
+  double2* p_m2 = part + (((long long)chunk * batch + b) * 2 + 1) * A.K + cb;
+  double x[R], pp[R], ppp[R], pm[R], pmp[R], fp[R], fm[R];
+  int ep[R], em[R];
+  double2 Vpn[R], Vps[R], Vmn[R], Vms[R];
+  const int tb = (chunk * nthr + tid) * R;
+  const double* sd = A.seed + 8 * m;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = tb + r;
+    x[r] = pp[r] = ppp[r] = pm[r] = pmp[r] = 0.0;
+    ep[r] = em[r] = 0;
+    Vpn[r] = Vps[r] = Vmn[r] = Vms[r] = make_double2(0.0, 0.0);
+    if (t < A.T) {
+      x[r] = A.x[t];
+      const double lc = A.lcos[t], ls = A.lsin[t];
+      double s;
+      seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep[r]);
+      ppp[r] = -s;
+      seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em[r]);
+      pmp[r] = -s;
+      Vpn[r] = Vp[(long long)t * ringM + m];
+      Vmn[r] = Vm[(long long)t * ringM + m];
+      if (t < A.T - 1) {
+        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        Vps[r] = Vp[rs];
+        Vms[r] = Vm[rs];
+      }
+    }
+  }
+  double2 P[NV];
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      double2 ag = make_double2(0.0, 0.0);
+      if (l <= lmax && l != lmin) ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+      s_ag[tid] = ag;
+    }
+    __syncthreads();
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += 8) {
+      const int nl = min(8, lmax - (l0 + jb) + 1);
+      bool act = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) act |= ((ep[r] == 0) || (em[r] == 0)) && (tb + r < A.T);
+      if (!__syncthreads_or(act)) {
+        s2_adj_block<R, RECONLY>(s_ag, jb, x, pp, ppp, pm, pmp, fp, fm, Vpn, Vps, Vmn, Vms, P);
+        rescale<R>(pp, ppp, ep);
+        rescale<R>(pm, pmp, em);
+        if (tid < nl) {
+          p_p2[l0 + jb + tid] = make_double2(0.0, 0.0);
+          p_m2[l0 + jb + tid] = make_double2(0.0, 0.0);
+        }
+        continue;
+      }
+      bool deep = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) deep |= (ep[r] < 0) | (em[r] < 0);
+      if (__ballot(deep) == 0ull) {
+        s2_adj_block<R, FULL>(s_ag, jb, x, pp, ppp, pm, pmp, fp, fm, Vpn, Vps, Vmn, Vms, P);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          fp[r] = (ep[r] == 0) ? 1.0 : 0.0;
+          fm[r] = (em[r] == 0) ? 1.0 : 0.0;
+        }
+        s2_adj_block<R, MASKED>(s_ag, jb, x, pp, ppp, pm, pmp, fp, fm, Vpn, Vps, Vmn, Vms, P);
+        rescale<R>(pp, ppp, ep);
+        rescale<R>(pm, pmp, em);
+      }
+      wg_reduce_store(P, red, fin, p_p2 + l0 + jb, p_m2 + l0 + jb, 2, nl, tid, nthr);
+    }
+  }
+}
+
+// Sum the ring-chunk partials, apply G_l (and for spin 2 the E/B algebra
+// aE = (-p2 - m2)/2, aB = i (p2 - m2)/2, healpix.py:353-354), write packed a_lm.
+__global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const double2* __restrict__ part,
+                             double2* __restrict__ alm) {
+  const int m = blockIdx.x;
+  const int b = blockIdx.y;
+  const int batch = gridDim.y;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int ncomp = spin ? 2 : 1;
+  for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
+    const long long idx = cb + l;
+    const double g = A.G[idx];
+    if (spin == 0) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int c = 0; c < nchunk; ++c) s = s + part[((long long)c * batch + b) * A.K + idx];
+      alm[(long long)b * A.K + idx] = make_double2(s.x * g, s.y * g);
+    } else {
+      double2* aE = alm + (long long)b * 2 * A.K;
+      double2* aB = aE + A.K;
+      if (l < 2) {
+        aE[idx] = make_double2(0.0, 0.0);
+        aB[idx] = make_double2(0.0, 0.0);
+        continue;
+      }
+      double2 p2 = make_double2(0.0, 0.0), m2 = p2;
+      for (int c = 0; c < nchunk; ++c) {
+        p2 = p2 + part[(((long long)c * batch + b) * ncomp + 0) * A.K + idx];
+        m2 = m2 + part[(((long long)c * batch + b) * ncomp + 1) * A.K + idx];
+      }
+      p2 = make_double2(p2.x * g, p2.y * g);
+      m2 = make_double2(m2.x * g, m2.y * g);
+      aE[idx] = make_double2(0.5 * (-p2.x - m2.x), 0.5 * (-p2.y - m2.y));
+      const double2 d = p2 - m2;
+      aB[idx] = make_double2(-0.5 * d.y, 0.5 * d.x);
+    }
+  }
+}
+
+RecArgs make_args(const jhtx_plan* p) {
+  RecArgs a;
+  a.alpha = p->rec.alpha;
+  a.gamma = p->rec.gamma;
+  a.G = p->rec.G;
+  a.x = p->rec.x;
+  a.lsin = p->rec.lsin;
+  a.lcos = p->rec.lcos;
+  a.seed = p->rec.seed;
+  a.lmax = p->lmax;
+  a.M = p->M;
+  a.T = p->T;
+  a.nside = p->nside;
+  a.nrings = p->nrings;
+  a.nchunk = p->nchunk;
+  a.K = p->K;
+  return a;
+}
+
+}  // namespace
+
+hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+                            hipStream_t s) {
+  const RecArgs a = make_args(p);
+  const int Mpad = (p->M + 7) & ~7;
+  dim3 grid(Mpad * p->nchunk, batch);
+  dim3 block(p->rec_block);
+  if (p->spin == 0) {
+    if (p->rec_R == 4)
+      hipLaunchKernelGGL(k_synth_s0<4>, grid, block, 0, s, a, alm, F);
+    else
+      hipLaunchKernelGGL(k_synth_s0<1>, grid, block, 0, s, a, alm, F);
+  } else {
+    if (p->rec_R == 2)
+      hipLaunchKernelGGL(k_synth_s2<2>, grid, block, 0, s, a, alm, F);
+    else
+      hipLaunchKernelGGL(k_synth_s2<1>, grid, block, 0, s, a, alm, F);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
+                          int batch, hipStream_t s) {
+  const RecArgs a = make_args(p);
+  const int Mpad = (p->M + 7) & ~7;
+  dim3 grid(Mpad * p->nchunk, batch);
+  dim3 block(p->rec_block);
+  if (p->spin == 0) {
+    if (p->rec_R == 4)
+      hipLaunchKernelGGL(k_adj_s0<4>, grid, block, 0, s, a, V, part);
+    else
+      hipLaunchKernelGGL(k_adj_s0<1>, grid, block, 0, s, a, V, part);
+  } else {
+    if (p->rec_R == 2)
+      hipLaunchKernelGGL(k_adj_s2<2>, grid, block, 0, s, a, V, part);
+    else
+      hipLaunchKernelGGL(k_adj_s2<1>, grid, block, 0, s, a, V, part);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, p->nchunk,
+                     part, alm);
+  return hipGetLastError();
+}
+
+}  // namespace jhtx
