@@ -74,6 +74,22 @@ void jhtx_plan_destroy(jhtx_plan* plan);
 /* Description of the last failure on the calling thread ("" if none). */
 const char* jhtx_last_error(void);
 
+/* Stage timing (no counterpart in the reference; for benchmarks and profilers): when
+ * enabled, every call brackets each kernel stage with HIP events on the call's stream.
+ * Stages: 0 synthesis recursion, 1 synthesis ring FFT, 2 adjoint ring FFT,
+ * 3 adjoint recursion (+ chunk reduction).  jhtx_plan_read_profile synchronises the
+ * recorded events, writes the summed milliseconds and launch counts per stage (arrays of
+ * 4) and clears them.  Up to 4096 calls are recorded between reads. */
+int jhtx_plan_set_profiling(jhtx_plan* plan, int enable);
+int jhtx_plan_read_profile(jhtx_plan* plan, double* ms, int* counts);
+
+/* Host-side recursion tables, computed without touching a device (no counterpart in the
+ * reference; for validating the plan tables on a CPU).  Array lengths: x, lsin, lcos
+ * 2 nside (lcos spin 2 only); alpha, G, gamma K (gamma spin 2 only); seed lmax+1 (spin 0)
+ * or 8 (lmax+1) (spin 2).  Any pointer may be NULL to skip that table. */
+int jhtx_host_rec_tables(int nside, int lmax, int spin, double* x, double* lsin, double* lcos,
+                         double* alpha, double* gamma, double* G, double* seed);
+
 /* Plan introspection: 0 nside, 1 lmax, 2 spin, 3 precision, 4 max_batch, 5 ncomp,
  * 6 nrings, 7 npix (low 31 bits), 8 number of ring-FFT launch classes. -1 if unknown. */
 int jhtx_plan_query(const jhtx_plan* plan, int what);

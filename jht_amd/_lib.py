@@ -20,6 +20,9 @@ EXPORTS = (
     "jhtx_plan_destroy",
     "jhtx_last_error",
     "jhtx_plan_query",
+    "jhtx_plan_set_profiling",
+    "jhtx_plan_read_profile",
+    "jhtx_host_rec_tables",
 )
 
 JHTX_OK = 0
@@ -70,6 +73,13 @@ def load() -> ctypes.CDLL:
     lib.jhtx_last_error.restype = ctypes.c_char_p
     lib.jhtx_plan_query.argtypes = [vp, ctypes.c_int]
     lib.jhtx_plan_query.restype = ctypes.c_int
+    lib.jhtx_plan_set_profiling.argtypes = [vp, ctypes.c_int]
+    lib.jhtx_plan_set_profiling.restype = ctypes.c_int
+    lib.jhtx_plan_read_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    lib.jhtx_plan_read_profile.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.jhtx_host_rec_tables.argtypes = [ctypes.c_int] * 3 + [dp] * 7
+    lib.jhtx_host_rec_tables.restype = ctypes.c_int
     _LIB = lib
     return lib
 

@@ -61,19 +61,32 @@ void wigner_seed_params(int M, int Mp, int lmin, double out[4]) {
   out[3] = (k0 % 2) ? -1.0 : 1.0;
 }
 
-int build_rec_tables(jhtx_plan* p) {
-  const int lmax = p->lmax, M = p->M, T = p->T, nside = p->nside;
-  const long long K = p->K;
-  std::vector<double> x(T), lsin(T), lcos;
+// Host tables of the recursion (no device calls; also exported for CPU validation).
+struct HostRec {
+  std::vector<double> x, lsin, lcos, alpha, gamma, G, seed;
+};
+
+void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
+  const int M = lmax + 1, T = 2 * nside;
+  const long long K = (long long)(lmax + 1) * (lmax + 2) / 2;
+  std::vector<double>& x = h.x;
+  std::vector<double>& lsin = h.lsin;
+  std::vector<double>& lcos = h.lcos;
+  std::vector<double>& alpha = h.alpha;
+  std::vector<double>& gamma = h.gamma;
+  std::vector<double>& G = h.G;
+  std::vector<double>& seed = h.seed;
+  x.assign(T, 0.0);
+  lsin.assign(T, 0.0);
   for (int t = 0; t < T; ++t) {
     const double z = ring_z(nside, t);
     x[t] = z;
-    if (p->spin == 0) {
+    if (spin == 0) {
       const double s2 = std::min(std::max(1.0 - z * z, 0.0), 1.0);
       lsin[t] = std::log(std::sqrt(s2));
     }
   }
-  if (p->spin == 2) {
+  if (spin == 2) {
     lcos.resize(T);
     for (int t = 0; t < T; ++t) {
       const double z = x[t];
@@ -83,10 +96,11 @@ int build_rec_tables(jhtx_plan* p) {
       lsin[t] = std::log(sh);
     }
   }
-  std::vector<double> alpha(K, 0.0), G(K, 0.0), gamma, seed;
-  if (p->spin == 2) gamma.assign(K, 0.0);
+  alpha.assign(K, 0.0);
+  G.assign(K, 0.0);
+  if (spin == 2) gamma.assign(K, 0.0);
   const long double four_pi = 4.0L * 3.14159265358979323846264338327950288L;
-  if (p->spin == 0) {
+  if (spin == 0) {
     seed.resize(M);
     long double acc = 0.0L;  // sum_{k<=m} log((2k+1)/(2k))
     for (int m = 0; m < M; ++m) {
@@ -151,14 +165,19 @@ int build_rec_tables(jhtx_plan* p) {
       }
     }
   }
+}
+
+int build_rec_tables(jhtx_plan* p) {
+  HostRec h;
+  host_rec_tables(p->nside, p->lmax, p->spin, h);
   int rc;
-  if ((rc = upload(&p->rec.alpha, alpha))) return rc;
-  if ((rc = upload(&p->rec.G, G))) return rc;
-  if ((rc = upload(&p->rec.gamma, gamma))) return rc;
-  if ((rc = upload(&p->rec.x, x))) return rc;
-  if ((rc = upload(&p->rec.lsin, lsin))) return rc;
-  if ((rc = upload(&p->rec.lcos, lcos))) return rc;
-  if ((rc = upload(&p->rec.seed, seed))) return rc;
+  if ((rc = upload(&p->rec.alpha, h.alpha))) return rc;
+  if ((rc = upload(&p->rec.G, h.G))) return rc;
+  if ((rc = upload(&p->rec.gamma, h.gamma))) return rc;
+  if ((rc = upload(&p->rec.x, h.x))) return rc;
+  if ((rc = upload(&p->rec.lsin, h.lsin))) return rc;
+  if ((rc = upload(&p->rec.lcos, h.lcos))) return rc;
+  if ((rc = upload(&p->rec.seed, h.seed))) return rc;
   return JHTX_OK;
 }
 
@@ -169,6 +188,47 @@ void free_rec_tables(jhtx_plan* p) {
     if (q) hipFree(q);
   p->rec = RecTables();
 }
+
+struct Profile {
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+
+// Bracket one stage with events when profiling is on.
+struct StageTimer {
+  Profile* pr;
+  hipStream_t s;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(const jhtx_plan* p, hipStream_t s_, int st) : pr(nullptr), s(s_), stage(st) {
+    if (p->profiling && p->prof && ((Profile*)p->prof)->recs.size() < 4 * 4096) {
+      pr = (Profile*)p->prof;
+      a = pr->get();
+      (void)hipEventRecord(a, s);
+    }
+  }
+  void done() {
+    if (!pr) return;
+    hipEvent_t b = pr->get();
+    (void)hipEventRecord(b, s);
+    pr->recs.push_back({stage, a, b});
+    pr = nullptr;
+  }
+};
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
@@ -260,8 +320,14 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
   if (rc) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   double2* F = reinterpret_cast<double2*>(ws);
+  StageTimer t0(p, s, 0);
   hipError_t e = launch_synth_rec(p, reinterpret_cast<const double2*>(alm), F, batch, s);
-  if (e == hipSuccess) e = launch_ring_synth(p, F, reinterpret_cast<double*>(map), batch, s);
+  t0.done();
+  if (e == hipSuccess) {
+    StageTimer t1(p, s, 1);
+    e = launch_ring_synth(p, F, reinterpret_cast<double*>(map), batch, s);
+    t1.done();
+  }
   if (e != hipSuccess) {
     set_error(std::string("synthesis launch failed: ") + hipGetErrorString(e));
     return JHTX_EDEVICE;
@@ -275,9 +341,14 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   double2* V = reinterpret_cast<double2*>(ws);
   double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(ws) + p->ws_F_bytes);
+  StageTimer t2(p, s, 2);
   hipError_t e = launch_ring_adjoint(p, reinterpret_cast<const double*>(map), V, batch, s);
-  if (e == hipSuccess)
+  t2.done();
+  if (e == hipSuccess) {
+    StageTimer t3(p, s, 3);
     e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
+    t3.done();
+  }
   if (e != hipSuccess) {
     set_error(std::string("adjoint launch failed: ") + hipGetErrorString(e));
     return JHTX_EDEVICE;
@@ -285,8 +356,75 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   return JHTX_OK;
 }
 
+int jhtx_host_rec_tables(int nside, int lmax, int spin, double* x, double* lsin, double* lcos,
+                         double* alpha, double* gamma, double* G, double* seed) {
+  if (nside < 1 || lmax < 0 || (spin != 0 && spin != 2)) {
+    set_error("invalid nside/lmax/spin");
+    return JHTX_EINVAL;
+  }
+  HostRec h;
+  host_rec_tables(nside, lmax, spin, h);
+  auto put = [](double* dst, const std::vector<double>& v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+  };
+  put(x, h.x);
+  put(lsin, h.lsin);
+  put(lcos, h.lcos);
+  put(alpha, h.alpha);
+  put(gamma, h.gamma);
+  put(G, h.G);
+  put(seed, h.seed);
+  return JHTX_OK;
+}
+
+int jhtx_plan_set_profiling(jhtx_plan* p, int enable) {
+  if (!p) {
+    set_error("null plan");
+    return JHTX_EINVAL;
+  }
+  if (enable && !p->prof) p->prof = new Profile();
+  p->profiling = enable ? 1 : 0;
+  return JHTX_OK;
+}
+
+int jhtx_plan_read_profile(jhtx_plan* p, double* ms, int* counts) {
+  if (!p || !ms || !counts) {
+    set_error("null argument");
+    return JHTX_EINVAL;
+  }
+  for (int i = 0; i < 4; ++i) {
+    ms[i] = 0.0;
+    counts[i] = 0;
+  }
+  Profile* pr = (Profile*)p->prof;
+  if (!pr) return JHTX_OK;
+  for (auto& r : pr->recs) {
+    if (hipEventSynchronize(r.b) != hipSuccess) {
+      set_error("hipEventSynchronize failed");
+      return JHTX_EDEVICE;
+    }
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, r.a, r.b);
+    ms[r.stage] += t;
+    counts[r.stage] += 1;
+    pr->pool.push_back(r.a);
+    pr->pool.push_back(r.b);
+  }
+  pr->recs.clear();
+  return JHTX_OK;
+}
+
 void jhtx_plan_destroy(jhtx_plan* p) {
   if (!p) return;
+  if (p->prof) {
+    Profile* pr = (Profile*)p->prof;
+    for (auto& r : pr->recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pr->pool) (void)hipEventDestroy(e);
+    delete pr;
+  }
   free_rec_tables(p);
   free_fft_tables(p);
   delete p;

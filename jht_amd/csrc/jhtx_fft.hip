@@ -220,7 +220,7 @@ __device__ int ring_dft(const FftArgs& A, int N, int kind, int L, int s, long lo
       out[j] = acc;
     }
     __syncthreads();
-    return 2 * N;
+    return 2 * N;  // double2 offset of out
   }
   const int lgL = ilog2(L);
   const int nt = L >> 4;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
     ring_fold(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, true, false, l2, scratch, tid, BLOCK);
     if (pair) ring_fold(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, true, true, l2, scratch, tid, BLOCK);
     const int o = ring_dft(A, N, kind, L, +1, bofs, lds, tid, BLOCK);
-    const double2* out = l2 + o / 2;
+    const double2* out = l2 + o;
     double* mb = map + (long long)b * A.npix;
     const long long sn = ring_start(A.nside, rn);
     const long long ss = ring_start(A.nside, rs);
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
       const int r = h ? rs : rn;
       ring_fold(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
       const int o = ring_dft(A, N, kind, L, +1, bofs, lds, tid, BLOCK);
-      const double2* out = l2 + o / 2;
+      const double2* out = l2 + o;
       const long long st = ring_start(A.nside, r);
       for (int j = tid; j < N; j += BLOCK) {
         const double2 z = out[j];
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
     for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(mb[sn + j], pair ? mb[ss + j] : 0.0);
     __syncthreads();
     const int o = ring_dft(A, N, kind, L, -1, bofs, lds, tid, BLOCK);
-    const double2* W = l2 + o / 2;
+    const double2* W = l2 + o;
     double2* Vn = V + ((long long)b * A.nrings + rn) * ringM;
     double2* Vs = V + ((long long)b * A.nrings + rs) * ringM;
     for (int m = tid; m < A.M; m += BLOCK) {
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
       for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(Q[st + j], U[st + j]);
       __syncthreads();
       const int o = ring_dft(A, N, kind, L, -1, bofs, lds, tid, BLOCK);
-      const double2* W = l2 + o / 2;
+      const double2* W = l2 + o;
       double2* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
       double2* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
       for (int m = tid; m < A.M; m += BLOCK) {

@@ -75,6 +75,9 @@ struct jhtx_plan {
   int rec_R;      // rings per lane
   jhtx::RecTables rec;
   jhtx::FftTables fft;
+  // stage timing (jhtx_plan_set_profiling)
+  int profiling;
+  void* prof;  // jhtx::Profile*
   // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2][bpart: nchunk*batch*ncomp*K double2]
   size_t ws_F_bytes, ws_part_bytes, ws_bytes;
 };

@@ -327,6 +327,11 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
   for (int r = 0; r < R; ++r) {
     const int t = tb + r;
     if (t >= A.T) continue;
+    if ((lmin - m) & 1) {  // classes are counted from lmin; the reflection needs (l - m)
+      const S2Acc tmp = c0[r];
+      c0[r] = c1[r];
+      c1[r] = tmp;
+    }
     const double2 PE = c0[r].pe + c1[r].pe, PB = c0[r].pb + c1[r].pb;
     const double2 ME = c0[r].me + c1[r].me, MB = c0[r].mb + c1[r].mb;
     // -PE - i PB  and  -ME + i MB
@@ -570,9 +575,11 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
       Vpn[r] = Vp[(long long)t * ringM + m];
       Vmn[r] = Vm[(long long)t * ringM + m];
       if (t < A.T - 1) {
+        // (-1)^(l-m) = (-1)^(l-lmin) (-1)^(lmin-m): fold the second factor into the data
+        const double sg = ((lmin - m) & 1) ? -1.0 : 1.0;
         const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
-        Vps[r] = Vp[rs];
-        Vms[r] = Vm[rs];
+        Vps[r] = make_double2(sg * Vp[rs].x, sg * Vp[rs].y);
+        Vms[r] = make_double2(sg * Vm[rs].x, sg * Vm[rs].y);
       }
     }
   }

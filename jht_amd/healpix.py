@@ -160,6 +160,18 @@ class Plan:
             self._lib.jhtx_plan_destroy(h)
             self._h = None
 
+    STAGES = ("synth_rec", "synth_fft", "adj_fft", "adj_rec")
+
+    def set_profiling(self, on: bool) -> None:
+        _lib.check(self._lib.jhtx_plan_set_profiling(self._h, 1 if on else 0))
+
+    def read_profile(self) -> dict:
+        """{stage: (summed ms, launches)} of the calls since the last read (HIP events)."""
+        ms = (_lib.ctypes.c_double * 4)()
+        cnt = (_lib.ctypes.c_int * 4)()
+        _lib.check(self._lib.jhtx_plan_read_profile(self._h, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
+
     def synth(self, alm: torch.Tensor, out: torch.Tensor, batch: int) -> torch.Tensor:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self._lib.jhtx_synth(self._h, alm.data_ptr(), out.data_ptr(), batch,
