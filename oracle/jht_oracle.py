@@ -757,3 +757,74 @@ def random_alm(lmax: int, rng, lmin: int = 0) -> np.ndarray:
     ell = alm_l_of_index(lmax)
     a[ell < lmin] = 0.0
     return a.astype(np.complex128)
+
+
+# --------------------------------------------------------------------------- #
+# row-sampled masked scan (CPU baseline timing) -- the reference's _rec_step
+# (_recursion.py:352-369) over every l for a subset of orders m, exactly as the fused
+# lax.scan runs it (inactive steps masked, not skipped), spin 2 (_recursion.py:512-572).
+# --------------------------------------------------------------------------- #
+
+
+class _RowTables:
+    def __init__(self, x, spin, lmax, ms):
+        ms = np.asarray(ms, dtype=np.int64)
+        L1, R = lmax + 1, ms.size
+        self.A = np.zeros((L1, R))
+        self.B = np.zeros((L1, R))
+        self.C = np.zeros((L1, R))
+        self.seed_sign = np.zeros(R)
+        self.seed_log = np.zeros((R, x.size))
+        self.lmin = np.maximum(ms, abs(spin))
+        for i, m in enumerate(ms):
+            lm = int(self.lmin[i])
+            if lm < lmax:
+                Ac, Cc, Bc = wigner_recurrence_coeffs(-int(m), spin, lm, lmax)
+                self.A[lm + 1:, i], self.C[lm + 1:, i], self.B[lm + 1:, i] = Ac, Cc, Bc
+            self.seed_sign[i], self.seed_log[i] = wigner_seed(x, -int(m), spin, lm)
+        l = np.arange(L1, dtype=np.float64)
+        self.pref = ((-1.0) ** ms)[None, :] * np.sqrt((2.0 * l + 1.0) / FOUR_PI)[:, None]
+
+
+def _masked_step(tb, l, x, p, q, s):
+    A = tb.A[l][:, None]
+    B = tb.B[l][:, None]
+    C = tb.C[l][:, None]
+    raw = (A * x[None, :] + C) * p - B * q
+    rp, rq, rs = _renorm_step(raw, p, s)
+    seed = (tb.lmin == l)[:, None]
+    rec = (tb.lmin < l)[:, None]
+    act = (tb.lmin <= l)[:, None]
+    p2 = np.where(seed, tb.seed_sign[:, None], np.where(rec, rp, p))
+    q2 = np.where(seed, 0.0, np.where(rec, rq, q))
+    s2 = np.where(seed, tb.seed_log, np.where(rec, rs, s))
+    lam = np.where(act, p2 * np.exp(s2), 0.0)
+    return p2, q2, s2, lam
+
+
+def sampled_spin2_pair(x, lmax, ms, bp, bm, Vpn, Sp, Vmn, Sm):
+    """Synthesis and adjoint contractions of the spin-2 fused scans restricted to the
+    orders ``ms`` (bp, bm: (R, L1); V*: (R, T)).  Returns (FpN, FpS, FmN, FmS, p2, m2)."""
+    x = np.asarray(x, dtype=np.float64)
+    tp, tm = _RowTables(x, 2, lmax, ms), _RowTables(x, -2, lmax, ms)
+    R, T = len(ms), x.size
+    z = lambda: np.zeros((R, T))  # noqa: E731
+    pp, qp, sp_, pm, qm, sm = z(), z(), z(), z(), z(), z()
+    F = [np.zeros((R, T), complex) for _ in range(4)]
+    p2 = np.zeros((R, lmax + 1), complex)
+    m2 = np.zeros((R, lmax + 1), complex)
+    with np.errstate(under="ignore", divide="ignore", invalid="ignore"):
+        for l in range(lmax + 1):
+            pp, qp, sp_, lp = _masked_step(tp, l, x, pp, qp, sp_)
+            pm, qm, sm, lmn = _masked_step(tm, l, x, pm, qm, sm)
+            par = -1.0 if l % 2 else 1.0
+            pr = tp.pref[l]
+            cp = (bp[:, l] * pr)[:, None]
+            cm = (bm[:, l] * pr)[:, None]
+            F[0] += cp * lp
+            F[1] += par * cp * lmn
+            F[2] += cm * lmn
+            F[3] += par * cm * lp
+            p2[:, l] = pr * (np.einsum("rt,rt->r", lp, Vpn) + par * np.einsum("rt,rt->r", lmn, Sp))
+            m2[:, l] = pr * (np.einsum("rt,rt->r", lmn, Vmn) + par * np.einsum("rt,rt->r", lp, Sm))
+    return (*F, p2, m2)
