@@ -9,6 +9,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 #include "jhtx_internal.h"
 
@@ -290,16 +291,42 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
   p->ncomp = spin ? 2 : 1;
   p->npix = 12LL * nside * nside;
   p->K = (long long)(lmax + 1) * (lmax + 2) / 2;
-  // rings per lane and work-group size of the recursion kernels
-  if (spin == 0)
-    p->rec_R = (p->T >= 512) ? 4 : 1;
-  else
-    p->rec_R = (p->T >= 256) ? 2 : 1;
-  const int lanes = (p->T + p->rec_R - 1) / p->rec_R;
-  p->rec_block = std::min(256, std::max(64, ((lanes + 63) / 64) * 64));
-  p->nchunk = (p->T + p->rec_block * p->rec_R - 1) / (p->rec_block * p->rec_R);
+  // rings per lane and work-group size of the recursion kernels; small grids use one
+  // ring per lane.  JHTX_SYNTH_R / JHTX_ADJ_R override the choice (tuning sweeps).
+  auto pick = [&](const char* env, int big, int small, int thresh, const int* allowed) {
+    int r = (p->T >= thresh) ? big : small;
+    if (const char* v = getenv(env)) {
+      const int want = atoi(v);
+      for (int i = 0; allowed[i]; ++i)
+        if (allowed[i] == want) r = want;
+    }
+    return r;
+  };
+  static const int s0_R[] = {1, 4, 8, 0};
+  static const int s2_R[] = {1, 2, 4, 0};
+  // defaults from sweeps at nside 2048 / lmax 4096 (synthesis: occupancy wins; adjoint:
+  // rings per lane amortise the per-lane ring reduction)
+  if (spin == 0) {
+    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 512, s0_R);
+    p->adj_R = pick("JHTX_ADJ_R", 4, 1, 1024, s0_R);
+  } else {
+    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
+    p->adj_R = pick("JHTX_ADJ_R", 2, 1, 512, s2_R);
+  }
+  int block_cap = 256;
+  if (const char* v = getenv("JHTX_REC_BLOCK")) {
+    const int b = atoi(v);
+    if (b == 64 || b == 128 || b == 192 || b == 256) block_cap = b;
+  }
+  auto shape = [&](int R, int& block, int& nchunk) {
+    const int lanes = (p->T + R - 1) / R;
+    block = std::min(block_cap, std::max(64, ((lanes + 63) / 64) * 64));
+    nchunk = (p->T + block * R - 1) / (block * R);
+  };
+  shape(p->rec_R, p->rec_block, p->nchunk);
+  shape(p->adj_R, p->adj_block, p->adj_nchunk);
   p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
-  p->ws_part_bytes = align256(sizeof(double2) * (size_t)p->nchunk * max_batch * p->ncomp * p->K);
+  p->ws_part_bytes = align256(sizeof(double2) * (size_t)p->adj_nchunk * max_batch * p->ncomp * p->K);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
   int rc = build_rec_tables(p);
   if (rc == JHTX_OK) rc = build_fft_tables(p);

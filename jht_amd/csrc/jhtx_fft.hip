@@ -34,7 +34,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-// e^{i pi t}
+// e^{i pi t} (only used in loops that are not unrolled: register pressure)
 __device__ __forceinline__ double2 cispi(double t) {
   double s, c;
   sincospi(t, &s, &c);
@@ -55,6 +55,8 @@ struct FftArgs {
   const int* pairs;
   const long long* bofs;  // per pair p: offset of its Bluestein kernel spectrum, -1 if none
   const double2* bspec;
+  const long long* cofs;  // per pair p: offset of its chirp table c_n = e^{i pi n^2/N}, n < N
+  const double2* chirps;
   int lds_doubles;
   int bluestein_all;
 };
@@ -82,12 +84,12 @@ __host__ __device__ inline int ilog2(int v) {
 }
 
 // --------------------------------------------------------------------------------- //
-// power-of-two FFT, cyclic register layout: x[e] holds element tid + nt*e (nt = L/16)
+// power-of-two FFT, cyclic register layout: x[e] holds element tid + nt*e (nt = L/E)
 // --------------------------------------------------------------------------------- //
-template <int R>
-__device__ __forceinline__ void fft_pass(double2 (&x)[16], int tid, int nt, int Ns, int dir,
+template <int R, int E>
+__device__ __forceinline__ void fft_pass(double2 (&x)[E], int tid, int nt, int Ns, int dir,
                                          const double2* __restrict__ tw, int twL) {
-  constexpr int NB = 16 / R;
+  constexpr int NB = E / R;
   if (tid >= nt) return;
   const int step = twL / (Ns * R);
 #pragma unroll
@@ -98,11 +100,14 @@ __device__ __forceinline__ void fft_pass(double2 (&x)[16], int tid, int nt, int 
 #pragma unroll
     for (int t = 0; t < R; ++t) v[t] = x[bb + t * NB];
     if (Ns > 1) {
-#pragma unroll
-      for (int t = 1; t < R; ++t) {
-        double2 w = tw[t * k * step];
-        if (dir > 0) w.y = -w.y;
-        v[t] = cmul(v[t], w);
+      // one table load per butterfly; w^2, w^3 by complex products (<= 2 ulp)
+      double2 w = tw[k * step];
+      if (dir > 0) w.y = -w.y;
+      v[1] = cmul(v[1], w);
+      if (R == 4) {
+        const double2 w2 = cmul(w, w);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], cmul(w2, w));
       }
     }
     if (R == 2) {
@@ -122,10 +127,10 @@ __device__ __forceinline__ void fft_pass(double2 (&x)[16], int tid, int nt, int 
 }
 
 // move pass outputs (at Stockham positions) back to the cyclic layout through LDS
-template <int R>
-__device__ __forceinline__ void fft_exchange(double2 (&x)[16], int tid, int nt, int Ns, int lgNs,
+template <int R, int E>
+__device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, int Ns, int lgNs,
                                              double* lds, bool split) {
-  constexpr int NB = 16 / R;
+  constexpr int NB = E / R;
   constexpr int lgR = (R == 2) ? 1 : 2;
   if (!split) {
     double2* l2 = reinterpret_cast<double2*>(lds);
@@ -141,7 +146,7 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[16], int tid, int nt, 
     __syncthreads();
     if (tid < nt) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) x[e] = l2[tid + nt * e];
+      for (int e = 0; e < E; ++e) x[e] = l2[tid + nt * e];
     }
     __syncthreads();
   } else {
@@ -159,7 +164,7 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[16], int tid, int nt, 
       __syncthreads();
       if (tid < nt) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
+        for (int e = 0; e < E; ++e) {
           if (part)
             x[e].y = lds[tid + nt * e];
           else
@@ -171,40 +176,65 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[16], int tid, int nt, 
   }
 }
 
-// In-place FFT of length L = 2^lgL >= 16; dir -1 forward (e^{-2 pi i}), +1 backward.
-// Starts with a barrier so LDS may still hold the input the caller loaded from.
-__device__ void fft_pow2(double2 (&x)[16], int lgL, int tid, int dir, const double2* tw, int twL,
-                         double* lds, int lds_doubles) {
-  const int L = 1 << lgL;
-  const int nt = L >> 4;
-  const bool split = (2 * L > lds_doubles);
-  __syncthreads();
-  int Ns = 1, lgNs = 0, rem = lgL;
-  while (rem > 0) {
-    if (rem & 1) {
-      fft_pass<2>(x, tid, nt, Ns, dir, tw, twL);
-      rem -= 1;
-      if (rem) fft_exchange<2>(x, tid, nt, Ns, lgNs, lds, split);
-      Ns <<= 1;
-      lgNs += 1;
-    } else {
-      fft_pass<4>(x, tid, nt, Ns, dir, tw, twL);
-      rem -= 2;
-      if (rem) fft_exchange<4>(x, tid, nt, Ns, lgNs, lds, split);
-      Ns <<= 2;
-      lgNs += 2;
+// Compile-time pass sequence for L = 2^LG: a radix-2 pass first when LG is odd, then
+// radix-4 passes; every pass but the last ends with an exchange through LDS.
+template <int E, int LG, int LGNS>
+__device__ __forceinline__ void fft_passes(double2 (&x)[E], int tid, int dir, const double2* tw,
+                                           int twL, double* lds, bool split) {
+  constexpr int REM = LG - LGNS;
+  if constexpr (REM > 0) {
+    constexpr int R = (REM & 1) ? 2 : 4;
+    constexpr int lgR = (R == 2) ? 1 : 2;
+    constexpr int nt = (1 << LG) / E;
+    constexpr int Ns = 1 << LGNS;
+    fft_pass<R, E>(x, tid, nt, Ns, dir, tw, twL);
+    if constexpr (REM > lgR) {
+      fft_exchange<R, E>(x, tid, nt, Ns, LGNS, lds, split);
+      fft_passes<E, LG, LGNS + lgR>(x, tid, dir, tw, twL, lds, split);
     }
   }
 }
+
+// In-place FFT of length L = 2^lgL with LGLO <= lgL <= LGHI (the range of one launch
+// class); dir -1 forward (e^{-2 pi i}), +1 backward.  Starts with a barrier so LDS may
+// still hold the input the caller loaded from.
+template <int E, int LG, int LGHI>
+__device__ __forceinline__ void fft_dispatch(double2 (&x)[E], int lgL, int tid, int dir,
+                                             const double2* tw, int twL, double* lds, bool split) {
+  if constexpr (LG < LGHI) {
+    if (lgL != LG) {
+      fft_dispatch<E, LG + 1, LGHI>(x, lgL, tid, dir, tw, twL, lds, split);
+      return;
+    }
+  }
+  fft_passes<E, LG, 0>(x, tid, dir, tw, twL, lds, split);
+}
+
+template <int E, int LGLO, int LGHI>
+__device__ __forceinline__ void fft_pow2(double2 (&x)[E], int lgL, int tid, int dir,
+                                         const double2* tw, int twL, double* lds, int lds_doubles) {
+  const bool split = (2 * (1 << lgL) > lds_doubles);
+  __syncthreads();
+  fft_dispatch<E, LGLO, LGHI>(x, lgL, tid, dir, tw, twL, lds, split);
+}
+
+// transform-length range of the launch class (threads, elements per thread)
+constexpr int lg_hi(int block, int e) {
+  int l = 0;
+  while ((1 << l) < block * e) ++l;
+  return l;
+}
+constexpr int lg_lo(int block, int e) { return block == 64 ? 6 : (block == 256 ? lg_hi(block, e) - 1 : lg_hi(block, e)); }
 
 // --------------------------------------------------------------------------------- //
 // length-N DFT  X_j = sum_k in_k e^{s 2 pi i j k / N}, in = LDS double2[N] at offset 0.
 // Returns the double2 offset of the output (N values) in LDS.  Ends with a barrier.
 // --------------------------------------------------------------------------------- //
-__device__ int ring_dft(const FftArgs& A, int N, int kind, int L, int s, long long bofs,
-                        double* lds, int tid, int nthr) {
+template <int E, int LGLO, int LGHI, int KIND>
+__device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, long long bofs,
+                        const double2* __restrict__ ch, double* lds, int tid, int nthr) {
   double2* l2 = reinterpret_cast<double2*>(lds);
-  if (kind == 0) {
+  if constexpr (KIND == 0) {
     double2* roots = l2 + N;
     double2* out = l2 + 2 * N;
     for (int q = tid; q < N; q += nthr) roots[q] = cispi((double)(2 * s * q) / (double)N);
@@ -223,17 +253,17 @@ __device__ int ring_dft(const FftArgs& A, int N, int kind, int L, int s, long lo
     return 2 * N;  // double2 offset of out
   }
   const int lgL = ilog2(L);
-  const int nt = L >> 4;
-  double2 x[16];
-  if (kind == 1) {
+  const int nt = L / E;
+  double2 x[E];
+  if constexpr (KIND == 1) {
     if (tid < nt) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) x[e] = l2[tid + nt * e];
+      for (int e = 0; e < E; ++e) x[e] = l2[tid + nt * e];
     }
-    fft_pow2(x, lgL, tid, s < 0 ? -1 : 1, A.tw, A.twL, lds, A.lds_doubles);
+    fft_pow2<E, LGLO, LGHI>(x, lgL, tid, s < 0 ? -1 : 1, A.tw, A.twL, lds, A.lds_doubles);
     if (tid < nt) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) l2[tid + nt * e] = x[e];
+      for (int e = 0; e < E; ++e) l2[tid + nt * e] = x[e];
     }
     __syncthreads();
     return 0;
@@ -241,27 +271,41 @@ __device__ int ring_dft(const FftArgs& A, int N, int kind, int L, int s, long lo
   // Bluestein: a_k = in_k c_k, b_n = conj(c_n), c_n = e^{s i pi n^2/N}; X_j = c_j (a * b)_j
   if (tid < nt) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
-      x[e] = (i < N) ? cmul(l2[i], chirp(i, N, s)) : make_double2(0.0, 0.0);
+      if (i < N) {
+        const double2 c = ch[i];
+        x[e] = cmul(l2[i], s > 0 ? c : cconj(c));
+      } else {
+        x[e] = make_double2(0.0, 0.0);
+      }
+      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  fft_pow2(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_doubles);
-  if (tid < nt) {
-    const double2* B = A.bspec + bofs;  // spectrum of e^{-i pi n^2/N}; conj for s < 0
+  // forward FFT, pointwise product with the kernel spectrum, backward FFT: one call site
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    fft_pow2<E, LGLO, LGHI>(x, lgL, tid, pass ? +1 : -1, A.tw, A.twL, lds, A.lds_doubles);
+    if (pass == 0 && tid < nt) {
+      const double2* B = A.bspec + bofs;  // spectrum of e^{-i pi n^2/N}; conj for s < 0
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const double2 bv = B[tid + nt * e];
-      x[e] = cmul(x[e], s > 0 ? bv : cconj(bv));
+      for (int e = 0; e < E; ++e) {
+        const double2 bv = B[tid + nt * e];
+        x[e] = cmul(x[e], s > 0 ? bv : cconj(bv));
+        if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bound in-flight loads
+      }
     }
   }
-  fft_pow2(x, lgL, tid, +1, A.tw, A.twL, lds, A.lds_doubles);
   const double invL = 1.0 / (double)L;
   if (tid < nt) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
-      if (i < N) l2[i] = cmul(cscale(x[e], invL), chirp(i, N, s));
+      if (i < N) {
+        const double2 c = ch[i];
+        l2[i] = cmul(cscale(x[e], invL), s > 0 ? c : cconj(c));
+      }
+      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
   __syncthreads();
@@ -341,9 +385,12 @@ __device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restr
   __syncthreads();
 }
 
-template <int BLOCK>
+// Work-group per mirror ring pair; template on the transform kind and spin so each
+// instantiation holds one code path (register pressure).
+template <int BLOCK, int E, int SPIN, int KIND>
 __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* __restrict__ F,
                                                       double* __restrict__ map) {
+  constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
   double2* l2 = reinterpret_cast<double2*>(lds);
   const int p = A.pairs[blockIdx.x];
@@ -355,15 +402,16 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
-  const int kind = ring_kind(N, A.bluestein_all, &L);
+  ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
+  const double2* ch = (A.cofs[p] >= 0) ? A.chirps + A.cofs[p] : nullptr;
   const long long ringM = A.M;
   double2* scratch = l2 + N;  // fold partials (only when N <= BLOCK)
-  if (A.spin == 0) {
+  if constexpr (SPIN == 0) {
     const double2* Fb = F + (long long)b * A.nrings * ringM;
     ring_fold(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, true, false, l2, scratch, tid, BLOCK);
     if (pair) ring_fold(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, true, true, l2, scratch, tid, BLOCK);
-    const int o = ring_dft(A, N, kind, L, +1, bofs, lds, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
     const double2* out = l2 + o;
     double* mb = map + (long long)b * A.npix;
     const long long sn = ring_start(A.nside, rn);
@@ -378,10 +426,11 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
     const double2* Fmb = Fpb + (long long)A.nrings * ringM;
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
+#pragma unroll 1
     for (int h = 0; h < (pair ? 2 : 1); ++h) {
       const int r = h ? rs : rn;
       ring_fold(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
-      const int o = ring_dft(A, N, kind, L, +1, bofs, lds, tid, BLOCK);
+      const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
       const double2* out = l2 + o;
       const long long st = ring_start(A.nside, r);
       for (int j = tid; j < N; j += BLOCK) {
@@ -394,9 +443,10 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
   }
 }
 
-template <int BLOCK>
+template <int BLOCK, int E, int SPIN, int KIND>
 __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double* __restrict__ map,
                                                         double2* __restrict__ V) {
+  constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
   double2* l2 = reinterpret_cast<double2*>(lds);
   const int p = A.pairs[blockIdx.x];
@@ -408,16 +458,17 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
-  const int kind = ring_kind(N, A.bluestein_all, &L);
+  ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
+  const double2* ch = (A.cofs[p] >= 0) ? A.chirps + A.cofs[p] : nullptr;
   const long long ringM = A.M;
   const long long sn = ring_start(A.nside, rn);
   const long long ss = ring_start(A.nside, rs);
-  if (A.spin == 0) {
+  if constexpr (SPIN == 0) {
     const double* mb = map + (long long)b * A.npix;
     for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(mb[sn + j], pair ? mb[ss + j] : 0.0);
     __syncthreads();
-    const int o = ring_dft(A, N, kind, L, -1, bofs, lds, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
     const double2* W = l2 + o;
     double2* Vn = V + ((long long)b * A.nrings + rn) * ringM;
     double2* Vs = V + ((long long)b * A.nrings + rs) * ringM;
@@ -436,13 +487,14 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   } else {
     const double* Q = map + (long long)b * 2 * A.npix;
     const double* U = Q + A.npix;
+#pragma unroll 1
     for (int h = 0; h < (pair ? 2 : 1); ++h) {
       const int r = h ? rs : rn;
       const long long st = h ? ss : sn;
       __syncthreads();
       for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(Q[st + j], U[st + j]);
       __syncthreads();
-      const int o = ring_dft(A, N, kind, L, -1, bofs, lds, tid, BLOCK);
+      const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
       const double2* W = l2 + o;
       double2* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
       double2* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
@@ -458,8 +510,18 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   }
 }
 
+// Chirp table c_n = e^{i pi (n^2 mod 2N) / N}, n < N, of each Bluestein ring pair.
+__global__ void k_chirp_table(FftArgs A) {
+  const int p = A.pairs[blockIdx.x];
+  const long long o = A.cofs[p];
+  if (o < 0) return;
+  const int N = ring_len(A.nside, p);
+  double2* c = const_cast<double2*>(A.chirps) + o;
+  for (int n = threadIdx.x; n < N; n += blockDim.x) c[n] = chirp(n, N, +1);
+}
+
 // Kernel spectrum of a Bluestein ring: FFT_L of b_n = e^{-i pi n^2 / N} (n < N, mirrored).
-template <int BLOCK>
+template <int BLOCK, int E>
 __global__ __launch_bounds__(BLOCK) void k_bluestein_spectrum(FftArgs A) {
   extern __shared__ double lds[];
   const int p = A.pairs[blockIdx.x];
@@ -467,26 +529,27 @@ __global__ __launch_bounds__(BLOCK) void k_bluestein_spectrum(FftArgs A) {
   if (bofs < 0) return;
   const int tid = threadIdx.x;
   const int N = ring_len(A.nside, p);
+  const double2* ch = A.chirps + A.cofs[p];
   int L;
   ring_kind(N, A.bluestein_all, &L);
   const int lgL = ilog2(L);
-  const int nt = L >> 4;
-  double2 x[16];
+  const int nt = L / E;
+  double2 x[E];
   if (tid < nt) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
       int n = -1;
       if (i < N) n = i;
       else if (i > L - N) n = L - i;
-      x[e] = (n >= 0) ? chirp(n, N, -1) : make_double2(0.0, 0.0);
+      x[e] = (n >= 0) ? cconj(ch[n]) : make_double2(0.0, 0.0);
     }
   }
-  fft_pow2(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_doubles);
+  fft_pow2<E, lg_lo(BLOCK, E), lg_hi(BLOCK, E)>(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_doubles);
   if (tid < nt) {
     double2* B = const_cast<double2*>(A.bspec) + bofs;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) B[tid + nt * e] = x[e];
+    for (int e = 0; e < E; ++e) B[tid + nt * e] = x[e];
   }
 }
 
@@ -504,13 +567,67 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   a.pairs = c.pairs;
   a.bofs = p->fft.bofs;
   a.bspec = p->fft.bspec;
+  a.cofs = p->fft.cofs;
+  a.chirps = p->fft.chirps;
   a.lds_doubles = c.lds_doubles;
   a.bluestein_all = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
   return a;
 }
 
-constexpr int kBlocks[3] = {64, 256, 1024};
-constexpr int kLdsCap[3] = {2048, 8192, 16384};  // doubles: 16, 64, 128 KiB
+// launch classes: (threads, elements per thread) = (64,16) L<=1024, (256,16) L<=4096,
+// (512,16) L<=8192, (512,32) L<=16384; crossed with the transform kind (0 direct,
+// 1 power of two, 2 chirp-z): class index = 3 * length class + kind
+constexpr int kNlen = 4;
+constexpr int kBlocks[kNlen] = {64, 256, 512, 512};
+constexpr int kElems[kNlen] = {16, 16, 16, 32};
+constexpr int kLdsCap[kNlen] = {2048, 8192, 16384, 16384};  // doubles: 16, 64, 128, 128 KiB
+
+inline int class_of(int L) { return (L <= 1024) ? 0 : (L <= 4096) ? 1 : (L <= 8192) ? 2 : 3; }
+
+template <int BLOCK, int E, int SPIN, int KIND>
+void launch_one(bool synth, const FftArgs& a, dim3 grid, size_t sh, hipStream_t s,
+                const void* in, void* out) {
+  if (sh > 65536) {
+    static const bool once = [] {
+      (void)hipFuncSetAttribute((const void*)k_ring_synth<BLOCK, E, SPIN, KIND>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+      (void)hipFuncSetAttribute((const void*)k_ring_adjoint<BLOCK, E, SPIN, KIND>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+      return true;
+    }();
+    (void)once;
+  }
+  if (synth)
+    hipLaunchKernelGGL((k_ring_synth<BLOCK, E, SPIN, KIND>), grid, dim3(BLOCK), sh, s, a,
+                       (const double2*)in, (double*)out);
+  else
+    hipLaunchKernelGGL((k_ring_adjoint<BLOCK, E, SPIN, KIND>), grid, dim3(BLOCK), sh, s, a,
+                       (const double*)in, (double2*)out);
+}
+
+template <int SPIN>
+void launch_class(bool synth, int lc, int kind, const FftArgs& a, dim3 grid, size_t sh,
+                  hipStream_t s, const void* in, void* out) {
+  switch (lc * 3 + kind) {
+    case 0: launch_one<64, 16, SPIN, 0>(synth, a, grid, sh, s, in, out); break;
+    case 1: launch_one<64, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
+    case 2: launch_one<64, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
+    case 4: launch_one<256, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
+    case 5: launch_one<256, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
+    case 7: launch_one<512, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
+    case 8: launch_one<512, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
+    default: launch_one<512, 32, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
+  }
+}
+
+void launch_spectrum(int lc, const FftArgs& a, dim3 grid, size_t sh) {
+  switch (lc) {
+    case 0: hipLaunchKernelGGL((k_bluestein_spectrum<64, 16>), grid, dim3(64), sh, 0, a); break;
+    case 1: hipLaunchKernelGGL((k_bluestein_spectrum<256, 16>), grid, dim3(256), sh, 0, a); break;
+    case 2: hipLaunchKernelGGL((k_bluestein_spectrum<512, 16>), grid, dim3(512), sh, 0, a); break;
+    default: hipLaunchKernelGGL((k_bluestein_spectrum<512, 32>), grid, dim3(512), sh, 0, a); break;
+  }
+}
 
 }  // namespace
 
@@ -518,23 +635,25 @@ int build_fft_tables(jhtx_plan* p) {
   const int nside = p->nside;
   const int npairs = 2 * nside;
   const int ball = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
-  std::vector<int> cls_pairs[3];
-  int need[3] = {0, 0, 0};
+  constexpr int kNcls = 3 * kNlen;
+  std::vector<int> cls_pairs[kNcls];
+  int need[kNcls] = {0};
   int twL = 16;
-  std::vector<long long> bofs(npairs, -1);
-  long long blen = 0;
+  std::vector<long long> bofs(npairs, -1), cofs(npairs, -1);
+  long long blen = 0, clen = 0;
   for (int q = 0; q < npairs; ++q) {
     const int N = ring_len(nside, q);
     int L;
     const int kind = ring_kind(N, ball, &L);
-    if (L > 16384) {
+    const int lc = class_of(L);
+    if (L > 16384 || (kind == 1 && 2 * N > kLdsCap[lc])) {
       set_error("ring transform length " + std::to_string(L) + " (nside=" + std::to_string(nside) +
-                ") exceeds the 16384-point single-work-group FFT");
+                ") exceeds the single-work-group ring FFT (nside <= 2048)");
       return JHTX_EUNSUPPORTED;
     }
-    const int c = (L <= 1024) ? 0 : (L <= 4096 ? 1 : 2);
+    const int c = 3 * lc + kind;
     cls_pairs[c].push_back(q);
-    const int blk = kBlocks[c];
+    const int blk = kBlocks[lc];
     int nd = (kind == 0) ? 6 * N : 2 * L;
     if (N <= blk) nd = std::max(nd, 2 * N + 4 * blk);
     nd = std::max(nd, 2 * N);
@@ -543,6 +662,8 @@ int build_fft_tables(jhtx_plan* p) {
     if (kind == 2) {
       bofs[q] = blen;
       blen += L;
+      cofs[q] = clen;
+      clen += N;
     }
   }
   // twiddles exp(-2 pi i k / twL), accurate to the last bit
@@ -554,48 +675,53 @@ int build_fft_tables(jhtx_plan* p) {
   hipError_t e;
   e = hipMalloc(&p->fft.tw, sizeof(double2) * twL);
   if (e != hipSuccess) return JHTX_ENOMEM;
-  hipMemcpy(p->fft.tw, tw.data(), sizeof(double2) * twL, hipMemcpyHostToDevice);
+  (void)hipMemcpy(p->fft.tw, tw.data(), sizeof(double2) * twL, hipMemcpyHostToDevice);
   p->fft.lmax_fft = twL;
   e = hipMalloc(&p->fft.bofs, sizeof(long long) * npairs);
   if (e != hipSuccess) return JHTX_ENOMEM;
-  hipMemcpy(p->fft.bofs, bofs.data(), sizeof(long long) * npairs, hipMemcpyHostToDevice);
+  (void)hipMemcpy(p->fft.bofs, bofs.data(), sizeof(long long) * npairs, hipMemcpyHostToDevice);
   p->fft.bspec_len = blen;
   if (blen > 0) {
     e = hipMalloc(&p->fft.bspec, sizeof(double2) * blen);
     if (e != hipSuccess) return JHTX_ENOMEM;
   }
+  e = hipMalloc(&p->fft.cofs, sizeof(long long) * npairs);
+  if (e != hipSuccess) return JHTX_ENOMEM;
+  (void)hipMemcpy(p->fft.cofs, cofs.data(), sizeof(long long) * npairs, hipMemcpyHostToDevice);
+  if (clen > 0) {
+    e = hipMalloc(&p->fft.chirps, sizeof(double2) * clen);
+    if (e != hipSuccess) return JHTX_ENOMEM;
+  }
   p->fft.ncls = 0;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kNcls; ++c) {
     FftClass& fc = p->fft.cls[c];
-    fc.block = kBlocks[c];
+    fc.lc = c / 3;
+    fc.kind = c % 3;
+    fc.block = kBlocks[fc.lc];
     fc.npairs = (int)cls_pairs[c].size();
-    fc.lds_doubles = std::min(std::max(need[c], 64), kLdsCap[c]);
+    fc.lds_doubles = std::min(std::max(need[c], 64), kLdsCap[fc.lc]);
     fc.pairs = nullptr;
     if (fc.npairs) {
       e = hipMalloc(&fc.pairs, sizeof(int) * fc.npairs);
       if (e != hipSuccess) return JHTX_ENOMEM;
-      hipMemcpy(fc.pairs, cls_pairs[c].data(), sizeof(int) * fc.npairs, hipMemcpyHostToDevice);
+      (void)hipMemcpy(fc.pairs, cls_pairs[c].data(), sizeof(int) * fc.npairs, hipMemcpyHostToDevice);
       p->fft.ncls++;
     }
   }
-  // dynamic LDS above 64 KiB must be opted into per kernel
-  const int big = kLdsCap[2] * (int)sizeof(double);
-  hipFuncSetAttribute((const void*)k_ring_synth<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_ring_adjoint<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_bluestein_spectrum<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  // Bluestein kernel spectra (one launch per class; pairs without one return at once)
+  // chirp tables, then the Bluestein kernel spectra
   if (blen > 0) {
-    for (int c = 0; c < 3; ++c) {
+    const int big = kLdsCap[2] * (int)sizeof(double);
+    (void)hipFuncSetAttribute((const void*)k_bluestein_spectrum<512, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_bluestein_spectrum<512, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    for (int c = 0; c < kNcls; ++c) {
       const FftClass& fc = p->fft.cls[c];
-      if (!fc.npairs) continue;
-      const FftArgs a = make_fft_args(p, fc);
-      const size_t sh = sizeof(double) * fc.lds_doubles;
-      if (c == 0)
-        hipLaunchKernelGGL(k_bluestein_spectrum<64>, dim3(fc.npairs), dim3(64), sh, 0, a);
-      else if (c == 1)
-        hipLaunchKernelGGL(k_bluestein_spectrum<256>, dim3(fc.npairs), dim3(256), sh, 0, a);
-      else
-        hipLaunchKernelGGL(k_bluestein_spectrum<1024>, dim3(fc.npairs), dim3(1024), sh, 0, a);
+      if (!fc.npairs || fc.kind != 2) continue;
+      hipLaunchKernelGGL(k_chirp_table, dim3(fc.npairs), dim3(256), 0, 0, make_fft_args(p, fc));
+    }
+    for (int c = 0; c < kNcls; ++c) {
+      const FftClass& fc = p->fft.cls[c];
+      if (!fc.npairs || fc.kind != 2) continue;
+      launch_spectrum(fc.lc, make_fft_args(p, fc), dim3(fc.npairs), sizeof(double) * fc.lds_doubles);
     }
     e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -607,52 +733,42 @@ int build_fft_tables(jhtx_plan* p) {
 }
 
 void free_fft_tables(jhtx_plan* p) {
-  if (p->fft.tw) hipFree(p->fft.tw);
-  if (p->fft.bofs) hipFree(p->fft.bofs);
-  if (p->fft.bspec) hipFree(p->fft.bspec);
-  for (int c = 0; c < 3; ++c)
-    if (p->fft.cls[c].pairs) hipFree(p->fft.cls[c].pairs);
+  if (p->fft.tw) (void)hipFree(p->fft.tw);
+  if (p->fft.bofs) (void)hipFree(p->fft.bofs);
+  if (p->fft.bspec) (void)hipFree(p->fft.bspec);
+  if (p->fft.cofs) (void)hipFree(p->fft.cofs);
+  if (p->fft.chirps) (void)hipFree(p->fft.chirps);
+  for (int c = 0; c < 12; ++c)
+    if (p->fft.cls[c].pairs) (void)hipFree(p->fft.cls[c].pairs);
   p->fft = FftTables();
+}
+
+static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, void* out, int batch,
+                              hipStream_t s) {
+  for (int c = 0; c < 12; ++c) {
+    const FftClass& fc = p->fft.cls[c];
+    if (!fc.npairs) continue;
+    const FftArgs a = make_fft_args(p, fc);
+    const size_t sh = sizeof(double) * fc.lds_doubles;
+    const dim3 grid(fc.npairs, batch);
+    if (p->spin == 0)
+      launch_class<0>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    else
+      launch_class<2>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
                              hipStream_t s) {
-  for (int c = 0; c < 3; ++c) {
-    const FftClass& fc = p->fft.cls[c];
-    if (!fc.npairs) continue;
-    const FftArgs a = make_fft_args(p, fc);
-    const size_t sh = sizeof(double) * fc.lds_doubles;
-    const dim3 grid(fc.npairs, batch);
-    if (c == 0)
-      hipLaunchKernelGGL(k_ring_synth<64>, grid, dim3(64), sh, s, a, F, map);
-    else if (c == 1)
-      hipLaunchKernelGGL(k_ring_synth<256>, grid, dim3(256), sh, s, a, F, map);
-    else
-      hipLaunchKernelGGL(k_ring_synth<1024>, grid, dim3(1024), sh, s, a, F, map);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return launch_ring(p, true, F, map, batch, s);
 }
 
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,
                                hipStream_t s) {
-  for (int c = 0; c < 3; ++c) {
-    const FftClass& fc = p->fft.cls[c];
-    if (!fc.npairs) continue;
-    const FftArgs a = make_fft_args(p, fc);
-    const size_t sh = sizeof(double) * fc.lds_doubles;
-    const dim3 grid(fc.npairs, batch);
-    if (c == 0)
-      hipLaunchKernelGGL(k_ring_adjoint<64>, grid, dim3(64), sh, s, a, map, V);
-    else if (c == 1)
-      hipLaunchKernelGGL(k_ring_adjoint<256>, grid, dim3(256), sh, s, a, map, V);
-    else
-      hipLaunchKernelGGL(k_ring_adjoint<1024>, grid, dim3(1024), sh, s, a, map, V);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return launch_ring(p, false, map, V, batch, s);
 }
 
 }  // namespace jhtx

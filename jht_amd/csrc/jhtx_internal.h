@@ -49,6 +49,7 @@ struct RecArgs {
 
 // Ring-FFT launch class: ring pairs whose transform length fits one work-group shape.
 struct FftClass {
+  int lc = 0, kind = 0;    // length class and transform kind (0 direct, 1 pow2, 2 chirp-z)
   int block = 64;          // threads per work-group
   int lds_doubles = 0;     // dynamic LDS (doubles)
   int npairs = 0;
@@ -58,11 +59,13 @@ struct FftClass {
 struct FftTables {
   double2* tw = nullptr;  // exp(-2 pi i k / lmax_fft), k < lmax_fft
   int lmax_fft = 0;       // largest power-of-two transform length in the plan
-  FftClass cls[3];
+  FftClass cls[12];
   int ncls = 0;
   long long* bofs = nullptr;  // device, per ring pair: offset of its Bluestein kernel spectrum or -1
   double2* bspec = nullptr;   // device, concatenated kernel spectra
   long long bspec_len = 0;
+  long long* cofs = nullptr;  // device, per ring pair: offset of its chirp table or -1
+  double2* chirps = nullptr;  // device, concatenated chirp tables e^{i pi n^2 / N}
 };
 
 }  // namespace jhtx
@@ -71,8 +74,9 @@ struct jhtx_plan {
   int nside, lmax, spin, precision, max_batch, flags;
   int M, T, nrings, ncomp, nchunk;
   long long npix, K;
-  int rec_block;  // threads per recursion work-group
-  int rec_R;      // rings per lane
+  int rec_block;  // threads per recursion work-group (synthesis)
+  int rec_R;      // rings per lane (synthesis)
+  int adj_block, adj_R, adj_nchunk;  // the same for the adjoint recursion
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)

@@ -353,33 +353,32 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
 // ---------------------------------------------------------------------------------- //
 constexpr int NV = 16;  // double2 partials per lane per block
 
-// LDS: red[(wave*NV + v)*64 + lane], fin[seg*NV + v]
-__device__ __forceinline__ void wg_reduce_store(double2 (&P)[NV], double2* red, double2* fin,
-                                                double2* out0, double2* out1, int ncomp, int nl,
-                                                int tid, int nthr) {
+// Block reduction over the work-group's lanes.  LDS red[(wave*NV + v)*64 + lane] is written
+// lane-contiguously; the barrier that publishes it also ORs the next block's activity flag.
+// Stage 1: thread (v = tid/16, s = tid%16) sums lanes s, s+16, ... (consecutive threads read
+// consecutive lanes: conflict-free).  Stage 2: xor-shuffles across the 16 lanes of a value.
+// A second barrier retires the reads before the next block overwrites red.
+__device__ __forceinline__ int wg_reduce_store(double2 (&P)[NV], double2* red, double2* out0,
+                                               double2* out1, int ncomp, int nl, int act_next,
+                                               int tid, int nthr) {
   const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
   for (int v = 0; v < NV; ++v) red[(wave * NV + v) * 64 + lane] = P[v];
-  __syncthreads();
-  {
-    const int v = tid & (NV - 1);
-    const int seg = tid >> 4;  // 16 lanes per segment
-    const int w = seg >> 2, l0 = (seg & 3) * 16;
-    double2 s = make_double2(0.0, 0.0);
-    const double2* src = red + (w * NV + v) * 64 + l0;
+  const int flag = __syncthreads_or(act_next);
+  const int sidx = tid & 15;
+  for (int v = tid >> 4; v < NV; v += nthr >> 4) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int L = sidx; L < nthr; L += 16) acc = acc + red[((L >> 6) * NV + v) * 64 + (L & 63)];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s = s + src[i];
-    fin[seg * NV + v] = s;
+    for (int off = 8; off > 0; off >>= 1) {
+      acc.x += __shfl_xor(acc.x, off, 16);
+      acc.y += __shfl_xor(acc.y, off, 16);
+    }
+    const int jl = v / ncomp, comp = v - jl * ncomp;  // value v = jl * ncomp + comp
+    if (sidx == 0 && jl < nl) (comp == 0 ? out0 : out1)[jl] = acc;
   }
   __syncthreads();
-  if (tid < NV) {
-    const int nseg = nthr >> 4;
-    double2 s = make_double2(0.0, 0.0);
-    for (int g = 0; g < nseg; ++g) s = s + fin[g * NV + tid];
-    // value v = jl * ncomp + comp
-    const int jl = tid / ncomp, comp = tid - jl * ncomp;
-    if (jl < nl) (comp == 0 ? out0 : out1)[jl] = s;
-  }
+  return flag;
 }
 
 template <int R, int MODE>
@@ -413,7 +412,6 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
                                                 double2* __restrict__ part) {
   __shared__ double s_al[CH];
   __shared__ double2 red[4 * NV * 64];
-  __shared__ double2 fin[16 * NV];
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
   const int b = blockIdx.y;
@@ -450,6 +448,13 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
     }
   }
   double2 P[NV];
+  auto active = [&]() {
+    bool a = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a |= (ex[r] == 0) && (tb + r < A.T);
+    return (int)a;
+  };
+  int wg_active = __syncthreads_or(active());
   for (int l0 = m; l0 <= lmax; l0 += CH) {
     __syncthreads();
     if (tid < CH) {
@@ -459,15 +464,13 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
     __syncthreads();
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += NV) {
       const int nl = min(NV, lmax - (l0 + jb) + 1);
-      bool act = false;
-#pragma unroll
-      for (int r = 0; r < R; ++r) act |= (ex[r] == 0) && (tb + r < A.T);
-      if (!__syncthreads_or(act)) {
+      if (!wg_active) {
         s0_adj_block<R, RECONLY>(s_al, jb, x, mu, mup, f, U0, U1, P);
         rescale<R>(mu, mup, ex);
         s0_adj_block<R, RECONLY>(s_al, jb + 8, x, mu, mup, f, U0, U1, P);
         rescale<R>(mu, mup, ex);
         if (tid < nl) pb[l0 + jb + tid] = make_double2(0.0, 0.0);
+        wg_active = __syncthreads_or(active());
         continue;
       }
       bool deep = false;
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
           rescale<R>(mu, mup, ex);
         }
       }
-      wg_reduce_store(P, red, fin, pb + l0 + jb, nullptr, 1, nl, tid, nthr);
+      wg_active = wg_reduce_store(P, red, pb + l0 + jb, nullptr, 1, nl, active(), tid, nthr);
     }
   }
 }
@@ -537,7 +540,6 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
                                                 double2* __restrict__ part) {
   __shared__ double2 s_ag[CH];
   __shared__ double2 red[4 * NV * 64];
-  __shared__ double2 fin[16 * NV];
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
   const int b = blockIdx.y;
@@ -584,6 +586,13 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
     }
   }
   double2 P[NV];
+  auto active = [&]() {
+    bool a = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a |= ((ep[r] == 0) || (em[r] == 0)) && (tb + r < A.T);
+    return (int)a;
+  };
+  int wg_active = __syncthreads_or(active());
   for (int l0 = lmin; l0 <= lmax; l0 += CH) {
     __syncthreads();
     if (tid < CH) {
@@ -595,10 +604,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
     __syncthreads();
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += 8) {
       const int nl = min(8, lmax - (l0 + jb) + 1);
-      bool act = false;
-#pragma unroll
-      for (int r = 0; r < R; ++r) act |= ((ep[r] == 0) || (em[r] == 0)) && (tb + r < A.T);
-      if (!__syncthreads_or(act)) {
+      if (!wg_active) {
         s2_adj_block<R, RECONLY>(s_ag, jb, x, pp, ppp, pm, pmp, fp, fm, Vpn, Vps, Vmn, Vms, P);
         rescale<R>(pp, ppp, ep);
         rescale<R>(pm, pmp, em);
@@ -606,6 +612,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
           p_p2[l0 + jb + tid] = make_double2(0.0, 0.0);
           p_m2[l0 + jb + tid] = make_double2(0.0, 0.0);
         }
+        wg_active = __syncthreads_or(active());
         continue;
       }
       bool deep = false;
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
         rescale<R>(pp, ppp, ep);
         rescale<R>(pm, pmp, em);
       }
-      wg_reduce_store(P, red, fin, p_p2 + l0 + jb, p_m2 + l0 + jb, 2, nl, tid, nthr);
+      wg_active = wg_reduce_store(P, red, p_p2 + l0 + jb, p_m2 + l0 + jb, 2, nl, active(), tid, nthr);
     }
   }
 }
@@ -667,7 +674,7 @@ __global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const double2* __r
   }
 }
 
-RecArgs make_args(const jhtx_plan* p) {
+RecArgs make_args(const jhtx_plan* p, bool adjoint) {
   RecArgs a;
   a.alpha = p->rec.alpha;
   a.gamma = p->rec.gamma;
@@ -681,7 +688,7 @@ RecArgs make_args(const jhtx_plan* p) {
   a.T = p->T;
   a.nside = p->nside;
   a.nrings = p->nrings;
-  a.nchunk = p->nchunk;
+  a.nchunk = adjoint ? p->adj_nchunk : p->nchunk;
   a.K = p->K;
   return a;
 }
@@ -690,44 +697,48 @@ RecArgs make_args(const jhtx_plan* p) {
 
 hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                             hipStream_t s) {
-  const RecArgs a = make_args(p);
+  const RecArgs a = make_args(p, false);
   const int Mpad = (p->M + 7) & ~7;
   dim3 grid(Mpad * p->nchunk, batch);
   dim3 block(p->rec_block);
   if (p->spin == 0) {
-    if (p->rec_R == 4)
-      hipLaunchKernelGGL(k_synth_s0<4>, grid, block, 0, s, a, alm, F);
-    else
-      hipLaunchKernelGGL(k_synth_s0<1>, grid, block, 0, s, a, alm, F);
+    switch (p->rec_R) {
+      case 8: hipLaunchKernelGGL(k_synth_s0<8>, grid, block, 0, s, a, alm, F); break;
+      case 4: hipLaunchKernelGGL(k_synth_s0<4>, grid, block, 0, s, a, alm, F); break;
+      default: hipLaunchKernelGGL(k_synth_s0<1>, grid, block, 0, s, a, alm, F); break;
+    }
   } else {
-    if (p->rec_R == 2)
-      hipLaunchKernelGGL(k_synth_s2<2>, grid, block, 0, s, a, alm, F);
-    else
-      hipLaunchKernelGGL(k_synth_s2<1>, grid, block, 0, s, a, alm, F);
+    switch (p->rec_R) {
+      case 4: hipLaunchKernelGGL(k_synth_s2<4>, grid, block, 0, s, a, alm, F); break;
+      case 2: hipLaunchKernelGGL(k_synth_s2<2>, grid, block, 0, s, a, alm, F); break;
+      default: hipLaunchKernelGGL(k_synth_s2<1>, grid, block, 0, s, a, alm, F); break;
+    }
   }
   return hipGetLastError();
 }
 
 hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
                           int batch, hipStream_t s) {
-  const RecArgs a = make_args(p);
+  const RecArgs a = make_args(p, true);
   const int Mpad = (p->M + 7) & ~7;
-  dim3 grid(Mpad * p->nchunk, batch);
-  dim3 block(p->rec_block);
+  dim3 grid(Mpad * p->adj_nchunk, batch);
+  dim3 block(p->adj_block);
   if (p->spin == 0) {
-    if (p->rec_R == 4)
-      hipLaunchKernelGGL(k_adj_s0<4>, grid, block, 0, s, a, V, part);
-    else
-      hipLaunchKernelGGL(k_adj_s0<1>, grid, block, 0, s, a, V, part);
+    switch (p->adj_R) {
+      case 8: hipLaunchKernelGGL(k_adj_s0<8>, grid, block, 0, s, a, V, part); break;
+      case 4: hipLaunchKernelGGL(k_adj_s0<4>, grid, block, 0, s, a, V, part); break;
+      default: hipLaunchKernelGGL(k_adj_s0<1>, grid, block, 0, s, a, V, part); break;
+    }
   } else {
-    if (p->rec_R == 2)
-      hipLaunchKernelGGL(k_adj_s2<2>, grid, block, 0, s, a, V, part);
-    else
-      hipLaunchKernelGGL(k_adj_s2<1>, grid, block, 0, s, a, V, part);
+    switch (p->adj_R) {
+      case 4: hipLaunchKernelGGL(k_adj_s2<4>, grid, block, 0, s, a, V, part); break;
+      case 2: hipLaunchKernelGGL(k_adj_s2<2>, grid, block, 0, s, a, V, part); break;
+      default: hipLaunchKernelGGL(k_adj_s2<1>, grid, block, 0, s, a, V, part); break;
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, p->nchunk,
+  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, p->adj_nchunk,
                      part, alm);
   return hipGetLastError();
 }

@@ -132,3 +132,31 @@ def test_input_validation_gpu():
         j.adjoint_synthesis(np.zeros(12 * nside**2), nside, lmax, spin=2)
     with pytest.raises(NotImplementedError):
         j.synthesis(np.zeros((2, K), dtype=np.complex128), nside, lmax, spin=1)
+
+
+@pytest.mark.parametrize("spin,synth_r,adj_r,block", [
+    (0, 4, 4, 64), (0, 8, 8, 64), (0, 1, 1, 64), (2, 2, 2, 64), (2, 4, 4, 64), (2, 4, 4, 256), (0, 8, 8, 256)])
+def test_lane_variants(spin, synth_r, adj_r, block, monkeypatch):
+    """Every rings-per-lane instantiation and multi-chunk ring reductions (forced through
+    the plan's tuning overrides) against the oracle."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_SYNTH_R", str(synth_r))
+    monkeypatch.setenv("JHTX_ADJ_R", str(adj_r))
+    monkeypatch.setenv("JHTX_REC_BLOCK", str(block))
+    H.clear_plans()
+    try:
+        nside, lmax = 128, 200
+        rng = np.random.default_rng(17 + spin + synth_r)
+        a = _alm(lmax, rng, spin)
+        npix = 12 * nside * nside
+        v = rng.standard_normal(npix if spin == 0 else (2, npix))
+        out = j.synthesis(torch.tensor(a, device="cuda"), nside, lmax, spin).cpu().numpy()
+        b = j.adjoint_synthesis(torch.tensor(v, device="cuda"), nside, lmax, spin).cpu().numpy()
+    finally:
+        H.clear_plans()
+    e1 = _rel(out, O.synthesis(a, nside, lmax, spin))
+    e2 = _rel(b, O.adjoint_synthesis(v, nside, lmax, spin))
+    print(f"variant spin={spin} R={synth_r}/{adj_r} block={block}: synth {e1:.2e} adj {e2:.2e}", flush=True)
+    assert e1 <= TOL and e2 <= TOL
