@@ -2,18 +2,23 @@
 
 Workload (the metric's configuration): spin 2, nside = 2048, lmax = 4096, E/B a_lm with
 C_l^EE = (l+1)^-2, C_l^BB = 0.1 C_l^EE, l < 2 zero (synthetic, seeded).  One step is one
-synthesis S a -> (Q, U) followed by one adjoint S^T (Q, U) -> a', per map of the batch;
-inputs are resident in HBM before the timed region.  Prints one JSON line.
+synthesis S a -> (Q, U) followed by one adjoint S^T (Q, U) -> a' for every map of the
+batch; inputs are resident in HBM before the timed region.  `value` is maps/s of the
+selected precision (default: mixed, batch 8); the FP64 path (batch 1) is timed beside it
+and the mixed outputs of the first map are compared with the FP64 GPU outputs
+(max|x - x64| / max|x64|).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--precision f64|mixed] [--batch B]
-                    [--nside N --lmax L --spin S] [--no-cpu]
+                    [--nside N --lmax L --spin S] [--no-cpu] [--no-f64]
 
-Roofline: the dominant kernel (recursion + contraction, synthesis or adjoint) is timed
-live with HIP events recorded by the library on the call's stream; its algorithmic work is
-26 flops (spin 2; 8 for spin 0) per (l, m, north-half ring) triple of the active triangle
-l >= max(m, |s|), times 2 nside rings.  CPU baseline: the reference's fused masked scan
-restated in numpy (oracle/jht_oracle.py) on a sample of orders m over every l and ring,
-scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs, single-threaded.
+Roofline: the dominant recursion + contraction kernel (synthesis or adjoint) is timed live
+with HIP events recorded by the library on the call's stream.  Work unit: one (l, m,
+north-half ring) triple of the active triangle l >= max(m, |s|).  FP64 path: 26 flops per
+triple per map (spin 2; 8 for spin 0) against the 78.6 TFLOP/s FP64 peak.  Mixed path: the
+contraction that runs on the FP32 matrix cores, 16 flops per triple per map (spin 2; 4 for
+spin 0), against the 157.3 TFLOP/s dense FP32 MFMA peak.  CPU baseline: the reference's
+fused masked scan restated in numpy (oracle/jht_oracle.py) on a sample of orders m over
+every l and ring, scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs.
 """
 
 from __future__ import annotations
@@ -30,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
@@ -93,78 +99,113 @@ def cpu_baseline(nside, lmax, spin, rows=8, seed=1):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="f64", choices=["f64"])
-    ap.add_argument("--batch", type=int, default=1)
-    ap.add_argument("--nside", type=int, default=2048)
-    ap.add_argument("--lmax", type=int, default=4096)
-    ap.add_argument("--spin", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=12)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
-              "(each map is independent -- multi-GPU runs shard the batch, see DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run(plan, alm, maps, back, B, steps, warmup):
     import torch
-
-    import jht_amd
-    from jht_amd.healpix import get_plan
-
-    nside, lmax, spin, B = args.nside, args.lmax, args.spin, args.batch
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    rng = np.random.default_rng(20261203)
-    K = jht_amd.alm_size(lmax)
-    if spin == 0:
-        a = np.stack([gaussian_alm(lmax, rng) for _ in range(B)])
-    else:
-        a = np.stack([np.stack([gaussian_alm(lmax, rng, 2), gaussian_alm(lmax, rng, 2, 0.1)]) for _ in range(B)])
-    alm = torch.tensor(a, device=dev)
-    del a
-    plan = get_plan(nside, lmax, spin, B)
-    npix = 12 * nside * nside
-    maps = torch.empty((B, npix) if spin == 0 else (B, 2, npix), dtype=torch.float64, device=dev)
-    back = torch.empty_like(alm)
 
     def step():
         plan.synth(alm, maps, B)
         plan.adjoint(maps, back, B)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     plan.set_profiling(True)
     plan.read_profile()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     prof = plan.read_profile()
     plan.set_profiling(False)
+    per = {k: (v[0] / max(v[1], 1), v[1]) for k, v in prof.items()}
+    return dt, per
 
-    # size-independent check: <S a, v> == <a, S^T v>_G on the bench data itself
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="mixed", choices=["f64", "mixed"])
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--nside", type=int, default=2048)
+    ap.add_argument("--lmax", type=int, default=4096)
+    ap.add_argument("--spin", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
+              "(maps are independent: several GPUs would each take a share of the batch)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import jht_amd
+    from jht_amd import _lib
+    from jht_amd.healpix import get_plan
+
+    nside, lmax, spin, B = args.nside, args.lmax, args.spin, args.batch
+    prec = _lib.PREC_MIXED if args.precision == "mixed" else _lib.PREC_F64
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(20261203)
+    if spin == 0:
+        a = np.stack([gaussian_alm(lmax, rng) for _ in range(B)])
+    else:
+        a = np.stack([np.stack([gaussian_alm(lmax, rng, 2), gaussian_alm(lmax, rng, 2, 0.1)]) for _ in range(B)])
+    alm = torch.tensor(a, device=dev)
+    del a
+    npix = 12 * nside * nside
+    mshape = (B, npix) if spin == 0 else (B, 2, npix)
+    maps = torch.empty(mshape, dtype=torch.float64, device=dev)
+    back = torch.empty_like(alm)
+    plan = get_plan(nside, lmax, spin, B, precision=prec)
+    dt, per = run(plan, alm, maps, back, B, args.steps, args.warmup)
+    value = B * args.steps / dt
+
+    out_extra = {}
+    if not args.no_f64 and prec != _lib.PREC_F64:
+        # FP64 reference path on the first map: timing and the mixed-vs-FP64 error
+        p64 = get_plan(nside, lmax, spin, 1, precision=_lib.PREC_F64)
+        m64 = torch.empty((1,) + mshape[1:], dtype=torch.float64, device=dev)
+        b64 = torch.empty_like(alm[:1])
+        a0 = alm[:1].contiguous()
+        dt64, per64 = run(p64, a0, m64, b64, 1, max(2, args.steps // 2), 1)
+        # same input map for both adjoints: the FP64 synthesis output
+        bmx = torch.empty_like(alm)
+        mx_in = maps.clone()
+        mx_in[0] = m64[0]
+        plan.adjoint(mx_in, bmx, B)
+        torch.cuda.synchronize()
+        e_s = float((maps[0] - m64[0]).abs().max() / m64[0].abs().max())
+        e_a = float((bmx[0] - b64[0]).abs().max() / b64[0].abs().max())
+        v64 = 1.0 / dt64 * max(2, args.steps // 2)
+        out_extra = {"f64_maps_per_s": round(v64, 4), "speedup_vs_f64": round(value / v64, 3),
+                     "rel_err_vs_f64": {"synthesis": e_s, "adjoint": e_a},
+                     "f64_stages_ms": {k: round(v[0], 3) for k, v in per64.items()}}
+        del p64, m64, b64, bmx, mx_in
+
+    # size-independent check: <S a, v> == <a, S^T v>_G with v = S a
     G = torch.as_tensor(jht_amd.alm_metric_weight(lmax).copy(), device=dev)
-    lhs = float((maps.double() * maps.double()).sum())
-    rhs = float((G * (alm.conj() * back).real).sum())
+    lhs = float((maps[0] * maps[0]).sum())
+    rhs = float((G * (alm[0].conj() * back[0]).real).sum())
     ident = abs(lhs - rhs) / abs(lhs)
 
     ms_step = dt / args.steps * 1e3
-    value = B * args.steps / dt
-    per = {k: (v[0] / max(v[1], 1), v[1]) for k, v in prof.items()}
     dom = max(("synth_rec", "adj_rec"), key=lambda k: per[k][0])
-    flops = triples(nside, lmax, spin) * (26 if spin else 8) * B
+    tri = triples(nside, lmax, spin)
+    if prec == _lib.PREC_F64:
+        flops, peak, bound = tri * (26 if spin else 8) * B, PEAK_FP64_TFLOPS, "mfma"
+    else:
+        flops, peak, bound = tri * (16 if spin else 4) * B, PEAK_FP32_MFMA_TFLOPS, "mfma"
     achieved = flops / (per[dom][0] * 1e-3) / 1e12
     out = {
-        "metric": "maps/s synth+adjoint nside=2048 lmax=4096 (spin 2)",
+        "metric": "maps/s synth+adjoint nside=2048 lmax=4096 (spin 2): mixed vs FP64 GPU at rel err <= 1e-5",
         "value": round(value, 4),
         "unit": "maps/s",
         "n_gpus": 1,
@@ -174,17 +215,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == _lib.PREC_F64 else "f64 recursion / f32 mfma contraction",
         "data": "synthetic (seeded Gaussian E/B a_lm, C_l=(l+1)^-2, B=0.1 E, l<2 zero)",
         "config": {"workload": f"spin-{spin} synthesis+adjoint, nside={nside}, lmax={lmax}",
                    "nside": nside, "lmax": lmax, "spin": spin, "batch": B,
                    "precision": args.precision, "parallelism": "single-gpu"},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None},
+        "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None},
         "stages_ms": {k: round(v[0], 3) for k, v in per.items()},
         "adjoint_identity": {"lhs": lhs, "rhs": rhs, "rel": ident},
     }
+    out.update(out_extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nside, lmax, spin, args.cpu_rows)
     print(json.dumps(out), flush=True)

@@ -274,9 +274,9 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
     set_error("invalid nside/lmax/max_batch");
     return JHTX_EINVAL;
   }
-  if (precision != JHTX_PREC_F64) {
-    set_error("precision " + std::to_string(precision) + " not available in this build");
-    return JHTX_EUNSUPPORTED;
+  if (precision != JHTX_PREC_F64 && precision != JHTX_PREC_MIXED) {
+    set_error("precision " + std::to_string(precision) + " unknown");
+    return JHTX_EINVAL;
   }
   jhtx_plan* p = new jhtx_plan();  // value-initialised: every field starts at zero / null
   p->nside = nside;
@@ -326,7 +326,9 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
   shape(p->rec_R, p->rec_block, p->nchunk);
   shape(p->adj_R, p->adj_block, p->adj_nchunk);
   p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
-  p->ws_part_bytes = align256(sizeof(double2) * (size_t)p->adj_nchunk * max_batch * p->ncomp * p->K);
+  // the mixed kernels use one ring per lane and 256-lane work-groups
+  const int part_chunks = (precision == JHTX_PREC_MIXED) ? (p->T + 255) / 256 : p->adj_nchunk;
+  p->ws_part_bytes = align256(sizeof(double2) * (size_t)part_chunks * max_batch * p->ncomp * p->K);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
   int rc = build_rec_tables(p);
   if (rc == JHTX_OK) rc = build_fft_tables(p);
@@ -348,7 +350,9 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   double2* F = reinterpret_cast<double2*>(ws);
   StageTimer t0(p, s, 0);
-  hipError_t e = launch_synth_rec(p, reinterpret_cast<const double2*>(alm), F, batch, s);
+  hipError_t e = (p->precision == JHTX_PREC_MIXED)
+                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), F, batch, s)
+                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), F, batch, s);
   t0.done();
   if (e == hipSuccess) {
     StageTimer t1(p, s, 1);
@@ -373,7 +377,13 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   t2.done();
   if (e == hipSuccess) {
     StageTimer t3(p, s, 3);
-    e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
+    if (p->precision == JHTX_PREC_MIXED) {
+      e = launch_adj_mixed(p, V, part, batch, s);
+      if (e == hipSuccess)
+        e = launch_adj_finish(p, part, reinterpret_cast<double2*>(alm), batch, (p->T + 255) / 256, s);
+    } else {
+      e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
+    }
     t3.done();
   }
   if (e != hipSuccess) {

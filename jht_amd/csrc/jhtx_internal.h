@@ -95,6 +95,12 @@ hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, 
                             hipStream_t s);
 hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
                           int batch, hipStream_t s);
+hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
+                             int nchunk, hipStream_t s);
+hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+                              hipStream_t s);
+hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
+                            hipStream_t s);
 hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
                              hipStream_t s);
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,

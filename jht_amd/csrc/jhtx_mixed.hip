@@ -1,0 +1,624 @@
+// Mixed-precision batched recursion + contraction: the l-recursion stays in FP64 on the
+// vector ALUs (one ring per lane, exponent-tracked exactly as in the FP64 kernels); the
+// contraction over l runs on the FP32 matrix cores for a group of MG maps at once.
+//
+// Synthesis, F[t][col] = sum_l lambda_l(t) a~_l[col]: v_mfma_f32_4x4x1f32 (16 blocks of 4x4)
+// takes A from lane 4b+i and B from lane 4b+j, so the recursion value of the lane's ring is
+// the A operand as it stands, B is the lane's column (component lane%4) of G_l a_lm, and each
+// instruction accumulates an outer product over one l.  Even and odd (l - lmin) go to
+// separate accumulators (north = sum, south = difference, as in the FP64 kernels).
+//
+// Adjoint, b_l[col] = sum_t lambda_l(t) U[t][col]: each wave writes its FP32 recursion values
+// for 32 consecutive l to an LDS tile; v_mfma_f32_16x16x4f32 then contracts 16 same-parity l
+// rows against 4 rings per instruction, so the matrix core performs the ring reduction.  The
+// per-wave results are summed over the work-group in FP64 and written as one partial per ring
+// chunk (the FP64 path's chunk reduction then finishes the sum).
+#include "jhtx_internal.h"
+
+namespace jhtx {
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr double kLn2Hi = 6.93147180369123816490e-01;
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+constexpr double kLog2e = 1.44269504088896338700e+00;
+constexpr int CH = 64;
+
+__device__ __forceinline__ void seed_split(double L, double sign, double& mu, int& ex) {
+  if (!(L > -1.0e300)) {
+    mu = 0.0;
+    ex = 0;
+    return;
+  }
+  int e = (int)floor((L * kLog2e + 256.0) * (1.0 / 512.0));
+  if (e > 0) e = 0;
+  const double n = (double)(kScaleStep * e);
+  const double r = (L - n * kLn2Hi) - n * kLn2Lo;
+  mu = sign * exp(r);
+  ex = e;
+}
+
+__device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, double ls) {
+  double L = sp[0];
+  if (sp[1] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[1], lc));
+  if (sp[2] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[2], ls));
+  return L;
+}
+
+__device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
+  const int P = blockIdx.x;
+  const int xcd = P & 7;
+  const int slot = P >> 3;
+  chunk = slot % A.nchunk;
+  m = (slot / A.nchunk) * 8 + xcd;
+  return m < A.M;
+}
+
+__device__ __forceinline__ void rescale1(double& mu, double& mup, int& ex) {
+  if (ex < 0 && fabs(mu) > kBig) {
+    mu *= kRescale;
+    mup *= kRescale;
+    ex += 1;
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+enum { FULL = 0, MASKED = 1, RECONLY = 2 };
+
+// ---------------------------------------------------------------------------------- //
+// spin 2 synthesis, MG maps per work-group (grid.y = map group)
+// ---------------------------------------------------------------------------------- //
+template <int MG>
+__global__ __launch_bounds__(256) void k_synth_mx_s2(RecArgs A, const double2* __restrict__ alm,
+                                                     double2* __restrict__ F, int batch) {
+  __shared__ double2 s_ag[CH];
+  __shared__ float s_a[CH][4][MG];       // [l][component: aE.re aE.im aB.re aB.im][map]
+  __shared__ float s_ep[4][2][2][64][4];  // [wave][class][sign][ring][component]
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * 256 + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
+  int ep = 0, em = 0;
+  if (valid && lmin <= lmax) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m;
+    const double lc = A.lcos[t], ls = A.lsin[t];
+    double s;
+    seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep);
+    ppp = -s;
+    seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
+    pmp = -s;
+  }
+  floatx4 accP[2][MG], accM[2][MG];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int g = 0; g < MG; ++g) accP[c][g] = accM[c][g] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      double2 ag = make_double2(0.0, 0.0);
+      if (l <= lmax && l != lmin) ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+      s_ag[tid] = ag;
+    }
+    for (int i = tid; i < CH * 4 * MG; i += 256) {
+      const int j = i / (4 * MG), comp = (i / MG) & 3, g = i % MG;
+      const int l = l0 + j, b = mg * MG + g;
+      float v = 0.f;
+      if (l <= lmax && b < batch) {
+        const long long idx = cb + l;
+        const double2 a = alm[((long long)b * 2 + (comp >> 1)) * A.K + idx];
+        v = (float)(((comp & 1) ? a.y : a.x) * A.G[idx]);
+      }
+      s_a[j][comp][g] = v;
+    }
+    __syncthreads();
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    for (int j0 = 0; j0 < nst; j0 += 8) {
+      const bool deep = valid && ((ep < 0) || (em < 0));
+      const bool alldeep = !valid || ((ep < 0) && (em < 0));
+      int mode = FULL;
+      if (__ballot(deep) != 0ull) mode = (__ballot(!alldeep) == 0ull) ? RECONLY : MASKED;
+      const double fp = (ep == 0) ? 1.0 : 0.0, fm = (em == 0) ? 1.0 : 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = j0 + jj;
+        const double2 ag = s_ag[j];
+        const double cp = fma(ag.x, x, ag.y);
+        const double cm = fma(ag.x, x, -ag.y);
+        const double np = fma(cp, pp, -ppp);
+        const double nm = fma(cm, pm, -pmp);
+        ppp = pp;
+        pp = np;
+        pmp = pm;
+        pm = nm;
+        if (mode != RECONLY) {
+          // mask in FP64: a lane still below 2^-256 may hold a mantissa above the FP32 range
+          const float ap = (float)((mode == MASKED) ? np * fp : np);
+          const float am = (float)((mode == MASKED) ? nm * fm : nm);
+          const float* bq = &s_a[j][lane & 3][0];
+#pragma unroll
+          for (int g = 0; g < MG; ++g) {
+            accP[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(ap, bq[g], accP[jj & 1][g], 0, 0, 0);
+            accM[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(am, bq[g], accM[jj & 1][g], 0, 0, 0);
+          }
+        }
+      }
+      if (mode != FULL) {
+        rescale1(pp, ppp, ep);
+        rescale1(pm, pmp, em);
+      }
+    }
+  }
+  // epilogue: lane l holds rings 4(l/4)+r, component l%4; gather a ring's 4 components
+  const int swap = (lmin - m) & 1;  // classes counted from lmin; reflection needs (l - m)
+  const long long ringM = A.M;
+#pragma unroll
+  for (int g = 0; g < MG; ++g) {
+    const int b = mg * MG + g;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        s_ep[wave][c ^ swap][0][(lane & ~3) + r][lane & 3] = accP[c][g][r];
+        s_ep[wave][c ^ swap][1][(lane & ~3) + r][lane & 3] = accM[c][g][r];
+      }
+    wave_sync();
+    if (valid && b < batch && lmin <= lmax) {
+      double P0[4], P1[4], M0[4], M1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        P0[k] = s_ep[wave][0][0][lane][k];
+        P1[k] = s_ep[wave][1][0][lane][k];
+        M0[k] = s_ep[wave][0][1][lane][k];
+        M1[k] = s_ep[wave][1][1][lane][k];
+      }
+      // PE = (P[0] + i P[1]), PB = (P[2] + i P[3]); Fp = -PE - i PB ; Fm = -ME + i MB
+      double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+      double2* Fm = Fp + (long long)A.nrings * ringM;
+      const double pe_r = P0[0] + P1[0], pe_i = P0[1] + P1[1], pb_r = P0[2] + P1[2], pb_i = P0[3] + P1[3];
+      const double me_r = M0[0] + M1[0], me_i = M0[1] + M1[1], mb_r = M0[2] + M1[2], mb_i = M0[3] + M1[3];
+      Fp[(long long)t * ringM + m] = make_double2(-pe_r + pb_i, -pe_i - pb_r);
+      Fm[(long long)t * ringM + m] = make_double2(-me_r - mb_i, -me_i + mb_r);
+      if (t < A.T - 1) {
+        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        const double de_r = M0[0] - M1[0], de_i = M0[1] - M1[1], db_r = M0[2] - M1[2], db_i = M0[3] - M1[3];
+        const double qe_r = P0[0] - P1[0], qe_i = P0[1] - P1[1], qb_r = P0[2] - P1[2], qb_i = P0[3] - P1[3];
+        Fp[rs] = make_double2(-de_r + db_i, -de_i - db_r);
+        Fm[rs] = make_double2(-qe_r - qb_i, -qe_i + qb_r);
+      }
+    } else if (valid && b < batch) {
+      double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+      double2* Fm = Fp + (long long)A.nrings * ringM;
+      Fp[(long long)t * ringM + m] = Fm[(long long)t * ringM + m] = make_double2(0.0, 0.0);
+      if (t < A.T - 1) {
+        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        Fp[rs] = Fm[rs] = make_double2(0.0, 0.0);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// spin 0 synthesis, MG = 8 maps: instruction q covers maps 2q, 2q+1 (lane%4 = 2 map + re/im)
+// ---------------------------------------------------------------------------------- //
+template <int MG>
+__global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* __restrict__ alm,
+                                                     double2* __restrict__ F, int batch) {
+  constexpr int NQ = MG / 2;
+  __shared__ double s_al[CH];
+  __shared__ float s_a[CH][4][NQ];    // [l][lane%4][q]
+  __shared__ float s_ep[4][2][64][4];  // [wave][class][ring][lane%4]
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * 256 + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double s;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, s, ex);
+    mup = -s;
+  }
+  floatx4 acc[2][NQ];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[c][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    for (int i = tid; i < CH * 4 * NQ; i += 256) {
+      const int j = i / (4 * NQ), c4 = (i / NQ) & 3, q = i % NQ;
+      const int l = l0 + j, b = mg * MG + 2 * q + (c4 >> 1);
+      float v = 0.f;
+      if (l <= lmax && b < batch) {
+        const long long idx = cb + l;
+        const double2 a = alm[(long long)b * A.K + idx];
+        v = (float)(((c4 & 1) ? a.y : a.x) * A.G[idx]);
+      }
+      s_a[j][c4][q] = v;
+    }
+    __syncthreads();
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    for (int j0 = 0; j0 < nst; j0 += 8) {
+      const bool deep = valid && ex < 0;
+      const bool alldeep = !valid || ex < 0;
+      int mode = FULL;
+      if (__ballot(deep) != 0ull) mode = (__ballot(!alldeep) == 0ull) ? RECONLY : MASKED;
+      const double f = (ex == 0) ? 1.0 : 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = j0 + jj;
+        const double c = s_al[j] * x;
+        const double mn = fma(c, mu, -mup);
+        mup = mu;
+        mu = mn;
+        if (mode != RECONLY) {
+          const float a = (float)((mode == MASKED) ? mn * f : mn);
+          const float* bq = &s_a[j][lane & 3][0];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            acc[jj & 1][q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bq[q], acc[jj & 1][q], 0, 0, 0);
+        }
+      }
+      if (mode != FULL) rescale1(mu, mup, ex);
+    }
+  }
+  const long long ringM = A.M;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_ep[wave][c][(lane & ~3) + r][lane & 3] = acc[c][q][r];
+    wave_sync();
+    if (valid) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int b = mg * MG + 2 * q + h;
+        if (b >= batch) continue;
+        const double c0r = s_ep[wave][0][lane][2 * h], c0i = s_ep[wave][0][lane][2 * h + 1];
+        const double c1r = s_ep[wave][1][lane][2 * h], c1i = s_ep[wave][1][lane][2 * h + 1];
+        double2* Fb = F + (long long)b * A.nrings * ringM;
+        Fb[(long long)t * ringM + m] = make_double2(c0r + c1r, c0i + c1i);
+        if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * ringM + m] = make_double2(c0r - c1r, c0i - c1i);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// adjoint: per-wave FP32 tile of 32 l x 64 rings (per recursion sign), 16x16x4 MFMA
+// ---------------------------------------------------------------------------------- //
+// 16x16x4 f32 layouts: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
+// D[i][j] at lane j + 16 (i / 4), register i % 4.
+constexpr int TL = 32;  // l values per tile (16 rows of each parity)
+// Tile row stride in floats: A-operand reads take rows 2i+p (i = lane%16) and ring column
+// 4q + lane/16; with a stride of 66 the bank (4i + lane/16) mod 64 is distinct per lane.
+constexpr int TS = 66;
+
+// Sum the per-wave D tiles (rows = l, cols = 16) over the work-group in FP64 and store the
+// chunk partials.  red aliases the wave tiles.  Caller provides the column -> output map.
+template <int NSIGN>
+__device__ __forceinline__ void adj_tile_mfma(const float* tile, int lane, const float (&Bv)[NSIGN][2][16],
+                                              floatx4 (&D)[2]) {
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    D[p] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sg = 0; sg < NSIGN; ++sg) {
+      const float* ts = tile + sg * TL * TS;
+      const int row = 2 * (lane & 15) + p;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const float a = ts[row * TS + 4 * q + (lane >> 4)];
+        D[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[sg][p][q], D[p], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// spin 2: columns j = 4 g + c, c in {p2.re, p2.im, m2.re, m2.im}, g map in group (MG = 4)
+__global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
+                                                   double2* __restrict__ part, int batch) {
+  constexpr int MG = 4;
+  __shared__ double2 s_ag[CH];
+  __shared__ float tiles[4][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  if (lmin > lmax) return;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * 256 + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
+  int ep = 0, em = 0;
+  if (valid) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m;
+    const double lc = A.lcos[t], ls = A.lsin[t];
+    double s;
+    seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep);
+    ppp = -s;
+    seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
+    pmp = -s;
+  }
+  // B operands: ring 4q + lane/16 of this wave, column lane%16 -> map g, component c
+  float Bv[2][2][16];
+  {
+    const int g = (lane & 15) >> 2, c = lane & 3, b = mg * MG + g;
+    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;  // (-1)^(l-m) at even (l - lmin)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
+      double vn = 0.0, vs = 0.0, wn = 0.0, ws = 0.0;  // sign + : (n, s·south) ; sign - : (s·south, n)
+      if (tr < A.T && b < batch) {
+        const double2* Vp = V + ((long long)b * 2 + 0) * A.nrings * ringM;
+        const double2* Vm = V + ((long long)b * 2 + 1) * A.nrings * ringM;
+        const long long rn = (long long)tr * ringM + m;
+        const double2 vpn = Vp[rn], vmn = Vm[rn];
+        double2 vps = make_double2(0.0, 0.0), vms = vps;
+        if (tr < A.T - 1) {
+          const long long rs = (long long)(4 * A.nside - 2 - tr) * ringM + m;
+          vps = Vp[rs];
+          vms = Vm[rs];
+        }
+        // mu+: p2 <- Vpn, m2 <- s Vms ; mu-: p2 <- s Vps, m2 <- Vmn
+        vn = (c == 0) ? vpn.x : (c == 1) ? vpn.y : 0.0;
+        vs = (c == 2) ? vms.x : (c == 3) ? vms.y : 0.0;
+        wn = (c == 2) ? vmn.x : (c == 3) ? vmn.y : 0.0;
+        ws = (c == 0) ? vps.x : (c == 1) ? vps.y : 0.0;
+      }
+      Bv[0][0][q] = (float)(vn + par0 * vs);
+      Bv[0][1][q] = (float)(vn - par0 * vs);
+      Bv[1][0][q] = (float)(wn + par0 * ws);
+      Bv[1][1][q] = (float)(wn - par0 * ws);
+    }
+  }
+  double2* p_p2 = part + (((long long)chunk * batch) * 2) * A.K;  // offset per map below
+  float* mytile = &tiles[wave][0][0][0];
+  double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16][16] FP64
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      double2 ag = make_double2(0.0, 0.0);
+      if (l <= lmax && l != lmin) ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+      s_ag[tid] = ag;
+    }
+    __syncthreads();
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+      // recursion for TL steps, values to the wave's tile
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const bool deep = valid && ((ep < 0) || (em < 0));
+        const int anydeep = __ballot(deep) != 0ull;
+        const double fp = (ep == 0 && valid) ? 1.0 : 0.0, fm = (em == 0 && valid) ? 1.0 : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int j = jb + j0 + jj;
+          const double2 ag = s_ag[j];
+          const double cp = fma(ag.x, x, ag.y);
+          const double cm = fma(ag.x, x, -ag.y);
+          const double np = fma(cp, pp, -ppp);
+          const double nm = fma(cm, pm, -pmp);
+          ppp = pp;
+          pp = np;
+          pmp = pm;
+          pm = nm;
+          mytile[(0 * TL + j0 + jj) * TS + lane] = (float)(np * fp);
+          mytile[(1 * TL + j0 + jj) * TS + lane] = (float)(nm * fm);
+        }
+        if (anydeep) {
+          rescale1(pp, ppp, ep);
+          rescale1(pm, pmp, em);
+        }
+      }
+      wave_sync();
+      floatx4 D[2];
+      adj_tile_mfma<2>(mytile, lane, Bv, D);
+      // publish per-wave sums (FP64) in the tile area after every wave has read its tile
+      __syncthreads();
+      double* myred = red + wave * 2 * 256;
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * (lane >> 4) + r, jc = lane & 15;
+          myred[p * 256 + i * 16 + jc] = (double)D[p][r];
+        }
+      __syncthreads();
+      // thread -> (p, i, g): sums 4 waves for the 4 components of map g at l = l0+jb+2i+p
+      if (tid < 2 * 16 * MG) {
+        const int p = tid / (16 * MG), i = (tid / MG) % 16, g = tid % MG;
+        const int l = l0 + jb + 2 * i + p;
+        const int b = mg * MG + g;
+        if (l <= lmax && b < batch) {
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s4[c] += red[(w * 2 + p) * 256 + i * 16 + 4 * g + c];
+          double2* pb = p_p2 + (long long)b * 2 * A.K;
+          pb[cb + l] = make_double2(s4[0], s4[1]);
+          pb[A.K + cb + l] = make_double2(s4[2], s4[3]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
+__global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __restrict__ V,
+                                                   double2* __restrict__ part, int batch) {
+  constexpr int MG = 8;
+  __shared__ double s_al[CH];
+  __shared__ float tiles[4][TL][TS];  // [wave][l][ring] (one sign)
+  __shared__ double redbuf[4][2][16][16];
+  int m, chunk;
+  if (!decode_block(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * 256 + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double s;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, s, ex);
+    mup = -s;
+  }
+  float Bv[1][2][16];
+  {
+    const int g = (lane & 15) >> 1, c = lane & 1, b = mg * MG + g;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
+      double vn = 0.0, vs = 0.0;
+      if (tr < A.T && b < batch) {
+        const double2* Vb = V + (long long)b * A.nrings * ringM;
+        const double2 n = Vb[(long long)tr * ringM + m];
+        vn = c ? n.y : n.x;
+        if (tr < A.T - 1) {
+          const double2 s = Vb[(long long)(4 * A.nside - 2 - tr) * ringM + m];
+          vs = c ? s.y : s.x;
+        }
+      }
+      Bv[0][0][q] = (float)(vn + vs);
+      Bv[0][1][q] = (float)(vn - vs);
+    }
+  }
+  float* mytile = &tiles[wave][0][0];
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    __syncthreads();
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const int anydeep = __ballot(valid && ex < 0) != 0ull;
+        const double f = (ex == 0 && valid) ? 1.0 : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double c = s_al[jb + j0 + jj] * x;
+          const double mn = fma(c, mu, -mup);
+          mup = mu;
+          mu = mn;
+          mytile[(j0 + jj) * TS + lane] = (float)(mn * f);
+        }
+        if (anydeep) rescale1(mu, mup, ex);
+      }
+      wave_sync();
+      floatx4 D[2];
+      adj_tile_mfma<1>(mytile, lane, Bv, D);
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) redbuf[wave][p][4 * (lane >> 4) + r][lane & 15] = (double)D[p][r];
+      __syncthreads();
+      if (tid < 2 * 16 * MG) {
+        const int p = tid / (16 * MG), i = (tid / MG) % 16, g = tid % MG;
+        const int l = l0 + jb + 2 * i + p;
+        const int b = mg * MG + g;
+        if (l <= lmax && b < batch) {
+          double re = 0.0, im = 0.0;
+          for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            re += redbuf[w][p][i][2 * g];
+            im += redbuf[w][p][i][2 * g + 1];
+          }
+          part[((long long)chunk * batch + b) * A.K + cb + l] = make_double2(re, im);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+RecArgs make_args_mixed(const jhtx_plan* p) {
+  RecArgs a;
+  a.alpha = p->rec.alpha;
+  a.gamma = p->rec.gamma;
+  a.G = p->rec.G;
+  a.x = p->rec.x;
+  a.lsin = p->rec.lsin;
+  a.lcos = p->rec.lcos;
+  a.seed = p->rec.seed;
+  a.lmax = p->lmax;
+  a.M = p->M;
+  a.T = p->T;
+  a.nside = p->nside;
+  a.nrings = p->nrings;
+  a.nchunk = (p->T + 255) / 256;
+  a.K = p->K;
+  return a;
+}
+
+int mixed_group(int spin, bool adjoint) { return (spin == 0) ? 8 : 4; }
+
+hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+                              hipStream_t s) {
+  const RecArgs a = make_args_mixed(p);
+  const int Mpad = (p->M + 7) & ~7;
+  const int MG = mixed_group(p->spin, false);
+  dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
+  if (p->spin == 0)
+    hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else
+    hipLaunchKernelGGL((k_synth_mx_s2<4>), grid, dim3(256), 0, s, a, alm, F, batch);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
+                            hipStream_t s) {
+  const RecArgs a = make_args_mixed(p);
+  const int Mpad = (p->M + 7) & ~7;
+  const int MG = mixed_group(p->spin, true);
+  dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
+  if (p->spin == 0)
+    hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
+  else
+    hipLaunchKernelGGL(k_adj_mx_s2, grid, dim3(256), 0, s, a, V, part, batch);
+  return hipGetLastError();
+}
+
+}  // namespace jhtx

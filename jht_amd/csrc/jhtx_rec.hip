@@ -738,8 +738,13 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, p->adj_nchunk,
-                     part, alm);
+  return launch_adj_finish(p, part, alm, batch, p->adj_nchunk, s);
+}
+
+hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
+                             int nchunk, hipStream_t s) {
+  const RecArgs a = make_args(p, true);
+  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
   return hipGetLastError();
 }
 

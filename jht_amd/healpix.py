@@ -245,18 +245,27 @@ def _as_device(x, dtype) -> torch.Tensor:
     return t.contiguous()
 
 
-def _synth_raw(alm: torch.Tensor, nside: int, lmax: int, spin: int) -> torch.Tensor:
+_PRECISIONS = {"f64": _lib.PREC_F64, "mixed": _lib.PREC_MIXED}
+
+
+def _prec(precision) -> int:
+    if precision not in _PRECISIONS:
+        raise ValueError(f"unknown precision {precision!r}; expected one of {tuple(_PRECISIONS)}")
+    return _PRECISIONS[precision]
+
+
+def _synth_raw(alm: torch.Tensor, nside: int, lmax: int, spin: int, precision: str = "f64") -> torch.Tensor:
     """alm (B, [2,] K) complex128 on GPU -> (B, [2,] npix) float64."""
     B = alm.shape[0]
-    plan = get_plan(nside, lmax, spin, B, device=alm.device)
+    plan = get_plan(nside, lmax, spin, B, precision=_prec(precision), device=alm.device)
     shape = (B, plan.npix) if spin == 0 else (B, 2, plan.npix)
     out = torch.empty(shape, dtype=torch.float64, device=alm.device)
     return plan.synth(alm, out, B)
 
 
-def _adjoint_raw(maps: torch.Tensor, nside: int, lmax: int, spin: int) -> torch.Tensor:
+def _adjoint_raw(maps: torch.Tensor, nside: int, lmax: int, spin: int, precision: str = "f64") -> torch.Tensor:
     B = maps.shape[0]
-    plan = get_plan(nside, lmax, spin, B, device=maps.device)
+    plan = get_plan(nside, lmax, spin, B, precision=_prec(precision), device=maps.device)
     shape = (B, plan.K) if spin == 0 else (B, 2, plan.K)
     out = torch.empty(shape, dtype=torch.complex128, device=maps.device)
     return plan.adjoint(maps, out, B)
@@ -289,28 +298,28 @@ def _adjoint_backward_alm(h: torch.Tensor, lmax: int) -> torch.Tensor:
 
 class _SynthesisFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, alm, nside, lmax, spin):
-        ctx.geom = (nside, lmax, spin)
-        return _synth_raw(alm, nside, lmax, spin)
+    def forward(ctx, alm, nside, lmax, spin, precision="f64"):
+        ctx.geom = (nside, lmax, spin, precision)
+        return _synth_raw(alm, nside, lmax, spin, precision)
 
     @staticmethod
     def backward(ctx, g):
-        nside, lmax, spin = ctx.geom
-        b = _AdjointFn.apply(g.contiguous(), nside, lmax, spin)
-        return _synth_backward(b, lmax, spin), None, None, None
+        nside, lmax, spin, precision = ctx.geom
+        b = _AdjointFn.apply(g.contiguous(), nside, lmax, spin, precision)
+        return _synth_backward(b, lmax, spin), None, None, None, None
 
 
 class _AdjointFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, maps, nside, lmax, spin):
-        ctx.geom = (nside, lmax, spin)
-        return _adjoint_raw(maps, nside, lmax, spin)
+    def forward(ctx, maps, nside, lmax, spin, precision="f64"):
+        ctx.geom = (nside, lmax, spin, precision)
+        return _adjoint_raw(maps, nside, lmax, spin, precision)
 
     @staticmethod
     def backward(ctx, h):
-        nside, lmax, spin = ctx.geom
+        nside, lmax, spin, precision = ctx.geom
         a = _adjoint_backward_alm(h.contiguous(), lmax)
-        return _SynthesisFn.apply(a.contiguous(), nside, lmax, spin), None, None, None
+        return _SynthesisFn.apply(a.contiguous(), nside, lmax, spin, precision), None, None, None, None
 
 
 _X64_WARNED = [False]
@@ -378,19 +387,23 @@ def adjoint_synthesis(m, nside: int, lmax: int, spin: int = 0) -> torch.Tensor:
     return out.to(torch.complex64) if single else out
 
 
-def synthesis_batch(alm, nside: int, lmax: int, spin: int = 0) -> torch.Tensor:
-    """Batched :func:`synthesis`: ``(B, K)`` / ``(B, 2, K)`` -> ``(B, npix)`` / ``(B, 2, npix)``."""
+def synthesis_batch(alm, nside: int, lmax: int, spin: int = 0, precision: str = "f64") -> torch.Tensor:
+    """Batched :func:`synthesis`: ``(B, K)`` / ``(B, 2, K)`` -> ``(B, npix)`` / ``(B, 2, npix)``.
+
+    ``precision="mixed"`` keeps the l-recursion in FP64 and runs the contraction over l on
+    the FP32 matrix cores for groups of maps (relative error ~1e-6 against the FP64 path).
+    """
     expect = (alm_size(lmax),) if spin == 0 else (2, alm_size(lmax))
     if tuple(alm.shape[1:]) != expect or len(alm.shape) != len(expect) + 1:
         raise ValueError(f"alm shape {tuple(alm.shape)} != expected (B, *{expect})")
     if spin not in (0, 2):
         raise NotImplementedError(f"spin={spin} unsupported (only 0 and 2)")
     a, _ = _prep(alm, True)
-    return _SynthesisFn.apply(a, nside, lmax, spin)
+    return _SynthesisFn.apply(a, nside, lmax, spin, precision)
 
 
-def adjoint_synthesis_batch(m, nside: int, lmax: int, spin: int = 0) -> torch.Tensor:
-    """Batched :func:`adjoint_synthesis`."""
+def adjoint_synthesis_batch(m, nside: int, lmax: int, spin: int = 0, precision: str = "f64") -> torch.Tensor:
+    """Batched :func:`adjoint_synthesis` (``precision`` as in :func:`synthesis_batch`)."""
     npix = 12 * nside * nside
     expect = (npix,) if spin == 0 else (2, npix)
     if tuple(m.shape[1:]) != expect or len(m.shape) != len(expect) + 1:
@@ -398,4 +411,4 @@ def adjoint_synthesis_batch(m, nside: int, lmax: int, spin: int = 0) -> torch.Te
     if spin not in (0, 2):
         raise NotImplementedError(f"spin={spin} unsupported (only 0 and 2)")
     v, _ = _prep(m, False)
-    return _AdjointFn.apply(v, nside, lmax, spin)
+    return _AdjointFn.apply(v, nside, lmax, spin, precision)

@@ -160,3 +160,32 @@ def test_lane_variants(spin, synth_r, adj_r, block, monkeypatch):
     e2 = _rel(b, O.adjoint_synthesis(v, nside, lmax, spin))
     print(f"variant spin={spin} R={synth_r}/{adj_r} block={block}: synth {e1:.2e} adj {e2:.2e}", flush=True)
     assert e1 <= TOL and e2 <= TOL
+
+
+MIXED_TOL = 1e-5  # the mixed-precision contract (FP64 recursion, FP32 MFMA contraction)
+
+
+@pytest.mark.parametrize("spin,batch", [(0, 5), (0, 8), (0, 9), (2, 3), (2, 4), (2, 6)])
+@pytest.mark.parametrize("nside,lmax", [(16, 32), (64, 128), (128, 256)])
+def test_mixed_matches_f64(nside, lmax, spin, batch):
+    """Batched mixed precision against the FP64 path (and the FP64 path against the
+    oracle for the first map), synthesis and adjoint."""
+    j = _jht()
+    rng = np.random.default_rng(1000 + nside + spin * 7 + batch)
+    a = np.stack([O.gaussian_alm(lmax, rng) if spin == 0 else
+                  np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                  for _ in range(batch)])
+    at = torch.tensor(a, device="cuda")
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        s64 = j.synthesis_batch(at, nside, lmax, spin, precision="f64")
+        smx = j.synthesis_batch(at, nside, lmax, spin, precision="mixed")
+        b64 = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="f64")
+        bmx = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="mixed")
+    s64, smx, b64, bmx = (t.cpu().numpy() for t in (s64, smx, b64, bmx))
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"mixed nside={nside} lmax={lmax} spin={spin} B={batch}: synth {es:.2e} adj {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
+    if nside <= 64:
+        assert _rel(s64[0], O.synthesis(a[0], nside, lmax, spin)) <= TOL
