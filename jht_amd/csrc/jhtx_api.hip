@@ -313,6 +313,10 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
     p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
     p->adj_R = pick("JHTX_ADJ_R", 2, 1, 512, s2_R);
   }
+  p->mx_synth_group = 4;
+  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
+    if (atoi(v) == 8) p->mx_synth_group = 8;
+  }
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);

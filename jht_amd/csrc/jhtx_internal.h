@@ -77,6 +77,7 @@ struct jhtx_plan {
   int rec_block;  // threads per recursion work-group (synthesis)
   int rec_R;      // rings per lane (synthesis)
   int adj_block, adj_R, adj_nchunk;  // the same for the adjoint recursion
+  int mx_synth_group;  // maps per work-group of the mixed spin-2 synthesis (4 or 8)
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)

@@ -73,14 +73,46 @@ __device__ __forceinline__ void wave_sync() {
 enum { FULL = 0, MASKED = 1, RECONLY = 2 };
 
 // ---------------------------------------------------------------------------------- //
-// spin 2 synthesis, MG maps per work-group (grid.y = map group)
+// spin 2 synthesis, MG = 4 NB maps per work-group (grid.y = map group)
 // ---------------------------------------------------------------------------------- //
-template <int MG>
+// Eight l steps of the +-2 recursions and their rank-1 MFMA updates.  The operands of the
+// eight steps are already in registers and MODE is fixed for the group, so the body is
+// straight-line code: no LDS wait or branch between the steps.
+template <int MODE, int NB>
+__device__ __forceinline__ void s2_steps8(const double2 (&ag)[8], const floatx4 (&bq)[8][NB], double x,
+                                          double fp, double fm, double& pp, double& ppp, double& pm,
+                                          double& pmp, floatx4 (&accP)[2][4 * NB],
+                                          floatx4 (&accM)[2][4 * NB]) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const double cp = fma(ag[jj].x, x, ag[jj].y);
+    const double cm = fma(ag[jj].x, x, -ag[jj].y);
+    const double np = fma(cp, pp, -ppp);
+    const double nm = fma(cm, pm, -pmp);
+    ppp = pp;
+    pp = np;
+    pmp = pm;
+    pm = nm;
+    if (MODE != RECONLY) {
+      // mask in FP64: a lane still below 2^-256 may hold a mantissa above the FP32 range
+      const float ap = (float)((MODE == MASKED) ? np * fp : np);
+      const float am = (float)((MODE == MASKED) ? nm * fm : nm);
+#pragma unroll
+      for (int g = 0; g < 4 * NB; ++g) {
+        accP[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(ap, bq[jj][g >> 2][g & 3], accP[jj & 1][g], 0, 0, 0);
+        accM[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(am, bq[jj][g >> 2][g & 3], accM[jj & 1][g], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int NB>
 __global__ __launch_bounds__(256) void k_synth_mx_s2(RecArgs A, const double2* __restrict__ alm,
                                                      double2* __restrict__ F, int batch) {
+  constexpr int MG = 4 * NB;
   __shared__ double2 s_ag[CH];
-  __shared__ float s_a[CH][4][MG];       // [l][component: aE.re aE.im aB.re aB.im][map]
-  __shared__ float s_ep[4][2][2][64][4];  // [wave][class][sign][ring][component]
+  __shared__ __align__(16) float s_a[CH][4][MG];  // [l][component: aE.re aE.im aB.re aB.im][map]
+  __shared__ float s_ep[4][2][2][64][4];          // [wave][class][sign][ring][component]
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -128,37 +160,40 @@ __global__ __launch_bounds__(256) void k_synth_mx_s2(RecArgs A, const double2* _
     }
     __syncthreads();
     const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
-    for (int j0 = 0; j0 < nst; j0 += 8) {
-      const bool deep = valid && ((ep < 0) || (em < 0));
-      const bool alldeep = !valid || ((ep < 0) && (em < 0));
-      int mode = FULL;
-      if (__ballot(deep) != 0ull) mode = (__ballot(!alldeep) == 0ull) ? RECONLY : MASKED;
-      const double fp = (ep == 0) ? 1.0 : 0.0, fm = (em == 0) ? 1.0 : 0.0;
+    int j0 = 0;
+    // Below 2^-256 on every lane: recursion only.  A lane's exponent never decreases, so
+    // once a wave leaves this phase it does not return to it.
+    while (j0 < nst && __ballot(valid && !((ep < 0) && (em < 0))) == 0ull) {
+      double2 ag[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[j0 + jj];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        const int j = j0 + jj;
-        const double2 ag = s_ag[j];
-        const double cp = fma(ag.x, x, ag.y);
-        const double cm = fma(ag.x, x, -ag.y);
-        const double np = fma(cp, pp, -ppp);
-        const double nm = fma(cm, pm, -pmp);
+        const double np = fma(fma(ag[jj].x, x, ag[jj].y), pp, -ppp);
+        const double nm = fma(fma(ag[jj].x, x, -ag[jj].y), pm, -pmp);
         ppp = pp;
         pp = np;
         pmp = pm;
         pm = nm;
-        if (mode != RECONLY) {
-          // mask in FP64: a lane still below 2^-256 may hold a mantissa above the FP32 range
-          const float ap = (float)((mode == MASKED) ? np * fp : np);
-          const float am = (float)((mode == MASKED) ? nm * fm : nm);
-          const float* bq = &s_a[j][lane & 3][0];
-#pragma unroll
-          for (int g = 0; g < MG; ++g) {
-            accP[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(ap, bq[g], accP[jj & 1][g], 0, 0, 0);
-            accM[jj & 1][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(am, bq[g], accM[jj & 1][g], 0, 0, 0);
-          }
-        }
       }
-      if (mode != FULL) {
+      rescale1(pp, ppp, ep);
+      rescale1(pm, pmp, em);
+      j0 += 8;
+    }
+    for (; j0 < nst; j0 += 8) {
+      const int anydeep = __ballot(valid && ((ep < 0) || (em < 0))) != 0ull;
+      const double fp = (ep == 0) ? 1.0 : 0.0, fm = (em == 0) ? 1.0 : 0.0;
+      double2 ag[8];
+      floatx4 bq[8][NB];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        ag[jj] = s_ag[j0 + jj];
+#pragma unroll
+        for (int n = 0; n < NB; ++n) bq[jj][n] = *reinterpret_cast<const floatx4*>(&s_a[j0 + jj][lane & 3][4 * n]);
+      }
+      // masks are exact 1/0 factors (1 on every lane outside the deep region)
+      s2_steps8<MASKED, NB>(ag, bq, x, fp, fm, pp, ppp, pm, pmp, accP, accM);
+      if (anydeep) {
         rescale1(pp, ppp, ep);
         rescale1(pm, pmp, em);
       }
@@ -222,7 +257,8 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
                                                      double2* __restrict__ F, int batch) {
   constexpr int NQ = MG / 2;
   __shared__ double s_al[CH];
-  __shared__ float s_a[CH][4][NQ];    // [l][lane%4][q]
+  static_assert(NQ % 4 == 0, "operands are read as float4");
+  __shared__ __align__(16) float s_a[CH][4][NQ];  // [l][lane%4][q]
   __shared__ float s_ep[4][2][64][4];  // [wave][class][ring][lane%4]
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
@@ -265,28 +301,44 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
     }
     __syncthreads();
     const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
-    for (int j0 = 0; j0 < nst; j0 += 8) {
-      const bool deep = valid && ex < 0;
-      const bool alldeep = !valid || ex < 0;
-      int mode = FULL;
-      if (__ballot(deep) != 0ull) mode = (__ballot(!alldeep) == 0ull) ? RECONLY : MASKED;
-      const double f = (ex == 0) ? 1.0 : 0.0;
+    int j0 = 0;
+    // every lane below 2^-256: recursion only (a lane's exponent never decreases)
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+      double al[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) al[jj] = s_al[j0 + jj];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        const int j = j0 + jj;
-        const double c = s_al[j] * x;
-        const double mn = fma(c, mu, -mup);
+        const double mn = fma(al[jj] * x, mu, -mup);
         mup = mu;
         mu = mn;
-        if (mode != RECONLY) {
-          const float a = (float)((mode == MASKED) ? mn * f : mn);
-          const float* bq = &s_a[j][lane & 3][0];
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-            acc[jj & 1][q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bq[q], acc[jj & 1][q], 0, 0, 0);
-        }
       }
-      if (mode != FULL) rescale1(mu, mup, ex);
+      rescale1(mu, mup, ex);
+      j0 += 8;
+    }
+    for (; j0 < nst; j0 += 8) {
+      const int anydeep = __ballot(valid && ex < 0) != 0ull;
+      const double f = (ex == 0) ? 1.0 : 0.0;
+      double al[8];
+      floatx4 bq[8][NQ / 4];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        al[jj] = s_al[j0 + jj];
+#pragma unroll
+        for (int n = 0; n < NQ / 4; ++n) bq[jj][n] = *reinterpret_cast<const floatx4*>(&s_a[j0 + jj][lane & 3][4 * n]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double mn = fma(al[jj] * x, mu, -mup);
+        mup = mu;
+        mu = mn;
+        // the 1/0 mask in FP64 first: a lane below 2^-256 may hold a mantissa above FP32 range
+        const float a = (float)(mn * f);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          acc[jj & 1][q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bq[jj][q >> 2][q & 3], acc[jj & 1][q], 0, 0, 0);
+      }
+      if (anydeep) rescale1(mu, mup, ex);
     }
   }
   const long long ringM = A.M;
@@ -593,18 +645,23 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
   return a;
 }
 
-int mixed_group(int spin, bool adjoint) { return (spin == 0) ? 8 : 4; }
+int mixed_group(const jhtx_plan* p, bool adjoint) {
+  if (p->spin == 0) return 8;
+  return adjoint ? 4 : p->mx_synth_group;
+}
 
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                               hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 7) & ~7;
-  const int MG = mixed_group(p->spin, false);
+  const int MG = mixed_group(p, false);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else if (MG == 8)
+    hipLaunchKernelGGL((k_synth_mx_s2<2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else
-    hipLaunchKernelGGL((k_synth_mx_s2<4>), grid, dim3(256), 0, s, a, alm, F, batch);
+    hipLaunchKernelGGL((k_synth_mx_s2<1>), grid, dim3(256), 0, s, a, alm, F, batch);
   return hipGetLastError();
 }
 
@@ -612,7 +669,7 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part,
                             hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 7) & ~7;
-  const int MG = mixed_group(p->spin, true);
+  const int MG = mixed_group(p, true);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);

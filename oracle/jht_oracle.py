@@ -22,7 +22,7 @@ Plus the chirp-z (Bluestein) cap path of ``src/jht/_azimuth.py:95-232`` and the
 analysis / ring-weight drivers of ``src/jht/_analysis.py`` / ``weights.py``.
 
 How each function is pinned (beyond fused == eager) is listed in the test
-files ``tests/test_oracle_*.py``: closed forms for l <= 2, scipy
+files ``tests/test_oracle.py``: closed forms for l <= 2, scipy
 ``sph_harm_y`` to l = 512, the explicit Wigner-d sum, mpmath, brute-force dense
 Y synthesis at small nside, the adjoint identity, and monopole/constant-map
 closed forms.
