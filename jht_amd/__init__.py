@@ -21,4 +21,22 @@ from .healpix import (  # noqa: F401
     synthesis_batch,
 )
 
+from .callers import (  # noqa: F401
+    alm_to_real,
+    analysis,
+    analysis_real,
+    bandpower,
+    bare_analysis,
+    constrained_realization,
+    deconvolve,
+    map2alm,
+    n_dof,
+    pixel_weights,
+    pseudo_alm,
+    real_to_alm,
+    ring_weights,
+    synthesis_real,
+    wiener,
+)
+
 __version__ = "0.1.0"
