@@ -184,9 +184,18 @@ def main():
         torch.cuda.synchronize()
         e_s = float((maps[0] - m64[0]).abs().max() / m64[0].abs().max())
         e_a = float((bmx[0] - b64[0]).abs().max() / b64[0].abs().max())
+        # adjoint error per l-band: max|b - b64| / max|b64| over the band's coefficients
+        ell = torch.as_tensor(np.concatenate([np.arange(mm, lmax + 1) for mm in range(lmax + 1)]), device=dev)
+        edges = [e for e in (0, 64, 256, 1024, 2048, 4096) if e <= lmax] + [lmax + 1]
+        bands = {}
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            sel = (ell >= lo) & (ell < hi)
+            ref = b64[0][..., sel].abs().max()
+            if float(ref) > 0:
+                bands[f"{lo}-{hi - 1}"] = float((bmx[0][..., sel] - b64[0][..., sel]).abs().max() / ref)
         v64 = 1.0 / dt64 * max(2, args.steps // 2)
         out_extra = {"f64_maps_per_s": round(v64, 4), "speedup_vs_f64": round(value / v64, 3),
-                     "rel_err_vs_f64": {"synthesis": e_s, "adjoint": e_a},
+                     "rel_err_vs_f64": {"synthesis": e_s, "adjoint": e_a, "adjoint_by_l_band": bands},
                      "f64_stages_ms": {k: round(v[0], 3) for k, v in per64.items()}}
         del p64, m64, b64, bmx, mx_in
 

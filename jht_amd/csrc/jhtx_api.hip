@@ -313,10 +313,20 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
     p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
     p->adj_R = pick("JHTX_ADJ_R", 2, 1, 512, s2_R);
   }
-  p->mx_synth_group = 4;
-  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
-    if (atoi(v) == 8) p->mx_synth_group = 8;
-  }
+  // mixed spin-2 kernels: maps per recursion (4 or 8) and the register budget of the
+  // 8-map variants (1 or 2 work-groups per CU); defaults from sweeps at nside 2048 / lmax 4096
+  auto pick2 = [](const char* env, int dflt, int a, int b) {
+    int r = dflt;
+    if (const char* v = getenv(env)) {
+      const int want = atoi(v);
+      if (want == a || want == b) r = want;
+    }
+    return r;
+  };
+  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", 8, 4, 8);
+  p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
+  p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", 8, 4, 8);
+  p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);

@@ -78,6 +78,9 @@ struct jhtx_plan {
   int rec_R;      // rings per lane (synthesis)
   int adj_block, adj_R, adj_nchunk;  // the same for the adjoint recursion
   int mx_synth_group;  // maps per work-group of the mixed spin-2 synthesis (4 or 8)
+  int mx_synth_occ;    // register budget of the 8-map synthesis: 1 or 2 work-groups per CU
+  int mx_adj_group;    // the same for the mixed spin-2 adjoint
+  int mx_adj_occ;      // register budget of the 8-map adjoint: 1 or 2 work-groups per CU
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)

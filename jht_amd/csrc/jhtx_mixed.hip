@@ -106,8 +106,8 @@ __device__ __forceinline__ void s2_steps8(const double2 (&ag)[8], const floatx4 
   }
 }
 
-template <int NB>
-__global__ __launch_bounds__(256) void k_synth_mx_s2(RecArgs A, const double2* __restrict__ alm,
+template <int NB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const double2* __restrict__ alm,
                                                      double2* __restrict__ F, int batch) {
   constexpr int MG = 4 * NB;
   __shared__ double2 s_ag[CH];
@@ -396,12 +396,40 @@ __device__ __forceinline__ void adj_tile_mfma(const float* tile, int lane, const
   }
 }
 
-// spin 2: columns j = 4 g + c, c in {p2.re, p2.im, m2.re, m2.im}, g map in group (MG = 4)
-__global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
+// spin 2: column block cb2, column j = 4 g + c, c in {p2.re, p2.im, m2.re, m2.im}, map
+// 4 cb2 + g of the group; MG = 4 NCB maps share one recursion
+template <int NCB>
+__device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, const float (&Bv)[NCB][2][2][16],
+                                                 floatx4 (&D)[NCB][2]) {
+  const int col = lane >> 4;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) D[c][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int row = 2 * (lane & 15) + p;
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      const float* ts = tile + sg * TL * TS + row * TS + col;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const float a = ts[4 * q];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c)
+          D[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[c][sg][p][q], D[c][p], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// MINB: work-groups per CU the register allocation must allow (2 caps it at 256 VGPRs)
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
                                                    double2* __restrict__ part, int batch) {
-  constexpr int MG = 4;
+  constexpr int MG = 4 * NCB;
   __shared__ double2 s_ag[CH];
   __shared__ float tiles[4][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
+  static_assert(sizeof(double) * 4 * 2 * 16 * 16 * NCB <= sizeof(float) * 4 * 2 * TL * TS,
+                "the FP64 reduction buffer aliases the tiles");
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -425,41 +453,46 @@ __global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __r
     seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
     pmp = -s;
   }
-  // B operands: ring 4q + lane/16 of this wave, column lane%16 -> map g, component c
-  float Bv[2][2][16];
+  // B operands: ring 4q + lane/16 of this wave, column lane%16 -> map 4 c + g, component cc
+  float Bv[NCB][2][2][16];
   {
-    const int g = (lane & 15) >> 2, c = lane & 3, b = mg * MG + g;
     const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;  // (-1)^(l-m) at even (l - lmin)
+    const int g = (lane & 15) >> 2, cc = lane & 3;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
-      double vn = 0.0, vs = 0.0, wn = 0.0, ws = 0.0;  // sign + : (n, s·south) ; sign - : (s·south, n)
-      if (tr < A.T && b < batch) {
-        const double2* Vp = V + ((long long)b * 2 + 0) * A.nrings * ringM;
-        const double2* Vm = V + ((long long)b * 2 + 1) * A.nrings * ringM;
-        const long long rn = (long long)tr * ringM + m;
-        const double2 vpn = Vp[rn], vmn = Vm[rn];
-        double2 vps = make_double2(0.0, 0.0), vms = vps;
-        if (tr < A.T - 1) {
-          const long long rs = (long long)(4 * A.nside - 2 - tr) * ringM + m;
-          vps = Vp[rs];
-          vms = Vm[rs];
+    for (int c = 0; c < NCB; ++c) {
+      const int b = mg * MG + 4 * c + g;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
+        double vn = 0.0, vs = 0.0, wn = 0.0, ws = 0.0;  // sign + : (n, s south) ; sign - : (s south, n)
+        if (tr < A.T && b < batch) {
+          const double2* Vp = V + ((long long)b * 2 + 0) * A.nrings * ringM;
+          const double2* Vm = V + ((long long)b * 2 + 1) * A.nrings * ringM;
+          const long long rn = (long long)tr * ringM + m;
+          const double2 vpn = Vp[rn], vmn = Vm[rn];
+          double2 vps = make_double2(0.0, 0.0), vms = vps;
+          if (tr < A.T - 1) {
+            const long long rs = (long long)(4 * A.nside - 2 - tr) * ringM + m;
+            vps = Vp[rs];
+            vms = Vm[rs];
+          }
+          // mu+: p2 <- Vpn, m2 <- s Vms ; mu-: p2 <- s Vps, m2 <- Vmn
+          vn = (cc == 0) ? vpn.x : (cc == 1) ? vpn.y : 0.0;
+          vs = (cc == 2) ? vms.x : (cc == 3) ? vms.y : 0.0;
+          wn = (cc == 2) ? vmn.x : (cc == 3) ? vmn.y : 0.0;
+          ws = (cc == 0) ? vps.x : (cc == 1) ? vps.y : 0.0;
         }
-        // mu+: p2 <- Vpn, m2 <- s Vms ; mu-: p2 <- s Vps, m2 <- Vmn
-        vn = (c == 0) ? vpn.x : (c == 1) ? vpn.y : 0.0;
-        vs = (c == 2) ? vms.x : (c == 3) ? vms.y : 0.0;
-        wn = (c == 2) ? vmn.x : (c == 3) ? vmn.y : 0.0;
-        ws = (c == 0) ? vps.x : (c == 1) ? vps.y : 0.0;
+        Bv[c][0][0][q] = (float)(vn + par0 * vs);
+        Bv[c][0][1][q] = (float)(vn - par0 * vs);
+        Bv[c][1][0][q] = (float)(wn + par0 * ws);
+        Bv[c][1][1][q] = (float)(wn - par0 * ws);
       }
-      Bv[0][0][q] = (float)(vn + par0 * vs);
-      Bv[0][1][q] = (float)(vn - par0 * vs);
-      Bv[1][0][q] = (float)(wn + par0 * ws);
-      Bv[1][1][q] = (float)(wn - par0 * ws);
     }
   }
   double2* p_p2 = part + (((long long)chunk * batch) * 2) * A.K;  // offset per map below
   float* mytile = &tiles[wave][0][0][0];
-  double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16][16] FP64
+  double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16 l][16 NCB cols] FP64
+  constexpr int NC = 16 * NCB;
   for (int l0 = lmin; l0 <= lmax; l0 += CH) {
     __syncthreads();
     if (tid < CH) {
@@ -476,12 +509,13 @@ __global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __r
         const bool deep = valid && ((ep < 0) || (em < 0));
         const int anydeep = __ballot(deep) != 0ull;
         const double fp = (ep == 0 && valid) ? 1.0 : 0.0, fm = (em == 0 && valid) ? 1.0 : 0.0;
+        double2 ag[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[jb + j0 + jj];
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
-          const int j = jb + j0 + jj;
-          const double2 ag = s_ag[j];
-          const double cp = fma(ag.x, x, ag.y);
-          const double cm = fma(ag.x, x, -ag.y);
+          const double cp = fma(ag[jj].x, x, ag[jj].y);
+          const double cm = fma(ag[jj].x, x, -ag[jj].y);
           const double np = fma(cp, pp, -ppp);
           const double nm = fma(cm, pm, -pmp);
           ppp = pp;
@@ -497,18 +531,20 @@ __global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __r
         }
       }
       wave_sync();
-      floatx4 D[2];
-      adj_tile_mfma<2>(mytile, lane, Bv, D);
+      floatx4 D[NCB][2];
+      adj_tile_mfma_s2<NCB>(mytile, lane, Bv, D);
       // publish per-wave sums (FP64) in the tile area after every wave has read its tile
       __syncthreads();
-      double* myred = red + wave * 2 * 256;
+      double* myred = red + wave * 2 * 16 * NC;
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < NCB; ++c)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * (lane >> 4) + r, jc = lane & 15;
-          myred[p * 256 + i * 16 + jc] = (double)D[p][r];
-        }
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 4 * (lane >> 4) + r, jc = lane & 15;
+            myred[(p * 16 + i) * NC + 16 * c + jc] = (double)D[c][p][r];
+          }
       __syncthreads();
       // thread -> (p, i, g): sums 4 waves for the 4 components of map g at l = l0+jb+2i+p
       if (tid < 2 * 16 * MG) {
@@ -517,9 +553,9 @@ __global__ __launch_bounds__(256) void k_adj_mx_s2(RecArgs A, const double2* __r
         const int b = mg * MG + g;
         if (l <= lmax && b < batch) {
           double s4[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+          for (int w = 0; w < 4; ++w)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) s4[c] += red[(w * 2 + p) * 256 + i * 16 + 4 * g + c];
+            for (int c = 0; c < 4; ++c) s4[c] += red[((w * 2 + p) * 16 + i) * NC + 4 * g + c];
           double2* pb = p_p2 + (long long)b * 2 * A.K;
           pb[cb + l] = make_double2(s4[0], s4[1]);
           pb[A.K + cb + l] = make_double2(s4[2], s4[3]);
@@ -647,7 +683,7 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
 
 int mixed_group(const jhtx_plan* p, bool adjoint) {
   if (p->spin == 0) return 8;
-  return adjoint ? 4 : p->mx_synth_group;
+  return adjoint ? p->mx_adj_group : p->mx_synth_group;
 }
 
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
@@ -658,10 +694,12 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else if (MG == 8 && p->mx_synth_occ == 2)
+    hipLaunchKernelGGL((k_synth_mx_s2<2, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8)
-    hipLaunchKernelGGL((k_synth_mx_s2<2>), grid, dim3(256), 0, s, a, alm, F, batch);
+    hipLaunchKernelGGL((k_synth_mx_s2<2, 1>), grid, dim3(256), 0, s, a, alm, F, batch);
   else
-    hipLaunchKernelGGL((k_synth_mx_s2<1>), grid, dim3(256), 0, s, a, alm, F, batch);
+    hipLaunchKernelGGL((k_synth_mx_s2<1, 1>), grid, dim3(256), 0, s, a, alm, F, batch);
   return hipGetLastError();
 }
 
@@ -673,8 +711,12 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part,
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
+  else if (MG == 8 && p->mx_adj_occ == 2)
+    hipLaunchKernelGGL((k_adj_mx_s2<2, 2>), grid, dim3(256), 0, s, a, V, part, batch);
+  else if (MG == 8)
+    hipLaunchKernelGGL((k_adj_mx_s2<2, 1>), grid, dim3(256), 0, s, a, V, part, batch);
   else
-    hipLaunchKernelGGL(k_adj_mx_s2, grid, dim3(256), 0, s, a, V, part, batch);
+    hipLaunchKernelGGL((k_adj_mx_s2<1, 1>), grid, dim3(256), 0, s, a, V, part, batch);
   return hipGetLastError();
 }
 

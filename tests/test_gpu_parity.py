@@ -189,3 +189,37 @@ def test_mixed_matches_f64(nside, lmax, spin, batch):
     assert es <= MIXED_TOL and ea <= MIXED_TOL
     if nside <= 64:
         assert _rel(s64[0], O.synthesis(a[0], nside, lmax, spin)) <= TOL
+
+
+@pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ", [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1)])
+def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkeypatch):
+    """The 8-maps-per-recursion spin-2 kernels (both register budgets), forced through the
+    plan's tuning overrides, against the FP64 path; batch 11 leaves a ragged last group."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_SYNTH_GROUP", str(synth_group))
+    monkeypatch.setenv("JHTX_MX_SYNTH_OCC", str(synth_occ))
+    monkeypatch.setenv("JHTX_MX_ADJ_GROUP", str(adj_group))
+    monkeypatch.setenv("JHTX_MX_ADJ_OCC", str(adj_occ))
+    H.clear_plans()
+    try:
+        nside, lmax, batch = 64, 128, 11
+        rng = np.random.default_rng(77 + synth_group + synth_occ + adj_group + adj_occ)
+        a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                      for _ in range(batch)])
+        at = torch.tensor(a, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, 2, precision="f64")
+            smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed")
+            b64 = j.adjoint_synthesis_batch(s64, nside, lmax, 2, precision="f64")
+            bmx = j.adjoint_synthesis_batch(s64, nside, lmax, 2, precision="mixed")
+    finally:
+        H.clear_plans()
+    s64, smx, b64, bmx = (t.cpu().numpy() for t in (s64, smx, b64, bmx))
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"mixed groups synth={synth_group}/{synth_occ} adj={adj_group}/{adj_occ}: synth {es:.2e} "
+          f"adj {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
