@@ -53,6 +53,30 @@ enum {
 int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch, int flags,
                      jhtx_plan** plan);
 
+/* Kernel shape choices of a plan (no counterpart in the reference: jht leaves tiling to
+ * XLA).  A zero field keeps the library's default; jht_amd's autotuner times candidates
+ * and passes the winner here.  Allowed values:
+ *   synth_rings, adj_rings   rings per lane of the FP64 recursion kernels: spin 0 {1,4,8},
+ *                            spin 2 {1,2,4}
+ *   block                    FP64 recursion work-group size cap {64,128,192,256}
+ *   mx_synth_group           maps per recursion, mixed spin-2 synthesis {4,8}
+ *   mx_synth_occ             its register budget in work-groups per CU {1,2}
+ *   mx_adj_group, mx_adj_occ the same for the mixed spin-2 adjoint {4,8}, {1,2}
+ *   mx_adj_wide              mixed spin-2 adjoint with 512-ring work-groups {1 on, 2 off} */
+typedef struct jhtx_tuning {
+  int synth_rings, adj_rings, block;
+  int mx_synth_group, mx_synth_occ;
+  int mx_adj_group, mx_adj_occ, mx_adj_wide;
+} jhtx_tuning;
+
+/* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An
+ * out-of-range field fails with JHTX_EINVAL. */
+int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max_batch, int flags,
+                           const jhtx_tuning* tuning, jhtx_plan** plan);
+
+/* The kernel shapes a plan uses, every field filled in. */
+int jhtx_plan_get_tuning(const jhtx_plan* plan, jhtx_tuning* tuning);
+
 /* Bytes of device workspace a call on this plan needs (for max_batch maps). */
 size_t jhtx_plan_workspace_bytes(const jhtx_plan* plan);
 

@@ -17,9 +17,12 @@ from .healpix import (  # noqa: F401
     get_azimuth_fft_mode,
     get_plan,
     set_azimuth_fft_mode,
+    set_tuning,
     synthesis,
     synthesis_batch,
 )
+from .graph import Instance, TransformGraph  # noqa: F401
+from .tune import autotune  # noqa: F401
 
 from .callers import (  # noqa: F401
     alm_to_real,

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libjhtx.so")
 # Every symbol include/jhtx.h declares.
 EXPORTS = (
     "jhtx_plan_create",
+    "jhtx_plan_create_tuned",
+    "jhtx_plan_get_tuning",
     "jhtx_plan_workspace_bytes",
     "jhtx_synth",
     "jhtx_adjoint",
@@ -34,6 +36,16 @@ JHTX_ENOMEM = -4
 PREC_F64 = 0
 PREC_MIXED = 1
 FLAG_BLUESTEIN_ALL = 1
+
+
+TUNING_FIELDS = ("synth_rings", "adj_rings", "block", "mx_synth_group", "mx_synth_occ",
+                 "mx_adj_group", "mx_adj_occ", "mx_adj_wide")
+
+
+class Tuning(ctypes.Structure):
+    """``jhtx_tuning``: kernel shape choices of a plan (0 = library default)."""
+
+    _fields_ = [(name, ctypes.c_int) for name in TUNING_FIELDS]
 
 
 class JhtxError(RuntimeError):
@@ -61,6 +73,10 @@ def load() -> ctypes.CDLL:
     vp = ctypes.c_void_p
     lib.jhtx_plan_create.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(vp)]
     lib.jhtx_plan_create.restype = ctypes.c_int
+    lib.jhtx_plan_create_tuned.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(Tuning), ctypes.POINTER(vp)]
+    lib.jhtx_plan_create_tuned.restype = ctypes.c_int
+    lib.jhtx_plan_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.jhtx_plan_get_tuning.restype = ctypes.c_int
     lib.jhtx_plan_workspace_bytes.argtypes = [vp]
     lib.jhtx_plan_workspace_bytes.restype = ctypes.c_size_t
     for name in ("jhtx_synth", "jhtx_adjoint"):

@@ -13,6 +13,8 @@
 
 #include "jhtx_internal.h"
 
+#include <initializer_list>
+
 namespace jhtx {
 
 static thread_local std::string g_err;
@@ -260,6 +262,27 @@ const char* jhtx_last_error(void) { return g_err.c_str(); }
 
 int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch, int flags,
                      jhtx_plan** out) {
+  return jhtx_plan_create_tuned(nside, lmax, spin, precision, max_batch, flags, nullptr, out);
+}
+
+int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
+  if (!p || !t) {
+    set_error("null plan or tuning pointer");
+    return JHTX_EINVAL;
+  }
+  t->synth_rings = p->rec_R;
+  t->adj_rings = p->adj_R;
+  t->block = p->block_cap;
+  t->mx_synth_group = p->mx_synth_group;
+  t->mx_synth_occ = p->mx_synth_occ;
+  t->mx_adj_group = p->mx_adj_group;
+  t->mx_adj_occ = p->mx_adj_occ;
+  t->mx_adj_wide = p->mx_adj_wide ? 1 : 2;
+  return JHTX_OK;
+}
+
+int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max_batch, int flags,
+                           const jhtx_tuning* tuning, jhtx_plan** out) {
   g_err.clear();
   if (!out) {
     set_error("null plan output pointer");
@@ -327,11 +350,43 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
   p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
   p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", 8, 4, 8);
   p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
+  p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);
     if (b == 64 || b == 128 || b == 192 || b == 256) block_cap = b;
   }
+  if (tuning) {
+    // explicit shapes override the defaults and the environment
+    auto in = [](int v, std::initializer_list<int> ok) {
+      for (int o : ok)
+        if (v == o) return true;
+      return false;
+    };
+    const jhtx_tuning& t = *tuning;
+    const bool ok =
+        (t.synth_rings == 0 || (spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
+        (t.adj_rings == 0 || (spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
+        (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
+        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8})) &&
+        (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2})) &&
+        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8})) &&
+        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2}));
+    if (!ok) {
+      delete p;
+      set_error("jhtx_tuning field out of range");
+      return JHTX_EINVAL;
+    }
+    if (t.synth_rings) p->rec_R = t.synth_rings;
+    if (t.adj_rings) p->adj_R = t.adj_rings;
+    if (t.block) block_cap = t.block;
+    if (t.mx_synth_group) p->mx_synth_group = t.mx_synth_group;
+    if (t.mx_synth_occ) p->mx_synth_occ = t.mx_synth_occ;
+    if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
+    if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
+    if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
+  }
+  p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
     const int lanes = (p->T + R - 1) / R;
     block = std::min(block_cap, std::max(64, ((lanes + 63) / 64) * 64));
@@ -341,7 +396,7 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
   shape(p->adj_R, p->adj_block, p->adj_nchunk);
   p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
   // the mixed kernels use one ring per lane and 256-lane work-groups
-  const int part_chunks = (precision == JHTX_PREC_MIXED) ? (p->T + 255) / 256 : p->adj_nchunk;
+  const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p) : p->adj_nchunk;
   p->ws_part_bytes = align256(sizeof(double2) * (size_t)part_chunks * max_batch * p->ncomp * p->K);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
   int rc = build_rec_tables(p);
@@ -394,7 +449,7 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
     if (p->precision == JHTX_PREC_MIXED) {
       e = launch_adj_mixed(p, V, part, batch, s);
       if (e == hipSuccess)
-        e = launch_adj_finish(p, part, reinterpret_cast<double2*>(alm), batch, (p->T + 255) / 256, s);
+        e = launch_adj_finish(p, part, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
     } else {
       e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
     }

@@ -77,10 +77,12 @@ struct jhtx_plan {
   int rec_block;  // threads per recursion work-group (synthesis)
   int rec_R;      // rings per lane (synthesis)
   int adj_block, adj_R, adj_nchunk;  // the same for the adjoint recursion
+  int block_cap;                     // work-group size cap of the FP64 recursion kernels
   int mx_synth_group;  // maps per work-group of the mixed spin-2 synthesis (4 or 8)
   int mx_synth_occ;    // register budget of the 8-map synthesis: 1 or 2 work-groups per CU
   int mx_adj_group;    // the same for the mixed spin-2 adjoint
   int mx_adj_occ;      // register budget of the 8-map adjoint: 1 or 2 work-groups per CU
+  int mx_adj_wide;     // 8-map adjoint with 512-ring work-groups on large grids (0 or 1)
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)
@@ -105,6 +107,8 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
                               hipStream_t s);
 hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
                             hipStream_t s);
+// ring chunks (partials per map) of the mixed adjoint
+int mixed_adj_chunks(const jhtx_plan* p);
 hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
                              hipStream_t s);
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,

@@ -421,15 +421,18 @@ __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, co
   }
 }
 
-// MINB: work-groups per CU the register allocation must allow (2 caps it at 256 VGPRs)
-template <int NCB, int MINB>
-__global__ __launch_bounds__(256, MINB) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
+// NW waves per work-group cover 64 NW rings (one partial per chunk of rings); MINB: the
+// register budget, in work-groups of this size per CU (NW * MINB = 8 caps it at 256 VGPRs)
+template <int NCB, int MINB, int NW>
+__global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
                                                    double2* __restrict__ part, int batch) {
   constexpr int MG = 4 * NCB;
+  constexpr int RINGS = 64 * NW;
   __shared__ double2 s_ag[CH];
-  __shared__ float tiles[4][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
-  static_assert(sizeof(double) * 4 * 2 * 16 * 16 * NCB <= sizeof(float) * 4 * 2 * TL * TS,
+  __shared__ float tiles[NW][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
+  static_assert(sizeof(double) * NW * 2 * 16 * 16 * NCB <= sizeof(float) * NW * 2 * TL * TS,
                 "the FP64 reduction buffer aliases the tiles");
+  static_assert(2 * 16 * MG <= RINGS, "one thread per reduced (parity, row, map)");
   int m, chunk;
   if (!decode_block(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(256, MINB) void k_adj_mx_s2(RecArgs A, const double
   if (lmin > lmax) return;
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const long long ringM = A.M;
-  const int t = chunk * 256 + tid;
+  const int t = chunk * RINGS + tid;
   const bool valid = t < A.T;
   double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
   int ep = 0, em = 0;
@@ -463,7 +466,7 @@ __global__ __launch_bounds__(256, MINB) void k_adj_mx_s2(RecArgs A, const double
       const int b = mg * MG + 4 * c + g;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
+        const int tr = chunk * RINGS + wave * 64 + 4 * q + (lane >> 4);
         double vn = 0.0, vs = 0.0, wn = 0.0, ws = 0.0;  // sign + : (n, s south) ; sign - : (s south, n)
         if (tr < A.T && b < batch) {
           const double2* Vp = V + ((long long)b * 2 + 0) * A.nrings * ringM;
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(256, MINB) void k_adj_mx_s2(RecArgs A, const double
         const int b = mg * MG + g;
         if (l <= lmax && b < batch) {
           double s4[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int w = 0; w < 4; ++w)
+          for (int w = 0; w < NW; ++w)
 #pragma unroll
             for (int c = 0; c < 4; ++c) s4[c] += red[((w * 2 + p) * 16 + i) * NC + 4 * g + c];
           double2* pb = p_p2 + (long long)b * 2 * A.K;
@@ -681,6 +684,17 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
   return a;
 }
 
+// waves per work-group of the mixed adjoint: 8 (512 rings per partial) for the 8-map
+// spin-2 kernel on large grids, else 4
+int mixed_adj_waves(const jhtx_plan* p) {
+  return (p->spin == 2 && p->mx_adj_group == 8 && p->mx_adj_wide && p->T >= 1024) ? 8 : 4;
+}
+
+int mixed_adj_chunks(const jhtx_plan* p) {
+  const int rings = 64 * mixed_adj_waves(p);
+  return (p->T + rings - 1) / rings;
+}
+
 int mixed_group(const jhtx_plan* p, bool adjoint) {
   if (p->spin == 0) return 8;
   return adjoint ? p->mx_adj_group : p->mx_synth_group;
@@ -705,18 +719,21 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
 
 hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
                             hipStream_t s) {
-  const RecArgs a = make_args_mixed(p);
+  RecArgs a = make_args_mixed(p);
+  a.nchunk = mixed_adj_chunks(p);
   const int Mpad = (p->M + 7) & ~7;
   const int MG = mixed_group(p, true);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
+  else if (MG == 8 && mixed_adj_waves(p) == 8)
+    hipLaunchKernelGGL((k_adj_mx_s2<2, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 8 && p->mx_adj_occ == 2)
-    hipLaunchKernelGGL((k_adj_mx_s2<2, 2>), grid, dim3(256), 0, s, a, V, part, batch);
+    hipLaunchKernelGGL((k_adj_mx_s2<2, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 8)
-    hipLaunchKernelGGL((k_adj_mx_s2<2, 1>), grid, dim3(256), 0, s, a, V, part, batch);
+    hipLaunchKernelGGL((k_adj_mx_s2<2, 1, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else
-    hipLaunchKernelGGL((k_adj_mx_s2<1, 1>), grid, dim3(256), 0, s, a, V, part, batch);
+    hipLaunchKernelGGL((k_adj_mx_s2<1, 1, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   return hipGetLastError();
 }
 

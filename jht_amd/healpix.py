@@ -134,7 +134,7 @@ class Plan:
     """A jhtx plan on one device plus its workspace (allocated once, reused per call)."""
 
     def __init__(self, nside: int, lmax: int, spin: int, max_batch: int = 1,
-                 precision: int = _lib.PREC_F64, flags: int = 0, device=None):
+                 precision: int = _lib.PREC_F64, flags: int = 0, device=None, tuning=None):
         lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.nside, self.lmax, self.spin = nside, lmax, spin
@@ -144,7 +144,15 @@ class Plan:
         self.npix = 12 * nside * nside
         h = _lib.ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.check(lib.jhtx_plan_create(nside, lmax, spin, precision, max_batch, flags, _lib.ctypes.byref(h)))
+            t = None
+            if tuning:
+                unknown = set(tuning) - set(_lib.TUNING_FIELDS)
+                if unknown:
+                    raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+                t = _lib.Tuning(**{k: int(v) for k, v in tuning.items()})
+            _lib.check(lib.jhtx_plan_create_tuned(nside, lmax, spin, precision, max_batch, flags,
+                                                  _lib.ctypes.byref(t) if t is not None else None,
+                                                  _lib.ctypes.byref(h)))
             self._h = h
             nbytes = int(lib.jhtx_plan_workspace_bytes(h))
             self.workspace = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=self.device)
@@ -159,6 +167,13 @@ class Plan:
                 pass
             self._lib.jhtx_plan_destroy(h)
             self._h = None
+
+    @property
+    def tuning(self) -> dict:
+        """The kernel shapes this plan uses (every field filled in)."""
+        t = _lib.Tuning()
+        _lib.check(self._lib.jhtx_plan_get_tuning(self._h, _lib.ctypes.byref(t)))
+        return {k: getattr(t, k) for k in _lib.TUNING_FIELDS}
 
     STAGES = ("synth_rec", "synth_fft", "adj_fft", "adj_rec")
 
@@ -205,8 +220,25 @@ def _warn_once(nside: int, lmax: int, spin: int) -> None:
         )
 
 
+# kernel shapes chosen for a geometry (jht_amd.tune.autotune or set_tuning)
+_TUNED: dict = {}
+
+
+def set_tuning(nside: int, lmax: int, spin: int, precision: int, tuning) -> None:
+    """Use ``tuning`` (a dict of ``jhtx_tuning`` fields, or None for the defaults) for every
+    plan of this geometry and precision created from now on."""
+    key = (nside, lmax, spin, precision)
+    with _PLAN_LOCK:
+        if tuning:
+            _TUNED[key] = dict(tuning)
+        else:
+            _TUNED.pop(key, None)
+        for k in [k for k in _PLANS if k[:3] == key[:3] and k[3] == precision]:
+            del _PLANS[k]
+
+
 def get_plan(nside: int, lmax: int, spin: int, batch: int = 1, precision: int = _lib.PREC_F64,
-             device=None) -> Plan:
+             device=None, tuning=None) -> Plan:
     if spin not in (0, 2):
         raise NotImplementedError(f"spin={spin} unsupported (only 0 and 2)")
     _warn_once(nside, lmax, spin)
@@ -214,12 +246,17 @@ def get_plan(nside: int, lmax: int, spin: int, batch: int = 1, precision: int = 
         raise RuntimeError("jht_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
     dev = torch.cuda.current_device() if device is None else torch.device(device).index
     flags = _lib.FLAG_BLUESTEIN_ALL if _AZIMUTH_FFT_MODE == "looped" else 0
-    key = (nside, lmax, spin, precision, flags, dev)
+    if tuning is None:
+        tuning = _TUNED.get((nside, lmax, spin, precision))
+    tkey = tuple(sorted(tuning.items())) if tuning else ()
+    key = (nside, lmax, spin, precision, flags, dev, tkey)
     with _PLAN_LOCK:
         p = _PLANS.get(key)
         if p is None or p.max_batch < batch:
             mb = max(batch, p.max_batch if p is not None else 1)
-            p = Plan(nside, lmax, spin, mb, precision, flags, torch.device("cuda", dev))
+            p = None  # release the smaller workspace first
+            _PLANS.pop(key, None)
+            p = Plan(nside, lmax, spin, mb, precision, flags, torch.device("cuda", dev), tuning)
             _PLANS[key] = p
         return p
 

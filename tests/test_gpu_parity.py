@@ -223,3 +223,25 @@ def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkey
     print(f"mixed groups synth={synth_group}/{synth_occ} adj={adj_group}/{adj_occ}: synth {es:.2e} "
           f"adj {ea:.2e}", flush=True)
     assert es <= MIXED_TOL and ea <= MIXED_TOL
+
+
+@pytest.mark.parametrize("wide", [1, 0])
+def test_mixed_adjoint_ring_chunks(wide, monkeypatch):
+    """Spin-2 mixed adjoint with 512-ring (wide) and 256-ring work-groups on a grid large
+    enough for the wide kernel (T = 1024 rings), against the FP64 adjoint."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_ADJ_WIDE", str(wide))
+    H.clear_plans()
+    try:
+        nside, lmax, batch = 512, 640, 9
+        rng = np.random.default_rng(500 + wide)
+        v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
+        b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
+        bmx = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="mixed").cpu().numpy()
+    finally:
+        H.clear_plans()
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"mixed adjoint wide={wide}: {ea:.2e}", flush=True)
+    assert ea <= MIXED_TOL

@@ -203,3 +203,18 @@ def test_ceiling_warning_once_per_geometry():
         warnings.simplefilter("error")
         H._warn_once(4, 7, 0)  # second time: silent
         H._warn_once(4, 6, 0)  # at the ceiling: silent
+
+
+def test_tuning_validation_without_gpu():
+    """An out-of-range jhtx_tuning field is rejected before any device work."""
+    from jht_amd import _lib as L
+
+    lib = _lib()
+    h = ctypes.c_void_p()
+    bad = L.Tuning(synth_rings=3)  # spin 0 allows 1, 4, 8
+    rc = lib.jhtx_plan_create_tuned(16, 24, 0, 0, 1, 0, ctypes.byref(bad), ctypes.byref(h))
+    assert rc == L.JHTX_EINVAL and not h.value
+    assert b"tuning" in lib.jhtx_last_error()
+    bad = L.Tuning(mx_adj_wide=3)
+    rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
+    assert rc == L.JHTX_EINVAL
