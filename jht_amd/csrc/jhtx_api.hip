@@ -348,6 +348,9 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   };
   p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", 8, 4, 8);
   p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
+  if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
+    if (atoi(v) == 3) p->mx_synth_occ = 3;
+  }
   p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", 8, 4, 8);
   p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
   p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
@@ -369,7 +372,7 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
         (t.adj_rings == 0 || (spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
         (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
         (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8})) &&
-        (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2})) &&
+        (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2}));
     if (!ok) {
@@ -397,7 +400,8 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
   // the mixed kernels use one ring per lane and 256-lane work-groups
   const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p) : p->adj_nchunk;
-  p->ws_part_bytes = align256(sizeof(double2) * (size_t)part_chunks * max_batch * p->ncomp * p->K);
+  const size_t part_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
+  p->ws_part_bytes = align256(part_elem * (size_t)part_chunks * max_batch * p->ncomp * p->K);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
   int rc = build_rec_tables(p);
   if (rc == JHTX_OK) rc = build_fft_tables(p);
@@ -447,9 +451,10 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   if (e == hipSuccess) {
     StageTimer t3(p, s, 3);
     if (p->precision == JHTX_PREC_MIXED) {
-      e = launch_adj_mixed(p, V, part, batch, s);
+      float2* part32 = reinterpret_cast<float2*>(part);
+      e = launch_adj_mixed(p, V, part32, batch, s);
       if (e == hipSuccess)
-        e = launch_adj_finish(p, part, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
+        e = launch_adj_finish(p, part32, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
     } else {
       e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
     }

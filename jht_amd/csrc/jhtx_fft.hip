@@ -90,6 +90,9 @@ template <int R, int E>
 __device__ __forceinline__ void fft_pass(double2 (&x)[E], int tid, int nt, int Ns, int dir,
                                          const double2* __restrict__ tw, int twL) {
   constexpr int NB = E / R;
+  // keep this pass's twiddle loads in this pass: hoisting every pass's loads to the top
+  // multiplies the live registers
+  __builtin_amdgcn_sched_barrier(0);
   if (tid >= nt) return;
   const int step = twL / (Ns * R);
 #pragma unroll

@@ -88,7 +88,8 @@ struct jhtx_plan {
   // stage timing (jhtx_plan_set_profiling)
   int profiling;
   void* prof;  // jhtx::Profile*
-  // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2][bpart: nchunk*batch*ncomp*K double2]
+  // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2]
+  // [chunk partials: nchunk*batch*ncomp*K double2 (FP64 path) or float2 (mixed path)]
   size_t ws_F_bytes, ws_part_bytes, ws_bytes;
 };
 
@@ -103,9 +104,11 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
                           int batch, hipStream_t s);
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);
+hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
+                             int nchunk, hipStream_t s);
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                               hipStream_t s);
-hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, float2* part, int batch,
                             hipStream_t s);
 // ring chunks (partials per map) of the mixed adjoint
 int mixed_adj_chunks(const jhtx_plan* p);

@@ -11,8 +11,8 @@
 // Adjoint, b_l[col] = sum_t lambda_l(t) U[t][col]: each wave writes its FP32 recursion values
 // for 32 consecutive l to an LDS tile; v_mfma_f32_16x16x4f32 then contracts 16 same-parity l
 // rows against 4 rings per instruction, so the matrix core performs the ring reduction.  The
-// per-wave results are summed over the work-group in FP64 and written as one partial per ring
-// chunk (the FP64 path's chunk reduction then finishes the sum).
+// per-wave results are summed over the work-group in FP64 and written as one FP32 partial per
+// ring chunk (they are sums of FP32 tiles); the chunk reduction then finishes the sum in FP64.
 #include "jhtx_internal.h"
 
 namespace jhtx {
@@ -397,9 +397,16 @@ __device__ __forceinline__ void adj_tile_mfma(const float* tile, int lane, const
 }
 
 // spin 2: column block cb2, column j = 4 g + c, c in {p2.re, p2.im, m2.re, m2.im}, map
-// 4 cb2 + g of the group; MG = 4 NCB maps share one recursion
+// 4 cb2 + g of the group; MG = 4 NCB maps share one recursion.
+//
+// One B operand per (block, sign, ring) serves both parities: in every lane one of the
+// north / south terms of B is zero (sign +: the south term for c < 2, the north term for
+// c >= 2; sign -: the reverse), so B_odd = sigma B_even with sigma = +-1 fixed by the column
+// and opposite for the two signs.  The odd rows of the sign - tile are stored negated, which
+// makes the odd-parity accumulator equal sigma_+(column) times the true one; the sign is
+// applied when the tile result is reduced.
 template <int NCB>
-__device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, const float (&Bv)[NCB][2][2][16],
+__device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, const float (&Bv)[NCB][2][16],
                                                  floatx4 (&D)[NCB][2]) {
   const int col = lane >> 4;
 #pragma unroll
@@ -415,7 +422,7 @@ __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, co
         const float a = ts[4 * q];
 #pragma unroll
         for (int c = 0; c < NCB; ++c)
-          D[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[c][sg][p][q], D[c][p], 0, 0, 0);
+          D[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[c][sg][q], D[c][p], 0, 0, 0);
       }
     }
   }
@@ -425,7 +432,7 @@ __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, co
 // register budget, in work-groups of this size per CU (NW * MINB = 8 caps it at 256 VGPRs)
 template <int NCB, int MINB, int NW>
 __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
-                                                   double2* __restrict__ part, int batch) {
+                                                   float2* __restrict__ part, int batch) {
   constexpr int MG = 4 * NCB;
   constexpr int RINGS = 64 * NW;
   __shared__ double2 s_ag[CH];
@@ -457,42 +464,42 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
     pmp = -s;
   }
   // B operands: ring 4q + lane/16 of this wave, column lane%16 -> map 4 c + g, component cc
-  float Bv[NCB][2][2][16];
+  float Bv[NCB][2][16];  // [block][sign][ring], even parity (the odd one is +-B, see above)
   {
+    // mu+: p2 <- Vpn, m2 <- s Vms ; mu-: p2 <- s Vps, m2 <- Vmn.  Component cc of the
+    // lane's column: sign + reads Vp north (cc < 2) or Vm south (cc >= 2), sign - reads Vm
+    // north (cc >= 2) or Vp south (cc < 2): one double each.
     const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;  // (-1)^(l-m) at even (l - lmin)
     const int g = (lane & 15) >> 2, cc = lane & 3;
+    const bool lo = cc < 2;
+    const int ri = cc & 1;  // re / im
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
       const int b = mg * MG + 4 * c + g;
+      const double* Vp = reinterpret_cast<const double*>(V + ((long long)b * 2 + 0) * A.nrings * ringM);
+      const double* Vm = reinterpret_cast<const double*>(V + ((long long)b * 2 + 1) * A.nrings * ringM);
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int tr = chunk * RINGS + wave * 64 + 4 * q + (lane >> 4);
-        double vn = 0.0, vs = 0.0, wn = 0.0, ws = 0.0;  // sign + : (n, s south) ; sign - : (s south, n)
+        double bp = 0.0, bm = 0.0;
         if (tr < A.T && b < batch) {
-          const double2* Vp = V + ((long long)b * 2 + 0) * A.nrings * ringM;
-          const double2* Vm = V + ((long long)b * 2 + 1) * A.nrings * ringM;
-          const long long rn = (long long)tr * ringM + m;
-          const double2 vpn = Vp[rn], vmn = Vm[rn];
-          double2 vps = make_double2(0.0, 0.0), vms = vps;
-          if (tr < A.T - 1) {
-            const long long rs = (long long)(4 * A.nside - 2 - tr) * ringM + m;
-            vps = Vp[rs];
-            vms = Vm[rs];
+          const long long rn = 2 * ((long long)tr * ringM + m) + ri;
+          const bool south = tr < A.T - 1;  // the equator ring has no mirror
+          const long long rs = 2 * ((long long)(4 * A.nside - 2 - tr) * ringM + m) + ri;
+          if (lo) {
+            bp = Vp[rn];
+            bm = south ? par0 * Vp[rs] : 0.0;
+          } else {
+            bp = south ? par0 * Vm[rs] : 0.0;
+            bm = Vm[rn];
           }
-          // mu+: p2 <- Vpn, m2 <- s Vms ; mu-: p2 <- s Vps, m2 <- Vmn
-          vn = (cc == 0) ? vpn.x : (cc == 1) ? vpn.y : 0.0;
-          vs = (cc == 2) ? vms.x : (cc == 3) ? vms.y : 0.0;
-          wn = (cc == 2) ? vmn.x : (cc == 3) ? vmn.y : 0.0;
-          ws = (cc == 0) ? vps.x : (cc == 1) ? vps.y : 0.0;
         }
-        Bv[c][0][0][q] = (float)(vn + par0 * vs);
-        Bv[c][0][1][q] = (float)(vn - par0 * vs);
-        Bv[c][1][0][q] = (float)(wn + par0 * ws);
-        Bv[c][1][1][q] = (float)(wn - par0 * ws);
+        Bv[c][0][q] = (float)bp;
+        Bv[c][1][q] = (float)bm;
       }
     }
   }
-  double2* p_p2 = part + (((long long)chunk * batch) * 2) * A.K;  // offset per map below
+  float2* p_p2 = part + (((long long)chunk * batch) * 2) * A.K;  // offset per map below
   float* mytile = &tiles[wave][0][0][0];
   double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16 l][16 NCB cols] FP64
   constexpr int NC = 16 * NCB;
@@ -505,6 +512,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
       s_ag[tid] = ag;
     }
     __syncthreads();
+#pragma unroll 1
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
       // recursion for TL steps, values to the wave's tile
 #pragma unroll 1
@@ -526,7 +534,8 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
           pmp = pm;
           pm = nm;
           mytile[(0 * TL + j0 + jj) * TS + lane] = (float)(np * fp);
-          mytile[(1 * TL + j0 + jj) * TS + lane] = (float)(nm * fm);
+          // odd rows (odd l - lmin) of the sign - tile negated: one B serves both parities
+          mytile[(1 * TL + j0 + jj) * TS + lane] = (float)(nm * ((jj & 1) ? -fm : fm));
         }
         if (anydeep) {
           rescale1(pp, ppp, ep);
@@ -546,7 +555,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = 4 * (lane >> 4) + r, jc = lane & 15;
-            myred[(p * 16 + i) * NC + 16 * c + jc] = (double)D[c][p][r];
+            // odd parity: sigma_+ = -1 for the m2 columns (component lane%4 >= 2)
+            const double sig = (p == 1 && (lane & 2)) ? -1.0 : 1.0;
+            myred[(p * 16 + i) * NC + 16 * c + jc] = sig * (double)D[c][p][r];
           }
       __syncthreads();
       // thread -> (p, i, g): sums 4 waves for the 4 components of map g at l = l0+jb+2i+p
@@ -559,9 +570,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
           for (int w = 0; w < NW; ++w)
 #pragma unroll
             for (int c = 0; c < 4; ++c) s4[c] += red[((w * 2 + p) * 16 + i) * NC + 4 * g + c];
-          double2* pb = p_p2 + (long long)b * 2 * A.K;
-          pb[cb + l] = make_double2(s4[0], s4[1]);
-          pb[A.K + cb + l] = make_double2(s4[2], s4[3]);
+          float2* pb = p_p2 + (long long)b * 2 * A.K;
+          pb[cb + l] = make_float2((float)s4[0], (float)s4[1]);
+          pb[A.K + cb + l] = make_float2((float)s4[2], (float)s4[3]);
         }
       }
       __syncthreads();
@@ -571,7 +582,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
 
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
 __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __restrict__ V,
-                                                   double2* __restrict__ part, int batch) {
+                                                   float2* __restrict__ part, int batch) {
   constexpr int MG = 8;
   __shared__ double s_al[CH];
   __shared__ float tiles[4][TL][TS];  // [wave][l][ring] (one sign)
@@ -612,6 +623,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __r
       }
       Bv[0][0][q] = (float)(vn + vs);
       Bv[0][1][q] = (float)(vn - vs);
+      if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
   float* mytile = &tiles[wave][0][0];
@@ -622,6 +634,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __r
       s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
     }
     __syncthreads();
+#pragma unroll 1
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
 #pragma unroll 1
       for (int j0 = 0; j0 < TL; j0 += 8) {
@@ -655,7 +668,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __r
             re += redbuf[w][p][i][2 * g];
             im += redbuf[w][p][i][2 * g + 1];
           }
-          part[((long long)chunk * batch + b) * A.K + cb + l] = make_double2(re, im);
+          part[((long long)chunk * batch + b) * A.K + cb + l] = make_float2((float)re, (float)im);
         }
       }
       __syncthreads();
@@ -708,6 +721,8 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else if (MG == 8 && p->mx_synth_occ == 3)
+    hipLaunchKernelGGL((k_synth_mx_s2<2, 3>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8 && p->mx_synth_occ == 2)
     hipLaunchKernelGGL((k_synth_mx_s2<2, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8)
@@ -717,7 +732,7 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
   return hipGetLastError();
 }
 
-hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, double2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, float2* part, int batch,
                             hipStream_t s) {
   RecArgs a = make_args_mixed(p);
   a.nchunk = mixed_adj_chunks(p);

@@ -637,7 +637,10 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
 
 // Sum the ring-chunk partials, apply G_l (and for spin 2 the E/B algebra
 // aE = (-p2 - m2)/2, aB = i (p2 - m2)/2, healpix.py:353-354), write packed a_lm.
-__global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const double2* __restrict__ part,
+// P: the chunk partials' type -- double2 (FP64 path) or float2 (mixed path, whose partials
+// are sums of FP32 matrix-core tiles); the chunk sum is taken in FP64 in a fixed order
+template <typename P>
+__global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
                              double2* __restrict__ alm) {
   const int m = blockIdx.x;
   const int b = blockIdx.y;
@@ -650,7 +653,11 @@ __global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const double2* __r
     const double g = A.G[idx];
     if (spin == 0) {
       double2 s = make_double2(0.0, 0.0);
-      for (int c = 0; c < nchunk; ++c) s = s + part[((long long)c * batch + b) * A.K + idx];
+      for (int c = 0; c < nchunk; ++c) {
+        const P v = part[((long long)c * batch + b) * A.K + idx];
+        s.x += (double)v.x;
+        s.y += (double)v.y;
+      }
       alm[(long long)b * A.K + idx] = make_double2(s.x * g, s.y * g);
     } else {
       double2* aE = alm + (long long)b * 2 * A.K;
@@ -662,8 +669,12 @@ __global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const double2* __r
       }
       double2 p2 = make_double2(0.0, 0.0), m2 = p2;
       for (int c = 0; c < nchunk; ++c) {
-        p2 = p2 + part[(((long long)c * batch + b) * ncomp + 0) * A.K + idx];
-        m2 = m2 + part[(((long long)c * batch + b) * ncomp + 1) * A.K + idx];
+        const P u = part[(((long long)c * batch + b) * ncomp + 0) * A.K + idx];
+        const P v = part[(((long long)c * batch + b) * ncomp + 1) * A.K + idx];
+        p2.x += (double)u.x;
+        p2.y += (double)u.y;
+        m2.x += (double)v.x;
+        m2.y += (double)v.y;
       }
       p2 = make_double2(p2.x * g, p2.y * g);
       m2 = make_double2(m2.x * g, m2.y * g);
@@ -744,7 +755,14 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s) {
   const RecArgs a = make_args(p, true);
-  hipLaunchKernelGGL(k_adj_finish, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
+  hipLaunchKernelGGL(k_adj_finish<double2>, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
+                             int nchunk, hipStream_t s) {
+  const RecArgs a = make_args(p, true);
+  hipLaunchKernelGGL(k_adj_finish<float2>, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
   return hipGetLastError();
 }
 
