@@ -397,7 +397,9 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   };
   shape(p->rec_R, p->rec_block, p->nchunk);
   shape(p->adj_R, p->adj_block, p->adj_nchunk);
-  p->ws_F_bytes = align256(sizeof(double2) * (size_t)max_batch * p->ncomp * p->nrings * p->M);
+  // ring coefficients F / V: complex128, or complex64 for the mixed path (FP32 ring FFT)
+  const size_t f_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
+  p->ws_F_bytes = align256(f_elem * (size_t)max_batch * p->ncomp * p->nrings * p->M);
   // the mixed kernels use one ring per lane and 256-lane work-groups
   const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p) : p->adj_nchunk;
   const size_t part_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
@@ -421,11 +423,11 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
   int rc = check_call(p, alm, map, batch, ws);
   if (rc) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  double2* F = reinterpret_cast<double2*>(ws);
+  void* F = ws;
   StageTimer t0(p, s, 0);
   hipError_t e = (p->precision == JHTX_PREC_MIXED)
-                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), F, batch, s)
-                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), F, batch, s);
+                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<float2*>(F), batch, s)
+                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<double2*>(F), batch, s);
   t0.done();
   if (e == hipSuccess) {
     StageTimer t1(p, s, 1);
@@ -452,7 +454,7 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
     StageTimer t3(p, s, 3);
     if (p->precision == JHTX_PREC_MIXED) {
       float2* part32 = reinterpret_cast<float2*>(part);
-      e = launch_adj_mixed(p, V, part32, batch, s);
+      e = launch_adj_mixed(p, reinterpret_cast<const float2*>(V), part32, batch, s);
       if (e == hipSuccess)
         e = launch_adj_finish(p, part32, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
     } else {

@@ -16,8 +16,12 @@
 //   otherwise               chirp-z (Bluestein, _azimuth.py:150-191) at the next power of
 //                           two L >= 2N-1 with the exact k^2 mod 2N chirp reduction and a
 //                           kernel spectrum precomputed at plan creation.
+// The transform arithmetic is templated on the scalar type T: FP64 plans use double, the
+// mixed-precision plans (whose ring coefficients F / V are FP32) use float.  Folds sum in
+// FP64 and every table is computed in FP64 (then rounded once for the float path).
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 #include "jhtx_internal.h"
 
@@ -27,14 +31,26 @@ namespace {
 
 constexpr int kDirectMax = 32;
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+template <typename T> struct CxOf;
+template <> struct CxOf<double> { using type = double2; };
+template <> struct CxOf<float> { using type = float2; };
+template <typename T> using cx = typename CxOf<T>::type;
+
+__device__ __forceinline__ double2 mk2(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ float2 mk2(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk2(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk2(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return mk2(fmaT(a.x, b.x, -a.y * b.y), fmaT(a.x, b.y, a.y * b.x));
 }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-// e^{i pi t} (only used in loops that are not unrolled: register pressure)
+template <typename C> __device__ __forceinline__ C cconj(C a) { return mk2(a.x, -a.y); }
+template <typename C, typename S> __device__ __forceinline__ C cscale(C a, S s) { return mk2(a.x * s, a.y * s); }
+// FP64 -> complex of scalar type T (identity for double)
+template <typename T> __device__ __forceinline__ cx<T> to_cx(double2 v) { return mk2((T)v.x, (T)v.y); }
+// e^{i pi t} in FP64 (only used in loops that are not unrolled: register pressure)
 __device__ __forceinline__ double2 cispi(double t) {
   double s, c;
   sincospi(t, &s, &c);
@@ -51,15 +67,28 @@ struct FftArgs {
   int nside, lmax, M, nrings, spin;
   long long npix;
   const double2* tw;  // exp(-2 pi i k / twL)
+  const float2* twf;  // the same rounded to FP32 (mixed plans)
   int twL;
   const int* pairs;
   const long long* bofs;  // per pair p: offset of its Bluestein kernel spectrum, -1 if none
   const double2* bspec;
+  const float2* bspecf;
   const long long* cofs;  // per pair p: offset of its chirp table c_n = e^{i pi n^2/N}, n < N
   const double2* chirps;
-  int lds_doubles;
+  const float2* chirpsf;
+  int lds_bytes;
   int bluestein_all;
 };
+
+template <typename T> __device__ __forceinline__ const cx<T>* tw_of(const FftArgs& A);
+template <> __device__ __forceinline__ const double2* tw_of<double>(const FftArgs& A) { return A.tw; }
+template <> __device__ __forceinline__ const float2* tw_of<float>(const FftArgs& A) { return A.twf; }
+template <typename T> __device__ __forceinline__ const cx<T>* bspec_of(const FftArgs& A);
+template <> __device__ __forceinline__ const double2* bspec_of<double>(const FftArgs& A) { return A.bspec; }
+template <> __device__ __forceinline__ const float2* bspec_of<float>(const FftArgs& A) { return A.bspecf; }
+template <typename T> __device__ __forceinline__ const cx<T>* chirps_of(const FftArgs& A);
+template <> __device__ __forceinline__ const double2* chirps_of<double>(const FftArgs& A) { return A.chirps; }
+template <> __device__ __forceinline__ const float2* chirps_of<float>(const FftArgs& A) { return A.chirpsf; }
 
 // transform kind for a ring of length N
 __host__ __device__ inline int ring_kind(int N, int bluestein_all, int* L) {
@@ -86,29 +115,26 @@ __host__ __device__ inline int ilog2(int v) {
 // --------------------------------------------------------------------------------- //
 // power-of-two FFT, cyclic register layout: x[e] holds element tid + nt*e (nt = L/E)
 // --------------------------------------------------------------------------------- //
-template <int R, int E>
-__device__ __forceinline__ void fft_pass(double2 (&x)[E], int tid, int nt, int Ns, int dir,
-                                         const double2* __restrict__ tw, int twL) {
+template <int R, int E, typename T>
+__device__ __forceinline__ void fft_pass(cx<T> (&x)[E], int tid, int nt, int Ns, int dir,
+                                         const cx<T>* __restrict__ tw, int twL) {
   constexpr int NB = E / R;
-  // keep this pass's twiddle loads in this pass: hoisting every pass's loads to the top
-  // multiplies the live registers
-  __builtin_amdgcn_sched_barrier(0);
   if (tid >= nt) return;
   const int step = twL / (Ns * R);
 #pragma unroll
   for (int bb = 0; bb < NB; ++bb) {
     const int j = tid + bb * nt;
     const int k = j & (Ns - 1);
-    double2 v[R];
+    cx<T> v[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) v[t] = x[bb + t * NB];
     if (Ns > 1) {
       // one table load per butterfly; w^2, w^3 by complex products (<= 2 ulp)
-      double2 w = tw[k * step];
+      cx<T> w = tw[k * step];
       if (dir > 0) w.y = -w.y;
       v[1] = cmul(v[1], w);
       if (R == 4) {
-        const double2 w2 = cmul(w, w);
+        const cx<T> w2 = cmul(w, w);
         v[2] = cmul(v[2], w2);
         v[3] = cmul(v[3], cmul(w2, w));
       }
@@ -117,10 +143,10 @@ __device__ __forceinline__ void fft_pass(double2 (&x)[E], int tid, int nt, int N
       x[bb] = cadd(v[0], v[1]);
       x[bb + NB] = csub(v[0], v[1]);
     } else {
-      const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-      const double2 a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+      const cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+      const cx<T> a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
       // dir < 0 (forward): a3 = -i d ; dir > 0: a3 = +i d
-      const double2 a3 = dir < 0 ? make_double2(d.y, -d.x) : make_double2(-d.y, d.x);
+      const cx<T> a3 = dir < 0 ? mk2(d.y, -d.x) : mk2(-d.y, d.x);
       x[bb] = cadd(a0, a2);
       x[bb + NB] = cadd(a1, a3);
       x[bb + 2 * NB] = csub(a0, a2);
@@ -130,13 +156,13 @@ __device__ __forceinline__ void fft_pass(double2 (&x)[E], int tid, int nt, int N
 }
 
 // move pass outputs (at Stockham positions) back to the cyclic layout through LDS
-template <int R, int E>
-__device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, int Ns, int lgNs,
-                                             double* lds, bool split) {
+template <int R, int E, typename T>
+__device__ __forceinline__ void fft_exchange(cx<T> (&x)[E], int tid, int nt, int Ns, int lgNs,
+                                             void* lds, bool split) {
   constexpr int NB = E / R;
   constexpr int lgR = (R == 2) ? 1 : 2;
   if (!split) {
-    double2* l2 = reinterpret_cast<double2*>(lds);
+    cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
     if (tid < nt) {
 #pragma unroll
       for (int bb = 0; bb < NB; ++bb) {
@@ -153,6 +179,7 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, i
     }
     __syncthreads();
   } else {
+    T* l1 = reinterpret_cast<T*>(lds);
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
       if (tid < nt) {
@@ -161,7 +188,7 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, i
           const int j = tid + bb * nt;
           const int base = ((j >> lgNs) << (lgNs + lgR)) + (j & (Ns - 1));
 #pragma unroll
-          for (int u = 0; u < R; ++u) lds[base + u * Ns] = part ? x[bb + u * NB].y : x[bb + u * NB].x;
+          for (int u = 0; u < R; ++u) l1[base + u * Ns] = part ? x[bb + u * NB].y : x[bb + u * NB].x;
         }
       }
       __syncthreads();
@@ -169,9 +196,9 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, i
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           if (part)
-            x[e].y = lds[tid + nt * e];
+            x[e].y = l1[tid + nt * e];
           else
-            x[e].x = lds[tid + nt * e];
+            x[e].x = l1[tid + nt * e];
         }
       }
       __syncthreads();
@@ -181,19 +208,19 @@ __device__ __forceinline__ void fft_exchange(double2 (&x)[E], int tid, int nt, i
 
 // Compile-time pass sequence for L = 2^LG: a radix-2 pass first when LG is odd, then
 // radix-4 passes; every pass but the last ends with an exchange through LDS.
-template <int E, int LG, int LGNS>
-__device__ __forceinline__ void fft_passes(double2 (&x)[E], int tid, int dir, const double2* tw,
-                                           int twL, double* lds, bool split) {
+template <int E, int LG, int LGNS, typename T>
+__device__ __forceinline__ void fft_passes(cx<T> (&x)[E], int tid, int dir, const cx<T>* tw,
+                                           int twL, void* lds, bool split) {
   constexpr int REM = LG - LGNS;
   if constexpr (REM > 0) {
     constexpr int R = (REM & 1) ? 2 : 4;
     constexpr int lgR = (R == 2) ? 1 : 2;
     constexpr int nt = (1 << LG) / E;
     constexpr int Ns = 1 << LGNS;
-    fft_pass<R, E>(x, tid, nt, Ns, dir, tw, twL);
+    fft_pass<R, E, T>(x, tid, nt, Ns, dir, tw, twL);
     if constexpr (REM > lgR) {
-      fft_exchange<R, E>(x, tid, nt, Ns, LGNS, lds, split);
-      fft_passes<E, LG, LGNS + lgR>(x, tid, dir, tw, twL, lds, split);
+      fft_exchange<R, E, T>(x, tid, nt, Ns, LGNS, lds, split);
+      fft_passes<E, LG, LGNS + lgR, T>(x, tid, dir, tw, twL, lds, split);
     }
   }
 }
@@ -201,24 +228,24 @@ __device__ __forceinline__ void fft_passes(double2 (&x)[E], int tid, int dir, co
 // In-place FFT of length L = 2^lgL with LGLO <= lgL <= LGHI (the range of one launch
 // class); dir -1 forward (e^{-2 pi i}), +1 backward.  Starts with a barrier so LDS may
 // still hold the input the caller loaded from.
-template <int E, int LG, int LGHI>
-__device__ __forceinline__ void fft_dispatch(double2 (&x)[E], int lgL, int tid, int dir,
-                                             const double2* tw, int twL, double* lds, bool split) {
+template <int E, int LG, int LGHI, typename T>
+__device__ __forceinline__ void fft_dispatch(cx<T> (&x)[E], int lgL, int tid, int dir,
+                                             const cx<T>* tw, int twL, void* lds, bool split) {
   if constexpr (LG < LGHI) {
     if (lgL != LG) {
-      fft_dispatch<E, LG + 1, LGHI>(x, lgL, tid, dir, tw, twL, lds, split);
+      fft_dispatch<E, LG + 1, LGHI, T>(x, lgL, tid, dir, tw, twL, lds, split);
       return;
     }
   }
-  fft_passes<E, LG, 0>(x, tid, dir, tw, twL, lds, split);
+  fft_passes<E, LG, 0, T>(x, tid, dir, tw, twL, lds, split);
 }
 
-template <int E, int LGLO, int LGHI>
-__device__ __forceinline__ void fft_pow2(double2 (&x)[E], int lgL, int tid, int dir,
-                                         const double2* tw, int twL, double* lds, int lds_doubles) {
-  const bool split = (2 * (1 << lgL) > lds_doubles);
+template <int E, int LGLO, int LGHI, typename T>
+__device__ __forceinline__ void fft_pow2(cx<T> (&x)[E], int lgL, int tid, int dir,
+                                         const cx<T>* tw, int twL, void* lds, int lds_bytes) {
+  const bool split = (2 * (1 << lgL) * (int)sizeof(T) > lds_bytes);
   __syncthreads();
-  fft_dispatch<E, LGLO, LGHI>(x, lgL, tid, dir, tw, twL, lds, split);
+  fft_dispatch<E, LGLO, LGHI, T>(x, lgL, tid, dir, tw, twL, lds, split);
 }
 
 // transform-length range of the launch class (threads, elements per thread)
@@ -228,42 +255,43 @@ constexpr int lg_hi(int block, int e) {
   return l;
 }
 constexpr int lg_lo(int block, int e) { return block == 64 ? 6 : (block == 256 ? lg_hi(block, e) - 1 : lg_hi(block, e)); }
+static_assert(lg_lo(1024, 16) == 14 && lg_hi(1024, 16) == 14, "the 1024-thread class is L = 16384 only");
 
 // --------------------------------------------------------------------------------- //
-// length-N DFT  X_j = sum_k in_k e^{s 2 pi i j k / N}, in = LDS double2[N] at offset 0.
-// Returns the double2 offset of the output (N values) in LDS.  Ends with a barrier.
+// length-N DFT  X_j = sum_k in_k e^{s 2 pi i j k / N}, in = LDS cx<T>[N] at offset 0.
+// Returns the complex offset of the output (N values) in LDS.  Ends with a barrier.
 // --------------------------------------------------------------------------------- //
-template <int E, int LGLO, int LGHI, int KIND>
+template <int E, int LGLO, int LGHI, int KIND, typename T>
 __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, long long bofs,
-                        const double2* __restrict__ ch, double* lds, int tid, int nthr) {
-  double2* l2 = reinterpret_cast<double2*>(lds);
+                                        const cx<T>* __restrict__ ch, void* lds, int tid, int nthr) {
+  cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
   if constexpr (KIND == 0) {
-    double2* roots = l2 + N;
-    double2* out = l2 + 2 * N;
-    for (int q = tid; q < N; q += nthr) roots[q] = cispi((double)(2 * s * q) / (double)N);
+    cx<T>* roots = l2 + N;
+    cx<T>* out = l2 + 2 * N;
+    for (int q = tid; q < N; q += nthr) roots[q] = to_cx<T>(cispi((double)(2 * s * q) / (double)N));
     __syncthreads();
     for (int j = tid; j < N; j += nthr) {
-      double2 acc = make_double2(0.0, 0.0);
+      cx<T> acc = mk2((T)0, (T)0);
       for (int k = 0; k < N; ++k) {
-        const double2 w = roots[(j * k) % N];
-        const double2 v = l2[k];
-        acc.x = fma(v.x, w.x, fma(-v.y, w.y, acc.x));
-        acc.y = fma(v.x, w.y, fma(v.y, w.x, acc.y));
+        const cx<T> w = roots[(j * k) % N];
+        const cx<T> v = l2[k];
+        acc.x = fmaT(v.x, w.x, fmaT(-v.y, w.y, acc.x));
+        acc.y = fmaT(v.x, w.y, fmaT(v.y, w.x, acc.y));
       }
       out[j] = acc;
     }
     __syncthreads();
-    return 2 * N;  // double2 offset of out
+    return 2 * N;  // complex offset of out
   }
   const int lgL = ilog2(L);
   const int nt = L / E;
-  double2 x[E];
+  cx<T> x[E];
   if constexpr (KIND == 1) {
     if (tid < nt) {
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = l2[tid + nt * e];
     }
-    fft_pow2<E, LGLO, LGHI>(x, lgL, tid, s < 0 ? -1 : 1, A.tw, A.twL, lds, A.lds_doubles);
+    fft_pow2<E, LGLO, LGHI, T>(x, lgL, tid, s < 0 ? -1 : 1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
     if (tid < nt) {
 #pragma unroll
       for (int e = 0; e < E; ++e) l2[tid + nt * e] = x[e];
@@ -277,10 +305,10 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
     for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
       if (i < N) {
-        const double2 c = ch[i];
+        const cx<T> c = ch[i];
         x[e] = cmul(l2[i], s > 0 ? c : cconj(c));
       } else {
-        x[e] = make_double2(0.0, 0.0);
+        x[e] = mk2((T)0, (T)0);
       }
       if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
@@ -288,24 +316,24 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
   // forward FFT, pointwise product with the kernel spectrum, backward FFT: one call site
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass) {
-    fft_pow2<E, LGLO, LGHI>(x, lgL, tid, pass ? +1 : -1, A.tw, A.twL, lds, A.lds_doubles);
+    fft_pow2<E, LGLO, LGHI, T>(x, lgL, tid, pass ? +1 : -1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
     if (pass == 0 && tid < nt) {
-      const double2* B = A.bspec + bofs;  // spectrum of e^{-i pi n^2/N}; conj for s < 0
+      const cx<T>* B = bspec_of<T>(A) + bofs;  // spectrum of e^{-i pi n^2/N}; conj for s < 0
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const double2 bv = B[tid + nt * e];
+        const cx<T> bv = B[tid + nt * e];
         x[e] = cmul(x[e], s > 0 ? bv : cconj(bv));
         if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bound in-flight loads
       }
     }
   }
-  const double invL = 1.0 / (double)L;
+  const T invL = (T)1 / (T)L;
   if (tid < nt) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
       if (i < N) {
-        const double2 c = ch[i];
+        const cx<T> c = ch[i];
         l2[i] = cmul(cscale(x[e], invL), s > 0 ? c : cconj(c));
       }
       if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -316,10 +344,17 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
 }
 
 // --------------------------------------------------------------------------------- //
-// synthesis fold: D_k (k < N) into LDS dst; combine -> dst_k += i D_k
+// synthesis fold: D_k (k < N) into LDS dst; combine -> dst_k += i D_k.  Sums in FP64.
 // --------------------------------------------------------------------------------- //
-__device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restrict__ Fm, int M,
-                          int N, bool sh, bool real0, bool combine, double2* dst,
+template <typename T>
+__device__ __forceinline__ double2 ld64(const cx<T>* p, int m) {
+  const cx<T> v = p[m];
+  return make_double2((double)v.x, (double)v.y);
+}
+
+template <typename T>
+__device__ void ring_fold(const cx<T>* __restrict__ Fp, const cx<T>* __restrict__ Fm, int M,
+                          int N, bool sh, bool real0, bool combine, cx<T>* dst,
                           double2* scratch, int tid, int nthr) {
   if (N <= nthr) {
     const int S = N * (nthr / N);
@@ -329,14 +364,14 @@ __device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restr
       for (int m = tid; m < M; m += S) {
         const int q = m / N;
         const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-        double2 v = Fp[m];
+        double2 v = ld64<T>(Fp, m);
         if (m == 0 && real0) v.y = 0.0;
         ap.x += sg * v.x;
         ap.y += sg * v.y;
         if (m >= 1) {
           const int qq = (m + km) / N;
           const double sm = (sh && (qq & 1)) ? -1.0 : 1.0;
-          const double2 w = Fm[m];
+          const double2 w = ld64<T>(Fm, m);
           am.x += sm * w.x;
           am.y -= sm * w.y;
         }
@@ -351,10 +386,10 @@ __device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restr
       for (int j = (N - k) % N; j < S; j += N) acc = cadd(acc, scratch[S + j]);
       if (sh) acc = cmul(acc, cispi((double)k / (double)N));
       if (combine) {
-        const double2 d = dst[k];
-        dst[k] = make_double2(d.x - acc.y, d.y + acc.x);
+        const cx<T> d = dst[k];
+        dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
       } else {
-        dst[k] = acc;
+        dst[k] = to_cx<T>(acc);
       }
     }
     __syncthreads();
@@ -365,7 +400,7 @@ __device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restr
     int q = 0;
     for (int m = k; m < M; m += N, ++q) {
       const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-      double2 v = Fp[m];
+      double2 v = ld64<T>(Fp, m);
       if (m == 0 && real0) v.y = 0.0;
       acc.x += sg * v.x;
       acc.y += sg * v.y;
@@ -373,29 +408,37 @@ __device__ void ring_fold(const double2* __restrict__ Fp, const double2* __restr
     q = 1;
     for (int m = N - k; m < M; m += N, ++q) {
       const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-      const double2 w = Fm[m];
+      const double2 w = ld64<T>(Fm, m);
       acc.x += sg * w.x;
       acc.y -= sg * w.y;
     }
     if (sh) acc = cmul(acc, cispi((double)k / (double)N));
     if (combine) {
-      const double2 d = dst[k];
-      dst[k] = make_double2(d.x - acc.y, d.y + acc.x);
+      const cx<T> d = dst[k];
+      dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
     } else {
-      dst[k] = acc;
+      dst[k] = to_cx<T>(acc);
     }
   }
   __syncthreads();
 }
 
-// Work-group per mirror ring pair; template on the transform kind and spin so each
-// instantiation holds one code path (register pressure).
-template <int BLOCK, int E, int SPIN, int KIND>
-__global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* __restrict__ F,
+// LDS: ring data as cx<T>[N] from offset 0; the fold's FP64 partials follow at a 16-byte
+// aligned offset (only for N <= BLOCK)
+template <typename T>
+__device__ __forceinline__ double2* fold_scratch(void* lds, int N) {
+  const size_t off = ((size_t)N * sizeof(cx<T>) + 15) & ~(size_t)15;
+  return reinterpret_cast<double2*>(reinterpret_cast<char*>(lds) + off);
+}
+
+// Work-group per mirror ring pair; template on the transform kind, spin and scalar type so
+// each instantiation holds one code path (register pressure).
+template <int BLOCK, int E, int SPIN, int KIND, typename T>
+__global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __restrict__ F,
                                                       double* __restrict__ map) {
   constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
-  double2* l2 = reinterpret_cast<double2*>(lds);
+  cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
   const int p = A.pairs[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
@@ -407,51 +450,51 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const double2* 
   int L;
   ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
-  const double2* ch = (A.cofs[p] >= 0) ? A.chirps + A.cofs[p] : nullptr;
+  const cx<T>* ch = (A.cofs[p] >= 0) ? chirps_of<T>(A) + A.cofs[p] : nullptr;
   const long long ringM = A.M;
-  double2* scratch = l2 + N;  // fold partials (only when N <= BLOCK)
+  double2* scratch = fold_scratch<T>(lds, N);  // fold partials (only when N <= BLOCK)
   if constexpr (SPIN == 0) {
-    const double2* Fb = F + (long long)b * A.nrings * ringM;
-    ring_fold(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, true, false, l2, scratch, tid, BLOCK);
-    if (pair) ring_fold(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, true, true, l2, scratch, tid, BLOCK);
-    const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
-    const double2* out = l2 + o;
+    const cx<T>* Fb = F + (long long)b * A.nrings * ringM;
+    ring_fold<T>(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, true, false, l2, scratch, tid, BLOCK);
+    if (pair) ring_fold<T>(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, true, true, l2, scratch, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
+    const cx<T>* out = l2 + o;
     double* mb = map + (long long)b * A.npix;
     const long long sn = ring_start(A.nside, rn);
     const long long ss = ring_start(A.nside, rs);
     for (int j = tid; j < N; j += BLOCK) {
-      const double2 z = out[j];
-      mb[sn + j] = z.x;
-      if (pair) mb[ss + j] = z.y;
+      const cx<T> z = out[j];
+      mb[sn + j] = (double)z.x;
+      if (pair) mb[ss + j] = (double)z.y;
     }
   } else {
-    const double2* Fpb = F + (long long)b * 2 * A.nrings * ringM;
-    const double2* Fmb = Fpb + (long long)A.nrings * ringM;
+    const cx<T>* Fpb = F + (long long)b * 2 * A.nrings * ringM;
+    const cx<T>* Fmb = Fpb + (long long)A.nrings * ringM;
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
 #pragma unroll 1
     for (int h = 0; h < (pair ? 2 : 1); ++h) {
       const int r = h ? rs : rn;
-      ring_fold(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
-      const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
-      const double2* out = l2 + o;
+      ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
+      const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
+      const cx<T>* out = l2 + o;
       const long long st = ring_start(A.nside, r);
       for (int j = tid; j < N; j += BLOCK) {
-        const double2 z = out[j];
-        Q[st + j] = z.x;
-        U[st + j] = z.y;
+        const cx<T> z = out[j];
+        Q[st + j] = (double)z.x;
+        U[st + j] = (double)z.y;
       }
       __syncthreads();
     }
   }
 }
 
-template <int BLOCK, int E, int SPIN, int KIND>
+template <int BLOCK, int E, int SPIN, int KIND, typename T>
 __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double* __restrict__ map,
-                                                        double2* __restrict__ V) {
+                                                        cx<T>* __restrict__ V) {
   constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
-  double2* l2 = reinterpret_cast<double2*>(lds);
+  cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
   const int p = A.pairs[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
@@ -463,28 +506,29 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   int L;
   ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
-  const double2* ch = (A.cofs[p] >= 0) ? A.chirps + A.cofs[p] : nullptr;
+  const cx<T>* ch = (A.cofs[p] >= 0) ? chirps_of<T>(A) + A.cofs[p] : nullptr;
   const long long ringM = A.M;
   const long long sn = ring_start(A.nside, rn);
   const long long ss = ring_start(A.nside, rs);
   if constexpr (SPIN == 0) {
     const double* mb = map + (long long)b * A.npix;
-    for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(mb[sn + j], pair ? mb[ss + j] : 0.0);
+    for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)mb[sn + j], (T)(pair ? mb[ss + j] : 0.0));
     __syncthreads();
-    const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
-    const double2* W = l2 + o;
-    double2* Vn = V + ((long long)b * A.nrings + rn) * ringM;
-    double2* Vs = V + ((long long)b * A.nrings + rs) * ringM;
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
+    const cx<T>* W = l2 + o;
+    cx<T>* Vn = V + ((long long)b * A.nrings + rn) * ringM;
+    cx<T>* Vs = V + ((long long)b * A.nrings + rs) * ringM;
     for (int m = tid; m < A.M; m += BLOCK) {
       const int k = m % N, q = m / N;
-      const double2 wk = W[k], wr = cconj(W[(N - k) % N]);
+      const cx<T> wk = W[k], wr = cconj(W[(N - k) % N]);
       const double sg = (sh && (q & 1)) ? -0.5 : 0.5;
-      double2 t = make_double2(sg, 0.0);
-      if (sh) t = cscale(cispi(-(double)k / (double)N), sg);
+      double2 t64 = make_double2(sg, 0.0);
+      if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
+      const cx<T> t = to_cx<T>(t64);
       Vn[m] = cmul(cadd(wk, wr), t);
       if (pair) {
-        const double2 d = csub(wk, wr);  // -i d / 2 is the south ring's spectrum
-        Vs[m] = cmul(make_double2(d.y, -d.x), t);
+        const cx<T> d = csub(wk, wr);  // -i d / 2 is the south ring's spectrum
+        Vs[m] = cmul(mk2(d.y, -d.x), t);
       }
     }
   } else {
@@ -495,17 +539,18 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
       const int r = h ? rs : rn;
       const long long st = h ? ss : sn;
       __syncthreads();
-      for (int j = tid; j < N; j += BLOCK) l2[j] = make_double2(Q[st + j], U[st + j]);
+      for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)Q[st + j], (T)U[st + j]);
       __syncthreads();
-      const int o = ring_dft<E, LGLO, LGHI, KIND>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
-      const double2* W = l2 + o;
-      double2* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
-      double2* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
+      const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
+      const cx<T>* W = l2 + o;
+      cx<T>* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
+      cx<T>* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
       for (int m = tid; m < A.M; m += BLOCK) {
         const int k = m % N, q = m / N;
         const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-        double2 t = make_double2(sg, 0.0);
-        if (sh) t = cscale(cispi(-(double)k / (double)N), sg);
+        double2 t64 = make_double2(sg, 0.0);
+        if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
+        const cx<T> t = to_cx<T>(t64);
         Vp[m] = cmul(W[k], t);
         Vm[m] = cmul(cconj(W[(N - k) % N]), t);
       }
@@ -523,7 +568,8 @@ __global__ void k_chirp_table(FftArgs A) {
   for (int n = threadIdx.x; n < N; n += blockDim.x) c[n] = chirp(n, N, +1);
 }
 
-// Kernel spectrum of a Bluestein ring: FFT_L of b_n = e^{-i pi n^2 / N} (n < N, mirrored).
+// Kernel spectrum of a Bluestein ring: FFT_L of b_n = e^{-i pi n^2 / N} (n < N, mirrored),
+// always in FP64.
 template <int BLOCK, int E>
 __global__ __launch_bounds__(BLOCK) void k_bluestein_spectrum(FftArgs A) {
   extern __shared__ double lds[];
@@ -548,7 +594,7 @@ __global__ __launch_bounds__(BLOCK) void k_bluestein_spectrum(FftArgs A) {
       x[e] = (n >= 0) ? cconj(ch[n]) : make_double2(0.0, 0.0);
     }
   }
-  fft_pow2<E, lg_lo(BLOCK, E), lg_hi(BLOCK, E)>(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_doubles);
+  fft_pow2<E, lg_lo(BLOCK, E), lg_hi(BLOCK, E), double>(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_bytes);
   if (tid < nt) {
     double2* B = const_cast<double2*>(A.bspec) + bofs;
 #pragma unroll
@@ -556,6 +602,11 @@ __global__ __launch_bounds__(BLOCK) void k_bluestein_spectrum(FftArgs A) {
   }
 }
 
+// FP64 table -> FP32 table (the float path's twiddles, chirps and kernel spectra)
+__global__ void k_round_f32(const double2* __restrict__ in, float2* __restrict__ out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
 
 FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   FftArgs a;
@@ -566,13 +617,16 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   a.spin = p->spin;
   a.npix = p->npix;
   a.tw = p->fft.tw;
+  a.twf = p->fft.twf;
   a.twL = p->fft.lmax_fft;
   a.pairs = c.pairs;
   a.bofs = p->fft.bofs;
   a.bspec = p->fft.bspec;
+  a.bspecf = p->fft.bspecf;
   a.cofs = p->fft.cofs;
   a.chirps = p->fft.chirps;
-  a.lds_doubles = c.lds_doubles;
+  a.chirpsf = p->fft.chirpsf;
+  a.lds_bytes = c.lds_bytes;
   a.bluestein_all = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
   return a;
 }
@@ -583,43 +637,61 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
 constexpr int kNlen = 4;
 constexpr int kBlocks[kNlen] = {64, 256, 512, 512};
 constexpr int kElems[kNlen] = {16, 16, 16, 32};
-constexpr int kLdsCap[kNlen] = {2048, 8192, 16384, 16384};  // doubles: 16, 64, 128, 128 KiB
+constexpr int kLdsCap[kNlen] = {16384, 65536, 131072, 131072};  // bytes
 
 inline int class_of(int L) { return (L <= 1024) ? 0 : (L <= 4096) ? 1 : (L <= 8192) ? 2 : 3; }
 
-template <int BLOCK, int E, int SPIN, int KIND>
+template <int BLOCK, int E, int SPIN, int KIND, typename T>
 void launch_one(bool synth, const FftArgs& a, dim3 grid, size_t sh, hipStream_t s,
                 const void* in, void* out) {
   if (sh > 65536) {
     static const bool once = [] {
-      (void)hipFuncSetAttribute((const void*)k_ring_synth<BLOCK, E, SPIN, KIND>,
+      (void)hipFuncSetAttribute((const void*)k_ring_synth<BLOCK, E, SPIN, KIND, T>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-      (void)hipFuncSetAttribute((const void*)k_ring_adjoint<BLOCK, E, SPIN, KIND>,
+      (void)hipFuncSetAttribute((const void*)k_ring_adjoint<BLOCK, E, SPIN, KIND, T>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
       return true;
     }();
     (void)once;
   }
   if (synth)
-    hipLaunchKernelGGL((k_ring_synth<BLOCK, E, SPIN, KIND>), grid, dim3(BLOCK), sh, s, a,
-                       (const double2*)in, (double*)out);
+    hipLaunchKernelGGL((k_ring_synth<BLOCK, E, SPIN, KIND, T>), grid, dim3(BLOCK), sh, s, a,
+                       (const cx<T>*)in, (double*)out);
   else
-    hipLaunchKernelGGL((k_ring_adjoint<BLOCK, E, SPIN, KIND>), grid, dim3(BLOCK), sh, s, a,
-                       (const double*)in, (double2*)out);
+    hipLaunchKernelGGL((k_ring_adjoint<BLOCK, E, SPIN, KIND, T>), grid, dim3(BLOCK), sh, s, a,
+                       (const double*)in, (cx<T>*)out);
 }
 
-template <int SPIN>
+// threads of a launch class: the FP32 path runs the L <= 16384 chirp-z class as
+// 1024 threads x 16 elements (the FP64 path as 512 x 32: its data would not fit 1024 lanes)
+// (JHTX_FFT_WIDE=0 keeps 512 x 32 for both: a tuning switch)
+inline bool fft_wide_big() {
+  static const bool wide = [] {
+    const char* v = getenv("JHTX_FFT_WIDE");
+    return !(v && atoi(v) == 0);
+  }();
+  return wide;
+}
+inline int class_block(int lc, bool f32) { return (lc == 3 && f32 && fft_wide_big()) ? 1024 : kBlocks[lc]; }
+
+template <int SPIN, typename T>
 void launch_class(bool synth, int lc, int kind, const FftArgs& a, dim3 grid, size_t sh,
                   hipStream_t s, const void* in, void* out) {
+  if constexpr (sizeof(T) == 4) {
+    if (lc == 3 && fft_wide_big()) {
+      launch_one<1024, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out);
+      return;
+    }
+  }
   switch (lc * 3 + kind) {
-    case 0: launch_one<64, 16, SPIN, 0>(synth, a, grid, sh, s, in, out); break;
-    case 1: launch_one<64, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
-    case 2: launch_one<64, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
-    case 4: launch_one<256, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
-    case 5: launch_one<256, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
-    case 7: launch_one<512, 16, SPIN, 1>(synth, a, grid, sh, s, in, out); break;
-    case 8: launch_one<512, 16, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
-    default: launch_one<512, 32, SPIN, 2>(synth, a, grid, sh, s, in, out); break;
+    case 0: launch_one<64, 16, SPIN, 0, T>(synth, a, grid, sh, s, in, out); break;
+    case 1: launch_one<64, 16, SPIN, 1, T>(synth, a, grid, sh, s, in, out); break;
+    case 2: launch_one<64, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out); break;
+    case 4: launch_one<256, 16, SPIN, 1, T>(synth, a, grid, sh, s, in, out); break;
+    case 5: launch_one<256, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out); break;
+    case 7: launch_one<512, 16, SPIN, 1, T>(synth, a, grid, sh, s, in, out); break;
+    case 8: launch_one<512, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out); break;
+    default: launch_one<512, 32, SPIN, 2, T>(synth, a, grid, sh, s, in, out); break;
   }
 }
 
@@ -638,9 +710,12 @@ int build_fft_tables(jhtx_plan* p) {
   const int nside = p->nside;
   const int npairs = 2 * nside;
   const int ball = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
+  // the ring kernels of mixed-precision plans run in FP32 (complex = 8 bytes)
+  const bool f32 = (p->precision == JHTX_PREC_MIXED);
+  const int cs = f32 ? 8 : 16;
   constexpr int kNcls = 3 * kNlen;
   std::vector<int> cls_pairs[kNcls];
-  int need[kNcls] = {0};
+  int need[kNcls] = {0}, need64[kNcls] = {0};
   int twL = 16;
   std::vector<long long> bofs(npairs, -1), cofs(npairs, -1);
   long long blen = 0, clen = 0;
@@ -649,18 +724,21 @@ int build_fft_tables(jhtx_plan* p) {
     int L;
     const int kind = ring_kind(N, ball, &L);
     const int lc = class_of(L);
-    if (L > 16384 || (kind == 1 && 2 * N > kLdsCap[lc])) {
+    if (L > 16384 || (kind == 1 && 16 * N > kLdsCap[lc])) {
       set_error("ring transform length " + std::to_string(L) + " (nside=" + std::to_string(nside) +
                 ") exceeds the single-work-group ring FFT (nside <= 2048)");
       return JHTX_EUNSUPPORTED;
     }
     const int c = 3 * lc + kind;
     cls_pairs[c].push_back(q);
-    const int blk = kBlocks[lc];
-    int nd = (kind == 0) ? 6 * N : 2 * L;
-    if (N <= blk) nd = std::max(nd, 2 * N + 4 * blk);
-    nd = std::max(nd, 2 * N);
+    const int blk = class_block(lc, f32);
+    // bytes: direct DFT in + roots + out; FFT data (split exchanges if above the cap);
+    // the fold's FP64 partials behind the ring data when N <= BLOCK
+    int nd = (kind == 0) ? 3 * N * cs : L * cs;
+    if (N <= blk) nd = std::max(nd, ((N * cs + 15) & ~15) + 2 * blk * 16);
+    nd = std::max(nd, N * cs);
     need[c] = std::max(need[c], nd);
+    need64[c] = std::max(need64[c], (kind == 0) ? 3 * N * 16 : L * 16);
     if (kind >= 1) twL = std::max(twL, L);
     if (kind == 2) {
       bofs[q] = blen;
@@ -679,6 +757,13 @@ int build_fft_tables(jhtx_plan* p) {
   e = hipMalloc(&p->fft.tw, sizeof(double2) * twL);
   if (e != hipSuccess) return JHTX_ENOMEM;
   (void)hipMemcpy(p->fft.tw, tw.data(), sizeof(double2) * twL, hipMemcpyHostToDevice);
+  if (f32) {
+    std::vector<float2> twf(twL);
+    for (int k = 0; k < twL; ++k) twf[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+    e = hipMalloc(&p->fft.twf, sizeof(float2) * twL);
+    if (e != hipSuccess) return JHTX_ENOMEM;
+    (void)hipMemcpy(p->fft.twf, twf.data(), sizeof(float2) * twL, hipMemcpyHostToDevice);
+  }
   p->fft.lmax_fft = twL;
   e = hipMalloc(&p->fft.bofs, sizeof(long long) * npairs);
   if (e != hipSuccess) return JHTX_ENOMEM;
@@ -687,6 +772,10 @@ int build_fft_tables(jhtx_plan* p) {
   if (blen > 0) {
     e = hipMalloc(&p->fft.bspec, sizeof(double2) * blen);
     if (e != hipSuccess) return JHTX_ENOMEM;
+    if (f32) {
+      e = hipMalloc(&p->fft.bspecf, sizeof(float2) * blen);
+      if (e != hipSuccess) return JHTX_ENOMEM;
+    }
   }
   e = hipMalloc(&p->fft.cofs, sizeof(long long) * npairs);
   if (e != hipSuccess) return JHTX_ENOMEM;
@@ -694,15 +783,20 @@ int build_fft_tables(jhtx_plan* p) {
   if (clen > 0) {
     e = hipMalloc(&p->fft.chirps, sizeof(double2) * clen);
     if (e != hipSuccess) return JHTX_ENOMEM;
+    if (f32) {
+      e = hipMalloc(&p->fft.chirpsf, sizeof(float2) * clen);
+      if (e != hipSuccess) return JHTX_ENOMEM;
+    }
   }
   p->fft.ncls = 0;
   for (int c = 0; c < kNcls; ++c) {
     FftClass& fc = p->fft.cls[c];
     fc.lc = c / 3;
     fc.kind = c % 3;
-    fc.block = kBlocks[fc.lc];
+    fc.block = class_block(fc.lc, f32);
     fc.npairs = (int)cls_pairs[c].size();
-    fc.lds_doubles = std::min(std::max(need[c], 64), kLdsCap[fc.lc]);
+    fc.lds_bytes = std::min(std::max(need[c], 512), kLdsCap[fc.lc]);
+    fc.lds64_bytes = std::min(std::max(need64[c], 512), kLdsCap[fc.lc]);
     fc.pairs = nullptr;
     if (fc.npairs) {
       e = hipMalloc(&fc.pairs, sizeof(int) * fc.npairs);
@@ -711,9 +805,9 @@ int build_fft_tables(jhtx_plan* p) {
       p->fft.ncls++;
     }
   }
-  // chirp tables, then the Bluestein kernel spectra
+  // chirp tables, then the Bluestein kernel spectra (FP64), then their FP32 copies
   if (blen > 0) {
-    const int big = kLdsCap[2] * (int)sizeof(double);
+    const int big = kLdsCap[2];
     (void)hipFuncSetAttribute((const void*)k_bluestein_spectrum<512, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
     (void)hipFuncSetAttribute((const void*)k_bluestein_spectrum<512, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
     for (int c = 0; c < kNcls; ++c) {
@@ -724,7 +818,13 @@ int build_fft_tables(jhtx_plan* p) {
     for (int c = 0; c < kNcls; ++c) {
       const FftClass& fc = p->fft.cls[c];
       if (!fc.npairs || fc.kind != 2) continue;
-      launch_spectrum(fc.lc, make_fft_args(p, fc), dim3(fc.npairs), sizeof(double) * fc.lds_doubles);
+      FftArgs a = make_fft_args(p, fc);
+      a.lds_bytes = fc.lds64_bytes;
+      launch_spectrum(fc.lc, a, dim3(fc.npairs), fc.lds64_bytes);
+    }
+    if (f32) {
+      hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)p->fft.chirps, p->fft.chirpsf, clen);
+      hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)p->fft.bspec, p->fft.bspecf, blen);
     }
     e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -737,10 +837,13 @@ int build_fft_tables(jhtx_plan* p) {
 
 void free_fft_tables(jhtx_plan* p) {
   if (p->fft.tw) (void)hipFree(p->fft.tw);
+  if (p->fft.twf) (void)hipFree(p->fft.twf);
   if (p->fft.bofs) (void)hipFree(p->fft.bofs);
   if (p->fft.bspec) (void)hipFree(p->fft.bspec);
+  if (p->fft.bspecf) (void)hipFree(p->fft.bspecf);
   if (p->fft.cofs) (void)hipFree(p->fft.cofs);
   if (p->fft.chirps) (void)hipFree(p->fft.chirps);
+  if (p->fft.chirpsf) (void)hipFree(p->fft.chirpsf);
   for (int c = 0; c < 12; ++c)
     if (p->fft.cls[c].pairs) (void)hipFree(p->fft.cls[c].pairs);
   p->fft = FftTables();
@@ -748,28 +851,36 @@ void free_fft_tables(jhtx_plan* p) {
 
 static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, void* out, int batch,
                               hipStream_t s) {
+  const bool f32 = (p->precision == JHTX_PREC_MIXED);
   for (int c = 0; c < 12; ++c) {
     const FftClass& fc = p->fft.cls[c];
     if (!fc.npairs) continue;
     const FftArgs a = make_fft_args(p, fc);
-    const size_t sh = sizeof(double) * fc.lds_doubles;
+    const size_t sh = fc.lds_bytes;
     const dim3 grid(fc.npairs, batch);
-    if (p->spin == 0)
-      launch_class<0>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
-    else
-      launch_class<2>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    if (p->spin == 0) {
+      if (f32)
+        launch_class<0, float>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+      else
+        launch_class<0, double>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    } else {
+      if (f32)
+        launch_class<2, float>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+      else
+        launch_class<2, double>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
+hipError_t launch_ring_synth(const jhtx_plan* p, const void* F, double* map, int batch,
                              hipStream_t s) {
   return launch_ring(p, true, F, map, batch, s);
 }
 
-hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,
+hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, int batch,
                                hipStream_t s) {
   return launch_ring(p, false, map, V, batch, s);
 }

@@ -51,13 +51,17 @@ struct RecArgs {
 struct FftClass {
   int lc = 0, kind = 0;    // length class and transform kind (0 direct, 1 pow2, 2 chirp-z)
   int block = 64;          // threads per work-group
-  int lds_doubles = 0;     // dynamic LDS (doubles)
+  int lds_bytes = 0;       // dynamic LDS of the ring kernels (bytes)
+  int lds64_bytes = 0;     // the same for the FP64 plan-time spectrum kernel
   int npairs = 0;
   int* pairs = nullptr;    // device list of ring-pair indices p (0 <= p < 2 nside)
 };
 
 struct FftTables {
   double2* tw = nullptr;  // exp(-2 pi i k / lmax_fft), k < lmax_fft
+  float2* twf = nullptr;  // FP32 copies of tw, bspec and chirps (mixed-precision plans)
+  float2* bspecf = nullptr;
+  float2* chirpsf = nullptr;
   int lmax_fft = 0;       // largest power-of-two transform length in the plan
   FftClass cls[12];
   int ncls = 0;
@@ -88,7 +92,7 @@ struct jhtx_plan {
   // stage timing (jhtx_plan_set_profiling)
   int profiling;
   void* prof;  // jhtx::Profile*
-  // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2]
+  // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2 (FP64) / float2 (mixed)]
   // [chunk partials: nchunk*batch*ncomp*K double2 (FP64 path) or float2 (mixed path)]
   size_t ws_F_bytes, ws_part_bytes, ws_bytes;
 };
@@ -106,15 +110,16 @@ hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* a
                              int nchunk, hipStream_t s);
 hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);
-hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
                               hipStream_t s);
-hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, float2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int batch,
                             hipStream_t s);
 // ring chunks (partials per map) of the mixed adjoint
 int mixed_adj_chunks(const jhtx_plan* p);
-hipError_t launch_ring_synth(const jhtx_plan* p, const double2* F, double* map, int batch,
+// F / V: complex128 (FP64 plans) or complex64 (mixed-precision plans)
+hipError_t launch_ring_synth(const jhtx_plan* p, const void* F, double* map, int batch,
                              hipStream_t s);
-hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, double2* V, int batch,
+hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, int batch,
                                hipStream_t s);
 
 // FFT plan building (host): fills p->fft

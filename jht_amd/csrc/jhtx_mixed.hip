@@ -64,6 +64,9 @@ __device__ __forceinline__ void rescale1(double& mu, double& mup, int& ex) {
   }
 }
 
+// ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
+__device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -108,7 +111,7 @@ __device__ __forceinline__ void s2_steps8(const double2 (&ag)[8], const floatx4 
 
 template <int NB, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const double2* __restrict__ alm,
-                                                     double2* __restrict__ F, int batch) {
+                                                     float2* __restrict__ F, int batch) {
   constexpr int MG = 4 * NB;
   __shared__ double2 s_ag[CH];
   __shared__ __align__(16) float s_a[CH][4][MG];  // [l][component: aE.re aE.im aB.re aB.im][map]
@@ -223,26 +226,26 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
         M1[k] = s_ep[wave][1][1][lane][k];
       }
       // PE = (P[0] + i P[1]), PB = (P[2] + i P[3]); Fp = -PE - i PB ; Fm = -ME + i MB
-      double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
-      double2* Fm = Fp + (long long)A.nrings * ringM;
+      float2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+      float2* Fm = Fp + (long long)A.nrings * ringM;
       const double pe_r = P0[0] + P1[0], pe_i = P0[1] + P1[1], pb_r = P0[2] + P1[2], pb_i = P0[3] + P1[3];
       const double me_r = M0[0] + M1[0], me_i = M0[1] + M1[1], mb_r = M0[2] + M1[2], mb_i = M0[3] + M1[3];
-      Fp[(long long)t * ringM + m] = make_double2(-pe_r + pb_i, -pe_i - pb_r);
-      Fm[(long long)t * ringM + m] = make_double2(-me_r - mb_i, -me_i + mb_r);
+      Fp[(long long)t * ringM + m] = f2(-pe_r + pb_i, -pe_i - pb_r);
+      Fm[(long long)t * ringM + m] = f2(-me_r - mb_i, -me_i + mb_r);
       if (t < A.T - 1) {
         const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
         const double de_r = M0[0] - M1[0], de_i = M0[1] - M1[1], db_r = M0[2] - M1[2], db_i = M0[3] - M1[3];
         const double qe_r = P0[0] - P1[0], qe_i = P0[1] - P1[1], qb_r = P0[2] - P1[2], qb_i = P0[3] - P1[3];
-        Fp[rs] = make_double2(-de_r + db_i, -de_i - db_r);
-        Fm[rs] = make_double2(-qe_r - qb_i, -qe_i + qb_r);
+        Fp[rs] = f2(-de_r + db_i, -de_i - db_r);
+        Fm[rs] = f2(-qe_r - qb_i, -qe_i + qb_r);
       }
     } else if (valid && b < batch) {
-      double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
-      double2* Fm = Fp + (long long)A.nrings * ringM;
-      Fp[(long long)t * ringM + m] = Fm[(long long)t * ringM + m] = make_double2(0.0, 0.0);
+      float2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+      float2* Fm = Fp + (long long)A.nrings * ringM;
+      Fp[(long long)t * ringM + m] = Fm[(long long)t * ringM + m] = f2(0.0, 0.0);
       if (t < A.T - 1) {
         const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
-        Fp[rs] = Fm[rs] = make_double2(0.0, 0.0);
+        Fp[rs] = Fm[rs] = f2(0.0, 0.0);
       }
     }
     wave_sync();
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
 // ---------------------------------------------------------------------------------- //
 template <int MG>
 __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* __restrict__ alm,
-                                                     double2* __restrict__ F, int batch) {
+                                                     float2* __restrict__ F, int batch) {
   constexpr int NQ = MG / 2;
   __shared__ double s_al[CH];
   static_assert(NQ % 4 == 0, "operands are read as float4");
@@ -356,9 +359,9 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
         if (b >= batch) continue;
         const double c0r = s_ep[wave][0][lane][2 * h], c0i = s_ep[wave][0][lane][2 * h + 1];
         const double c1r = s_ep[wave][1][lane][2 * h], c1i = s_ep[wave][1][lane][2 * h + 1];
-        double2* Fb = F + (long long)b * A.nrings * ringM;
-        Fb[(long long)t * ringM + m] = make_double2(c0r + c1r, c0i + c1i);
-        if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * ringM + m] = make_double2(c0r - c1r, c0i - c1i);
+        float2* Fb = F + (long long)b * A.nrings * ringM;
+        Fb[(long long)t * ringM + m] = f2(c0r + c1r, c0i + c1i);
+        if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * ringM + m] = f2(c0r - c1r, c0i - c1i);
       }
     }
     wave_sync();
@@ -431,7 +434,7 @@ __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, co
 // NW waves per work-group cover 64 NW rings (one partial per chunk of rings); MINB: the
 // register budget, in work-groups of this size per CU (NW * MINB = 8 caps it at 256 VGPRs)
 template <int NCB, int MINB, int NW>
-__global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const double2* __restrict__ V,
+__global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const float2* __restrict__ V,
                                                    float2* __restrict__ part, int batch) {
   constexpr int MG = 4 * NCB;
   constexpr int RINGS = 64 * NW;
@@ -476,8 +479,8 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
       const int b = mg * MG + 4 * c + g;
-      const double* Vp = reinterpret_cast<const double*>(V + ((long long)b * 2 + 0) * A.nrings * ringM);
-      const double* Vm = reinterpret_cast<const double*>(V + ((long long)b * 2 + 1) * A.nrings * ringM);
+      const float* Vp = reinterpret_cast<const float*>(V + ((long long)b * 2 + 0) * A.nrings * ringM);
+      const float* Vm = reinterpret_cast<const float*>(V + ((long long)b * 2 + 1) * A.nrings * ringM);
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int tr = chunk * RINGS + wave * 64 + 4 * q + (lane >> 4);
@@ -487,11 +490,11 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
           const bool south = tr < A.T - 1;  // the equator ring has no mirror
           const long long rs = 2 * ((long long)(4 * A.nside - 2 - tr) * ringM + m) + ri;
           if (lo) {
-            bp = Vp[rn];
-            bm = south ? par0 * Vp[rs] : 0.0;
+            bp = (double)Vp[rn];
+            bm = south ? par0 * (double)Vp[rs] : 0.0;
           } else {
-            bp = south ? par0 * Vm[rs] : 0.0;
-            bm = Vm[rn];
+            bp = south ? par0 * (double)Vm[rs] : 0.0;
+            bm = (double)Vm[rn];
           }
         }
         Bv[c][0][q] = (float)bp;
@@ -581,7 +584,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const do
 }
 
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
-__global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __restrict__ V,
+__global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __restrict__ V,
                                                    float2* __restrict__ part, int batch) {
   constexpr int MG = 8;
   __shared__ double s_al[CH];
@@ -613,11 +616,11 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const double2* __r
       const int tr = chunk * 256 + wave * 64 + 4 * q + (lane >> 4);
       double vn = 0.0, vs = 0.0;
       if (tr < A.T && b < batch) {
-        const double2* Vb = V + (long long)b * A.nrings * ringM;
-        const double2 n = Vb[(long long)tr * ringM + m];
+        const float2* Vb = V + (long long)b * A.nrings * ringM;
+        const float2 n = Vb[(long long)tr * ringM + m];
         vn = c ? n.y : n.x;
         if (tr < A.T - 1) {
-          const double2 s = Vb[(long long)(4 * A.nside - 2 - tr) * ringM + m];
+          const float2 s = Vb[(long long)(4 * A.nside - 2 - tr) * ringM + m];
           vs = c ? s.y : s.x;
         }
       }
@@ -713,7 +716,7 @@ int mixed_group(const jhtx_plan* p, bool adjoint) {
   return adjoint ? p->mx_adj_group : p->mx_synth_group;
 }
 
-hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
                               hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 7) & ~7;
@@ -732,7 +735,7 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, double2* F
   return hipGetLastError();
 }
 
-hipError_t launch_adj_mixed(const jhtx_plan* p, const double2* V, float2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int batch,
                             hipStream_t s) {
   RecArgs a = make_args_mixed(p);
   a.nchunk = mixed_adj_chunks(p);
