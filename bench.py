@@ -4,7 +4,7 @@ Workload (the metric's configuration): spin 2, nside = 2048, lmax = 4096, E/B a_
 C_l^EE = (l+1)^-2, C_l^BB = 0.1 C_l^EE, l < 2 zero (synthetic, seeded).  One step is one
 synthesis S a -> (Q, U) followed by one adjoint S^T (Q, U) -> a' for every map of the
 batch; inputs are resident in HBM before the timed region.  `value` is maps/s of the
-selected precision (default: mixed, batch 8); the FP64 path (batch 1) is timed beside it
+selected precision (default: mixed, batch 16); the FP64 path (batch 1) is timed beside it
 and the mixed outputs of the first map are compared with the FP64 GPU outputs
 (max|x - x64| / max|x64|).  Prints one JSON line.
 
@@ -129,13 +129,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="mixed", choices=["f64", "mixed"])
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=4096)
     ap.add_argument("--spin", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=12)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
@@ -148,6 +151,13 @@ def main():
     import jht_amd
     from jht_amd import _lib
     from jht_amd.healpix import get_plan
+
+    if args.copy_probe_mb > 0:
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     nside, lmax, spin, B = args.nside, args.lmax, args.spin, args.batch
     prec = _lib.PREC_MIXED if args.precision == "mixed" else _lib.PREC_F64

@@ -61,7 +61,7 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *   block                    FP64 recursion work-group size cap {64,128,192,256}
  *   mx_synth_group           maps per recursion, mixed spin-2 synthesis {4,8}
  *   mx_synth_occ             its register budget in work-groups per CU {1,2,3}
- *   mx_adj_group, mx_adj_occ the same for the mixed spin-2 adjoint {4,8}, {1,2}
+ *   mx_adj_group, mx_adj_occ the same for the mixed spin-2 adjoint {4,8,16}, {1,2}
  *   mx_adj_wide              mixed spin-2 adjoint with 512-ring work-groups {1 on, 2 off} */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;

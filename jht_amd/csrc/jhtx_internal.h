@@ -18,6 +18,8 @@ namespace jhtx {
 constexpr double kBig = 0x1p256;
 constexpr double kRescale = 0x1p-512;
 constexpr int kScaleStep = 512;
+// consecutive orders m per XCD group in the recursion kernels' work-group order
+constexpr int kMGroup = 16;
 
 // Per-plan device tables for the recursion kernels.
 struct RecTables {

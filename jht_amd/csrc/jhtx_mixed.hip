@@ -47,12 +47,26 @@ __device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, doub
   return L;
 }
 
+// Work-groups are dealt round-robin to the 8 XCDs (blockIdx & 7); consecutive orders m
+// alternate between XCDs so cheap high-m and costly low-m columns spread evenly.
+// GROUPED (the adjoint kernels): each XCD takes groups of kMGroup consecutive m, the
+// groups alternating between XCDs, and within a group the work-groups of one ring chunk
+// run back to back -- they read adjacent m of the same ring rows of V, so those cache
+// lines are shared in the XCD's L2 instead of being fetched once per m (7x less fetch).
+template <bool GROUPED>
 __device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
   const int P = blockIdx.x;
   const int xcd = P & 7;
   const int slot = P >> 3;
-  chunk = slot % A.nchunk;
-  m = (slot / A.nchunk) * 8 + xcd;
+  if (GROUPED) {
+    const int mi = slot % kMGroup;
+    chunk = (slot / kMGroup) % A.nchunk;
+    const int g = slot / (kMGroup * A.nchunk);
+    m = (g * 8 + xcd) * kMGroup + mi;
+  } else {
+    chunk = slot % A.nchunk;
+    m = (slot / A.nchunk) * 8 + xcd;
+  }
   return m < A.M;
 }
 
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
   __shared__ __align__(16) float s_a[CH][4][MG];  // [l][component: aE.re aE.im aB.re aB.im][map]
   __shared__ float s_ep[4][2][2][64][4];          // [wave][class][sign][ring][component]
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<false>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -264,7 +278,7 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
   __shared__ __align__(16) float s_a[CH][4][NQ];  // [l][lane%4][q]
   __shared__ float s_ep[4][2][64][4];  // [wave][class][ring][lane%4]
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<false>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -442,9 +456,8 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   __shared__ float tiles[NW][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
   static_assert(sizeof(double) * NW * 2 * 16 * 16 * NCB <= sizeof(float) * NW * 2 * TL * TS,
                 "the FP64 reduction buffer aliases the tiles");
-  static_assert(2 * 16 * MG <= RINGS, "one thread per reduced (parity, row, map)");
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -563,9 +576,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
             myred[(p * 16 + i) * NC + 16 * c + jc] = sig * (double)D[c][p][r];
           }
       __syncthreads();
-      // thread -> (p, i, g): sums 4 waves for the 4 components of map g at l = l0+jb+2i+p
-      if (tid < 2 * 16 * MG) {
-        const int p = tid / (16 * MG), i = (tid / MG) % 16, g = tid % MG;
+      // thread -> (p, i, g): sums the waves for the 4 components of map g at l = l0+jb+2i+p
+      for (int r = tid; r < 2 * 16 * MG; r += RINGS) {
+        const int p = r / (16 * MG), i = (r / MG) % 16, g = r % MG;
         const int l = l0 + jb + 2 * i + p;
         const int b = mg * MG + g;
         if (l <= lmax && b < batch) {
@@ -591,7 +604,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __re
   __shared__ float tiles[4][TL][TS];  // [wave][l][ring] (one sign)
   __shared__ double redbuf[4][2][16][16];
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -719,7 +732,7 @@ int mixed_group(const jhtx_plan* p, bool adjoint) {
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
                               hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
-  const int Mpad = (p->M + 7) & ~7;
+  const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, false);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
@@ -739,11 +752,13 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
                             hipStream_t s) {
   RecArgs a = make_args_mixed(p);
   a.nchunk = mixed_adj_chunks(p);
-  const int Mpad = (p->M + 7) & ~7;
+  const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, true);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
+  else if (MG == 16)
+    hipLaunchKernelGGL((k_adj_mx_s2<4, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 8 && mixed_adj_waves(p) == 8)
     hipLaunchKernelGGL((k_adj_mx_s2<2, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 8 && p->mx_adj_occ == 2)

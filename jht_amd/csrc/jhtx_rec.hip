@@ -46,12 +46,26 @@ __device__ __forceinline__ void seed_split(double L, double sign, double& mu, in
 // over the 8 XCDs, so all ring chunks of one m are placed on the same XCD (shared L2 for
 // the a_lm column and coefficients) and each XCD sweeps m in increasing order (longest
 // columns first).
+// Work-groups are dealt round-robin to the 8 XCDs (blockIdx & 7); consecutive orders m
+// alternate between XCDs so cheap high-m and costly low-m columns spread evenly.
+// GROUPED (the adjoint kernels): each XCD takes groups of kMGroup consecutive m, the
+// groups alternating between XCDs, and within a group the work-groups of one ring chunk
+// run back to back -- they read adjacent m of the same ring rows of V, so those cache
+// lines are shared in the XCD's L2 instead of being fetched once per m (7x less fetch).
+template <bool GROUPED>
 __device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
   const int P = blockIdx.x;
   const int xcd = P & 7;
   const int slot = P >> 3;
-  chunk = slot % A.nchunk;
-  m = (slot / A.nchunk) * 8 + xcd;
+  if (GROUPED) {
+    const int mi = slot % kMGroup;
+    chunk = (slot / kMGroup) % A.nchunk;
+    const int g = slot / (kMGroup * A.nchunk);
+    m = (g * 8 + xcd) * kMGroup + mi;
+  } else {
+    chunk = slot % A.nchunk;
+    m = (slot / A.nchunk) * 8 + xcd;
+  }
   return m < A.M;
 }
 
@@ -106,7 +120,7 @@ __global__ __launch_bounds__(256) void k_synth_s0(RecArgs A, const double2* __re
   __shared__ double s_al[CH];
   __shared__ double2 s_a[CH];
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<false>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lmax = A.lmax;
@@ -244,7 +258,7 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
   __shared__ double2 s_e[CH];
   __shared__ double2 s_b[CH];
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<false>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lmax = A.lmax;
@@ -413,7 +427,7 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
   __shared__ double s_al[CH];
   __shared__ double2 red[4 * NV * 64];
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int batch = gridDim.y;
   const int tid = threadIdx.x;
@@ -541,7 +555,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
   __shared__ double2 s_ag[CH];
   __shared__ double2 red[4 * NV * 64];
   int m, chunk;
-  if (!decode_block(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int batch = gridDim.y;
   const int tid = threadIdx.x;
@@ -709,7 +723,7 @@ RecArgs make_args(const jhtx_plan* p, bool adjoint) {
 hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                             hipStream_t s) {
   const RecArgs a = make_args(p, false);
-  const int Mpad = (p->M + 7) & ~7;
+  const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   dim3 grid(Mpad * p->nchunk, batch);
   dim3 block(p->rec_block);
   if (p->spin == 0) {
@@ -731,7 +745,7 @@ hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, 
 hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
                           int batch, hipStream_t s) {
   const RecArgs a = make_args(p, true);
-  const int Mpad = (p->M + 7) & ~7;
+  const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   dim3 grid(Mpad * p->adj_nchunk, batch);
   dim3 block(p->adj_block);
   if (p->spin == 0) {

@@ -25,7 +25,7 @@ _CANDIDATES = {
     ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)], [{"adj_rings": r} for r in (2, 4, 1)]),
     ("mixed", 2): ([{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
                    [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_wide": w}
-                    for g, o, w in ((8, 2, 1), (8, 2, 2), (4, 1, 2), (8, 1, 2))]),
+                    for g, o, w in ((16, 2, 2), (8, 2, 2), (8, 2, 1), (4, 1, 2), (8, 1, 2))]),
 }
 
 
