@@ -348,7 +348,10 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   };
   // maps per recursion follow the batch the plan is sized for (a partly empty group costs
   // matrix-core time, a larger one amortises the FP64 recursion over more maps)
-  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", max_batch >= 5 ? 8 : 4, 4, 8);
+  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", max_batch >= 16 ? 16 : (max_batch >= 5 ? 8 : 4), 4, 8);
+  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
+    if (atoi(v) == 16) p->mx_synth_group = 16;  // the 16x16x4 tile kernel
+  }
   p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
   if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
     if (atoi(v) == 3) p->mx_synth_occ = 3;
@@ -376,7 +379,7 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
         (t.synth_rings == 0 || (spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
         (t.adj_rings == 0 || (spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
         (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
-        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8})) &&
+        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16})) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2}));

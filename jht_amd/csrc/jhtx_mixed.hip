@@ -89,6 +89,60 @@ __device__ __forceinline__ void wave_sync() {
 
 enum { FULL = 0, MASKED = 1, RECONLY = 2 };
 
+// Stage G_l a_lm of MG spin-2 maps for the CH orders l0..l0+CH-1 into LDS as FP32.  Thread
+// t takes l = l0 + t % CH and (map, E/B) pairs t / CH, t / CH + 256 / CH, ...: a wave reads
+// 64 consecutive coefficients of one column (coalesced), G_l is loaded once, and all of the
+// thread's loads are issued before the first use (one memory latency per chunk).
+template <int MG>
+__device__ __forceinline__ void load_alm_s2(const RecArgs& A, const double2* __restrict__ alm, long long cb,
+                                            int l0, int lmax, int mg, int batch, int tid,
+                                            double2 (&a)[2 * MG * CH / 256], double& g) {
+  constexpr int NP = 2 * MG * CH / 256;  // (map, E/B) pairs per thread
+  static_assert(NP * 256 == 2 * MG * CH, "256 threads cover the chunk exactly");
+  const int j = tid % CH, pr0 = tid / CH;
+  const int l = l0 + j;
+  const bool lok = l <= lmax;
+  g = lok ? A.G[cb + l] : 0.0;
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    const int pr = pr0 + u * (256 / CH);  // pair = 2 map + E/B
+    const int b = mg * MG + (pr >> 1);
+    a[u] = (lok && b < batch) ? alm[((long long)b * 2 + (pr & 1)) * A.K + cb + l] : make_double2(0.0, 0.0);
+  }
+}
+
+// the tile kernel's layout: s_a[j * AS + 4 map + comp], comp = (E.re, E.im, B.re, B.im)
+template <int MG, int AS>
+__device__ __forceinline__ void store_alm_s2(const double2 (&a)[2 * MG * CH / 256], double g, int tid, float* s_a) {
+  constexpr int NP = 2 * MG * CH / 256;
+  const int j = tid % CH, pr0 = tid / CH;
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    const int pr = pr0 + u * (256 / CH);
+    float* d = s_a + j * AS + 4 * (pr >> 1) + 2 * (pr & 1);
+    d[0] = (float)(a[u].x * g);
+    d[1] = (float)(a[u].y * g);
+  }
+}
+
+// the rank-1 kernel's layout: s_a[j][comp][map]
+template <int MG>
+__device__ __forceinline__ void stage_alm_s2_cm(const RecArgs& A, const double2* __restrict__ alm, long long cb,
+                                                int l0, int lmax, int mg, int batch, int tid, float* s_a) {
+  constexpr int NP = 2 * MG * CH / 256;
+  double2 a[NP];
+  double g;
+  load_alm_s2<MG>(A, alm, cb, l0, lmax, mg, batch, tid, a, g);
+  const int j = tid % CH, pr0 = tid / CH;
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    const int pr = pr0 + u * (256 / CH);
+    const int map = pr >> 1, c0 = 2 * (pr & 1);
+    s_a[(j * 4 + c0) * MG + map] = (float)(a[u].x * g);
+    s_a[(j * 4 + c0 + 1) * MG + map] = (float)(a[u].y * g);
+  }
+}
+
 // ---------------------------------------------------------------------------------- //
 // spin 2 synthesis, MG = 4 NB maps per work-group (grid.y = map group)
 // ---------------------------------------------------------------------------------- //
@@ -164,17 +218,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
       if (l <= lmax && l != lmin) ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
       s_ag[tid] = ag;
     }
-    for (int i = tid; i < CH * 4 * MG; i += 256) {
-      const int j = i / (4 * MG), comp = (i / MG) & 3, g = i % MG;
-      const int l = l0 + j, b = mg * MG + g;
-      float v = 0.f;
-      if (l <= lmax && b < batch) {
-        const long long idx = cb + l;
-        const double2 a = alm[((long long)b * 2 + (comp >> 1)) * A.K + idx];
-        v = (float)(((comp & 1) ? a.y : a.x) * A.G[idx]);
-      }
-      s_a[j][comp][g] = v;
-    }
+    stage_alm_s2_cm<MG>(A, alm, cb, l0, lmax, mg, batch, tid, &s_a[0][0][0]);
     __syncthreads();
     const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
     int j0 = 0;
@@ -260,6 +304,183 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
       if (t < A.T - 1) {
         const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
         Fp[rs] = Fm[rs] = f2(0.0, 0.0);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// spin 2 synthesis on 16x16x4 tiles, MG = 4 NCB maps per work-group (grid.y = map group)
+// ---------------------------------------------------------------------------------- //
+// Lane = (ring, sign): lanes 0-31 run the +2 recursion of 32 rings, lanes 32-63 the -2
+// recursion of the same rings, so a wave holds one recursion per lane.  Every 8 l-steps
+// the wave's FP32 values go to an LDS tile [step][lane]; v_mfma_f32_16x16x4f32 then
+// contracts 4 same-parity steps (K) of 16 (ring, sign) rows (M) against 16 columns
+// (4 maps x 4 components, N) per instruction: 4 row blocks x 2 parities x NCB column
+// blocks = 8 NCB instructions per 8 steps, 32 cycles each, for 32 rings x MG maps.  Against
+// the rank-1 4x4x1 form this issues a quarter of the matrix instructions and runs half the
+// FP64 instructions per lane, for the same matrix-core work.
+constexpr int kTileTS = 72;  // tile row stride (floats): A reads at rows 2k+p, bank 16k+i
+
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const double2* __restrict__ alm,
+                                                             float2* __restrict__ F, int batch) {
+  constexpr int MG = 4 * NCB;
+  constexpr int AS = 4 * MG + 8;  // s_a row stride (floats): B reads at rows 2k+p, bank 16k+j
+  constexpr int RINGS = 128;      // rings per work-group (4 waves x 32)
+  // loop: s_a [CH][AS] and the wave tiles; epilogue: per-wave staging of D (aliases both)
+  constexpr int LOOP_FLOATS = CH * AS + 4 * 8 * kTileTS;
+  constexpr int EPI_FLOATS = 4 * 2 * 64 * 16;
+  __shared__ double2 s_ag[CH][2];  // [l][sign]: (alpha, +gamma) and (alpha, -gamma)
+  __shared__ __align__(16) float s_buf[LOOP_FLOATS > EPI_FLOATS ? LOOP_FLOATS : EPI_FLOATS];
+  float* s_a = s_buf;                  // [l][16 cb + 4 g + component]
+  float* tiles = s_buf + CH * AS;      // [wave][step][lane]
+  int m, chunk;
+  if (!decode_block<false>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r32 = lane & 31, sg = lane >> 5;  // ring within the wave, recursion sign
+  const int ii = lane & 15, kk = lane >> 4;   // MFMA lane decomposition
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb0 = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + wave * 32 + r32;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid && lmin <= lmax) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m + 4 * sg;
+    double sv;
+    seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
+    mup = -sv;
+  }
+  floatx4 D[2][4][NCB];  // [parity][row block][column block]
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float* mytile = tiles + wave * 8 * kTileTS;
+  // the next chunk's coefficients are loaded into registers while this chunk computes
+  constexpr int NP = 2 * MG * CH / 256;
+  double2 pf_a[NP], pf_ag = make_double2(0.0, 0.0);
+  double pf_g = 0.0;
+  auto prefetch = [&](int l0n) {
+    if (tid < 2 * CH) {
+      const int l = l0n + (tid >> 1);
+      pf_ag = make_double2(0.0, 0.0);
+      if (l <= lmax && l != lmin)
+        pf_ag = make_double2(A.alpha[cb0 + l], (tid & 1) ? -A.gamma[cb0 + l] : A.gamma[cb0 + l]);
+    }
+    load_alm_s2<MG>(A, alm, cb0, l0n, lmax, mg, batch, tid, pf_a, pf_g);
+  };
+  if (lmin <= lmax) prefetch(lmin);
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < 2 * CH) s_ag[tid >> 1][tid & 1] = pf_ag;
+    store_alm_s2<MG, AS>(pf_a, pf_g, tid, s_a);
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    int j0 = 0;
+    // every lane below 2^-256: recursion only (a lane's exponent never decreases)
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+      double2 ag[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[j0 + jj][sg];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+        mup = mu;
+        mu = n;
+      }
+      rescale1(mu, mup, ex);
+      j0 += 8;
+    }
+    for (; j0 < nst; j0 += 8) {
+      const int anydeep = __ballot(valid && ex < 0) != 0ull;
+      const double f = (ex == 0) ? 1.0 : 0.0;
+      double2 ag[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[j0 + jj][sg];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+        mup = mu;
+        mu = n;
+        // the 1/0 mask in FP64 first: a lane below 2^-256 may hold a mantissa above FP32 range
+        mytile[jj * kTileTS + lane] = (float)(n * f);
+      }
+      wave_sync();
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        float bv[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) bv[c] = s_a[(j0 + 2 * kk + p) * AS + 16 * c + ii];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const float av = mytile[(2 * kk + p) * kTileTS + 16 * rb + ii];
+#pragma unroll
+          for (int c = 0; c < NCB; ++c)
+            D[p][rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], D[p][rb][c], 0, 0, 0);
+        }
+      }
+      wave_sync();
+      if (anydeep) rescale1(mu, mup, ex);
+    }
+  }
+  // epilogue, one column block (4 maps) at a time: D[i][j] sits in lane j + 16 (i / 4),
+  // register i % 4; row i of block rb is lane 16 rb + i of the recursion (sign = rb / 2)
+  __syncthreads();  // the staging below aliases s_a and the tiles
+  float* ep = s_buf + wave * 2 * 64 * 16;  // [parity][row 0..63][column 0..15]
+  const int swap = (lmin - m) & 1;         // classes counted from lmin; reflection needs (l - m)
+  const long long ringM = A.M;
+  const int tt = chunk * RINGS + wave * 32 + r32;  // this lane's output ring
+  const int half = lane >> 5;                      // this lane's maps: 2 half, 2 half + 1
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep[((p ^ swap) * 64 + 16 * rb + 4 * kk + r) * 16 + ii] = D[p][rb][c][r];
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int g = 2 * half + h;       // map within the column block
+      const int b = mg * MG + 4 * c + g;
+      if (tt < A.T && b < batch) {
+        float2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+        float2* Fm = Fp + (long long)A.nrings * ringM;
+        const long long rs = (long long)(4 * A.nside - 2 - tt) * ringM + m;
+        if (lmin <= lmax) {
+          double P0[4], P1[4], M0[4], M1[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            P0[k] = ep[(0 * 64 + r32) * 16 + 4 * g + k];
+            P1[k] = ep[(1 * 64 + r32) * 16 + 4 * g + k];
+            M0[k] = ep[(0 * 64 + 32 + r32) * 16 + 4 * g + k];
+            M1[k] = ep[(1 * 64 + 32 + r32) * 16 + 4 * g + k];
+          }
+          // PE = (P[0] + i P[1]), PB = (P[2] + i P[3]); Fp = -PE - i PB ; Fm = -ME + i MB
+          const double pe_r = P0[0] + P1[0], pe_i = P0[1] + P1[1], pb_r = P0[2] + P1[2], pb_i = P0[3] + P1[3];
+          const double me_r = M0[0] + M1[0], me_i = M0[1] + M1[1], mb_r = M0[2] + M1[2], mb_i = M0[3] + M1[3];
+          Fp[(long long)tt * ringM + m] = f2(-pe_r + pb_i, -pe_i - pb_r);
+          Fm[(long long)tt * ringM + m] = f2(-me_r - mb_i, -me_i + mb_r);
+          if (tt < A.T - 1) {
+            const double de_r = M0[0] - M1[0], de_i = M0[1] - M1[1], db_r = M0[2] - M1[2], db_i = M0[3] - M1[3];
+            const double qe_r = P0[0] - P1[0], qe_i = P0[1] - P1[1], qb_r = P0[2] - P1[2], qb_i = P0[3] - P1[3];
+            Fp[rs] = f2(-de_r + db_i, -de_i - db_r);
+            Fm[rs] = f2(-qe_r - qb_i, -qe_i + qb_r);
+          }
+        } else {
+          Fp[(long long)tt * ringM + m] = Fm[(long long)tt * ringM + m] = make_float2(0.f, 0.f);
+          if (tt < A.T - 1) Fp[rs] = Fm[rs] = make_float2(0.f, 0.f);
+        }
       }
     }
     wave_sync();
@@ -519,15 +740,19 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   float* mytile = &tiles[wave][0][0][0];
   double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16 l][16 NCB cols] FP64
   constexpr int NC = 16 * NCB;
+  // the next chunk's (alpha, gamma) are loaded into registers while this chunk computes
+  double2 pf_ag = make_double2(0.0, 0.0);
+  auto prefetch = [&](int l0n) {
+    const int l = l0n + tid;
+    pf_ag = make_double2(0.0, 0.0);
+    if (tid < CH && l <= lmax && l != lmin) pf_ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+  };
+  prefetch(lmin);
   for (int l0 = lmin; l0 <= lmax; l0 += CH) {
     __syncthreads();
-    if (tid < CH) {
-      const int l = l0 + tid;
-      double2 ag = make_double2(0.0, 0.0);
-      if (l <= lmax && l != lmin) ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
-      s_ag[tid] = ag;
-    }
+    if (tid < CH) s_ag[tid] = pf_ag;
     __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
 #pragma unroll 1
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
       // recursion for TL steps, values to the wave's tile
@@ -735,7 +960,12 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, false);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 0)
+  if (p->spin == 2 && MG == 16) {
+    RecArgs at = a;
+    at.nchunk = (p->T + 127) / 128;
+    dim3 gt(Mpad * at.nchunk, (batch + 15) / 16);
+    hipLaunchKernelGGL((k_synth_tile_s2<4, 2>), gt, dim3(256), 0, s, at, alm, F, batch);
+  } else if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8 && p->mx_synth_occ == 3)
     hipLaunchKernelGGL((k_synth_mx_s2<2, 3>), grid, dim3(256), 0, s, a, alm, F, batch);

@@ -192,7 +192,8 @@ def test_mixed_matches_f64(nside, lmax, spin, batch):
 
 
 @pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ",
-                         [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2)])
+                         [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2),
+                          (16, 2, 16, 2), (16, 2, 4, 1)])
 def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkeypatch):
     """The 8-maps-per-recursion spin-2 kernels (both register budgets), forced through the
     plan's tuning overrides, against the FP64 path; batch 11 leaves a ragged last group."""
