@@ -35,6 +35,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+# HBM bytes per call of the dominant stage, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of
+# this script (each counter in its own run, scaled by the counters of a 1 GiB device copy:
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax, spin, batch, stage)
+MEASURED_TRAFFIC = {
+    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 95.0e9,
+}
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
@@ -241,7 +247,8 @@ def main():
                    "precision": args.precision, "parallelism": "single-gpu"},
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None},
+                     "frac": round(achieved / peak, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, nside, lmax, spin, B, dom))},
         "stages_ms": {k: round(v[0], 3) for k, v in per.items()},
         "adjoint_identity": {"lhs": lhs, "rhs": rhs, "rel": ident},
     }
