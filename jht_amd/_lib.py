@@ -39,7 +39,7 @@ FLAG_BLUESTEIN_ALL = 1
 
 
 TUNING_FIELDS = ("synth_rings", "adj_rings", "block", "mx_synth_group", "mx_synth_occ",
-                 "mx_adj_group", "mx_adj_occ", "mx_adj_wide")
+                 "mx_adj_group", "mx_adj_occ", "mx_adj_wide", "f64_adj_mfma")
 
 
 class Tuning(ctypes.Structure):

@@ -278,6 +278,7 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->mx_adj_group = p->mx_adj_group;
   t->mx_adj_occ = p->mx_adj_occ;
   t->mx_adj_wide = p->mx_adj_wide ? 1 : 2;
+  t->f64_adj_mfma = p->f64_adj_mfma ? 1 : 2;
   return JHTX_OK;
 }
 
@@ -362,6 +363,7 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   }
   p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
   p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
+  p->f64_adj_mfma = (spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);
@@ -382,7 +384,8 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
         (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16})) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
-        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2}));
+        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
+        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2}));
     if (!ok) {
       delete p;
       set_error("jhtx_tuning field out of range");
@@ -396,6 +399,7 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
     if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
     if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
     if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
+    if (t.f64_adj_mfma && spin == 2) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
   }
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
@@ -409,7 +413,10 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   const size_t f_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
   p->ws_F_bytes = align256(f_elem * (size_t)max_batch * p->ncomp * p->nrings * p->M);
   // the mixed kernels use one ring per lane and 256-lane work-groups
-  const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p) : p->adj_nchunk;
+  // the FP64 matrix-core adjoint writes one partial per 256 rings
+  const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p)
+                          : p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256)
+                                            : p->adj_nchunk;
   const size_t part_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
   p->ws_part_bytes = align256(part_elem * (size_t)part_chunks * max_batch * p->ncomp * p->K);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;

@@ -22,7 +22,8 @@ from .healpix import Plan, _prec, alm_size, set_tuning
 # candidate shapes per (precision, spin): synthesis fields, adjoint fields
 _CANDIDATES = {
     ("f64", 0): ([{"synth_rings": r} for r in (1, 4, 8)], [{"adj_rings": r} for r in (4, 8, 1)]),
-    ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)], [{"adj_rings": r} for r in (2, 4, 1)]),
+    ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)],
+                 [{"f64_adj_mfma": 1}] + [{"f64_adj_mfma": 2, "adj_rings": r} for r in (2, 4, 1)]),
     ("mixed", 2): ([{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
                    [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_wide": w}
                     for g, o, w in ((16, 2, 2), (8, 2, 2), (8, 2, 1), (4, 1, 2), (8, 1, 2))]),

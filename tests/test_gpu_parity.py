@@ -138,13 +138,15 @@ def test_input_validation_gpu():
     (0, 4, 4, 64), (0, 8, 8, 64), (0, 1, 1, 64), (2, 2, 2, 64), (2, 4, 4, 64), (2, 4, 4, 256), (0, 8, 8, 256)])
 def test_lane_variants(spin, synth_r, adj_r, block, monkeypatch):
     """Every rings-per-lane instantiation and multi-chunk ring reductions (forced through
-    the plan's tuning overrides) against the oracle."""
+    the plan's tuning overrides) against the oracle; spin 2 runs the vector-ALU adjoint
+    (the matrix-core variant is compared with it below)."""
     j = _jht()
     from jht_amd import healpix as H
 
     monkeypatch.setenv("JHTX_SYNTH_R", str(synth_r))
     monkeypatch.setenv("JHTX_ADJ_R", str(adj_r))
     monkeypatch.setenv("JHTX_REC_BLOCK", str(block))
+    monkeypatch.setenv("JHTX_F64_ADJ_MFMA", "0")  # the vector-ALU spin-2 adjoint with adj_r
     H.clear_plans()
     try:
         nside, lmax = 128, 200
@@ -247,3 +249,28 @@ def test_mixed_adjoint_ring_chunks(wide, monkeypatch):
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
     print(f"mixed adjoint wide={wide}: {ea:.2e}", flush=True)
     assert ea <= MIXED_TOL
+
+
+@pytest.mark.parametrize("nside,lmax", [(8, 12), (64, 128), (128, 300), (256, 200)])
+def test_f64_adjoint_kernels_agree(nside, lmax, monkeypatch):
+    """Spin-2 FP64 adjoint on the matrix cores against the vector-ALU kernel and the oracle
+    (multi-chunk grids, lmax above and below the ring count)."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    rng = np.random.default_rng(900 + nside)
+    v = rng.standard_normal((2, 12 * nside * nside))
+    vt = torch.tensor(v, device="cuda")
+    outs = {}
+    for mf in ("1", "0"):
+        monkeypatch.setenv("JHTX_F64_ADJ_MFMA", mf)
+        H.clear_plans()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            outs[mf] = j.adjoint_synthesis(vt, nside, lmax, 2).cpu().numpy()
+    H.clear_plans()
+    e = _rel(outs["1"], outs["0"])
+    print(f"f64 adjoint mfma vs valu nside={nside} lmax={lmax}: {e:.2e}", flush=True)
+    assert e <= TOL
+    if nside <= 64:
+        assert _rel(outs["1"], O.adjoint_synthesis(v, nside, lmax, 2)) <= TOL
