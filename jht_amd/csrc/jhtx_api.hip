@@ -335,7 +335,7 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
     p->adj_R = pick("JHTX_ADJ_R", 4, 1, 1024, s0_R);
   } else {
     p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
-    p->adj_R = pick("JHTX_ADJ_R", 2, 1, 512, s2_R);
+    p->adj_R = pick("JHTX_ADJ_R", p->T >= 1024 ? 4 : 2, 1, 512, s2_R);
   }
   // mixed spin-2 kernels: maps per recursion (4 or 8) and the register budget of the
   // 8-map variants (1 or 2 work-groups per CU); defaults from sweeps at nside 2048 / lmax 4096

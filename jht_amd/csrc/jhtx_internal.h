@@ -18,6 +18,18 @@ namespace jhtx {
 constexpr double kBig = 0x1p256;
 constexpr double kRescale = 0x1p-512;
 constexpr int kScaleStep = 512;
+
+// One rescale check of a recursion pair: once the value has climbed past 2^256 while its
+// exponent is still negative, both terms are scaled by 2^-512 and the exponent advances.
+// Branch-free (lanes that stay put are multiplied by exactly 1.0), so divergent lanes cost
+// no exec-mask juggling inside the unrolled recursion.
+__device__ __forceinline__ void rescale_pair(double& mu, double& mup, int& ex) {
+  const bool up = (ex < 0) & (__builtin_fabs(mu) > kBig);
+  const double f = up ? kRescale : 1.0;
+  mu *= f;
+  mup *= f;
+  ex += up ? 1 : 0;
+}
 // consecutive orders m per XCD group in the recursion kernels' work-group order
 constexpr int kMGroup = 16;
 

@@ -70,14 +70,6 @@ __device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chun
   return m < A.M;
 }
 
-__device__ __forceinline__ void rescale1(double& mu, double& mup, int& ex) {
-  if (ex < 0 && fabs(mu) > kBig) {
-    mu *= kRescale;
-    mup *= kRescale;
-    ex += 1;
-  }
-}
-
 // ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
 __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
 
@@ -237,8 +229,8 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
         pmp = pm;
         pm = nm;
       }
-      rescale1(pp, ppp, ep);
-      rescale1(pm, pmp, em);
+      rescale_pair(pp, ppp, ep);
+      rescale_pair(pm, pmp, em);
       j0 += 8;
     }
     for (; j0 < nst; j0 += 8) {
@@ -255,8 +247,8 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
       // masks are exact 1/0 factors (1 on every lane outside the deep region)
       s2_steps8<MASKED, NB>(ag, bq, x, fp, fm, pp, ppp, pm, pmp, accP, accM);
       if (anydeep) {
-        rescale1(pp, ppp, ep);
-        rescale1(pm, pmp, em);
+        rescale_pair(pp, ppp, ep);
+        rescale_pair(pm, pmp, em);
       }
     }
   }
@@ -397,7 +389,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
         mup = mu;
         mu = n;
       }
-      rescale1(mu, mup, ex);
+      rescale_pair(mu, mup, ex);
       j0 += 8;
     }
     for (; j0 < nst; j0 += 8) {
@@ -429,7 +421,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
         }
       }
       wave_sync();
-      if (anydeep) rescale1(mu, mup, ex);
+      if (anydeep) rescale_pair(mu, mup, ex);
     }
   }
   // epilogue, one column block (4 maps) at a time: D[i][j] sits in lane j + 16 (i / 4),
@@ -551,7 +543,7 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
         mup = mu;
         mu = mn;
       }
-      rescale1(mu, mup, ex);
+      rescale_pair(mu, mup, ex);
       j0 += 8;
     }
     for (; j0 < nst; j0 += 8) {
@@ -576,7 +568,7 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
         for (int q = 0; q < NQ; ++q)
           acc[jj & 1][q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bq[jj][q >> 2][q & 3], acc[jj & 1][q], 0, 0, 0);
       }
-      if (anydeep) rescale1(mu, mup, ex);
+      if (anydeep) rescale_pair(mu, mup, ex);
     }
   }
   const long long ringM = A.M;
@@ -779,8 +771,8 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
           mytile[(1 * TL + j0 + jj) * TS + lane] = (float)(nm * ((jj & 1) ? -fm : fm));
         }
         if (anydeep) {
-          rescale1(pp, ppp, ep);
-          rescale1(pm, pmp, em);
+          rescale_pair(pp, ppp, ep);
+          rescale_pair(pm, pmp, em);
         }
       }
       wave_sync();
@@ -889,7 +881,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __re
           mu = mn;
           mytile[(j0 + jj) * TS + lane] = (float)(mn * f);
         }
-        if (anydeep) rescale1(mu, mup, ex);
+        if (anydeep) rescale_pair(mu, mup, ex);
       }
       wave_sync();
       floatx4 D[2];

@@ -105,13 +105,7 @@ __device__ __forceinline__ void s0_synth_block(const double* s_al, const double2
 template <int R>
 __device__ __forceinline__ void rescale(double (&mu)[R], double (&mup)[R], int (&ex)[R]) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (ex[r] < 0 && fabs(mu[r]) > kBig) {
-      mu[r] *= kRescale;
-      mup[r] *= kRescale;
-      ex[r] += 1;
-    }
-  }
+  for (int r = 0; r < R; ++r) rescale_pair(mu[r], mup[r], ex[r]);
 }
 
 template <int R>
@@ -367,28 +361,64 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
 // ---------------------------------------------------------------------------------- //
 constexpr int NV = 16;  // double2 partials per lane per block
 
-// Block reduction over the work-group's lanes.  LDS red[(wave*NV + v)*64 + lane] is written
-// lane-contiguously; the barrier that publishes it also ORs the next block's activity flag.
-// Stage 1: thread (v = tid/16, s = tid%16) sums lanes s, s+16, ... (consecutive threads read
-// consecutive lanes: conflict-free).  Stage 2: xor-shuffles across the 16 lanes of a value.
-// A second barrier retires the reads before the next block overwrites red.
+// Moves a double between lanes with a DPP row control (two 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 16 lanes of a DPP row, every lane receiving the total: row rotations by 8,
+// 4, 2 and 1 lanes (each step doubles the set of lanes a value covers).
+__device__ __forceinline__ double row_sum16(double v) {
+  v += dpp_f64<0x128>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x122>(v);
+  v += dpp_f64<0x121>(v);
+  return v;
+}
+
+// Block reduction over the work-group's lanes (nthr a multiple of 64, at most 256) of the
+// NV double2 values P[v] every lane holds, into out0/out1 (value v = jl * ncomp + comp).
+// LDS red[v * 256 + tid] is written lane-contiguously; the barrier that publishes it also
+// ORs the next block's activity flag.  Thread (v = tid / 16, s = tid % 16) then loads lanes
+// s, s + 16, ... of value v in one batch of independent reads (consecutive threads read
+// consecutive lanes), sums them as a tree, and the 16 partials of a value are summed
+// across the DPP row.  A second barrier retires the reads before the next block
+// overwrites red.  The floating-point order is fixed: results do not depend on the batch
+// or the launch.  (Halving the LDS traffic first with gfx950 lane swaps measured no faster
+// at nside 1024-2048: the reduction is not LDS-bound.)
+constexpr int kRedSize = NV * 256;  // double2 elements of red
+
+// Zeroes the slots of absent lanes (work-groups under 256 threads) so the reduction reads
+// every slot unconditionally.  Published by the first barrier of wg_reduce_store.
+__device__ __forceinline__ void red_zero_tail(double2* red, int tid, int nthr) {
+  for (int L = nthr + tid; L < 256; L += nthr) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) red[v * 256 + L] = make_double2(0.0, 0.0);
+  }
+}
+
 __device__ __forceinline__ int wg_reduce_store(double2 (&P)[NV], double2* red, double2* out0,
                                                double2* out1, int ncomp, int nl, int act_next,
                                                int tid, int nthr) {
-  const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) red[(wave * NV + v) * 64 + lane] = P[v];
+  for (int v = 0; v < NV; ++v) red[v * 256 + tid] = P[v];
   const int flag = __syncthreads_or(act_next);
   const int sidx = tid & 15;
   for (int v = tid >> 4; v < NV; v += nthr >> 4) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int L = sidx; L < nthr; L += 16) acc = acc + red[((L >> 6) * NV + v) * 64 + (L & 63)];
+    double2 q[16];
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-      acc.x += __shfl_xor(acc.x, off, 16);
-      acc.y += __shfl_xor(acc.y, off, 16);
+    for (int k = 0; k < 16; ++k) q[k] = red[v * 256 + sidx + 16 * k];
+#pragma unroll
+    for (int w = 8; w > 0; w >>= 1) {
+#pragma unroll
+      for (int k = 0; k < w; ++k) q[k] = q[k] + q[k + w];
     }
-    const int jl = v / ncomp, comp = v - jl * ncomp;  // value v = jl * ncomp + comp
+    const double2 acc = make_double2(row_sum16(q[0].x), row_sum16(q[0].y));
+    const int jl = v / ncomp, comp = v - jl * ncomp;
     if (sidx == 0 && jl < nl) (comp == 0 ? out0 : out1)[jl] = acc;
   }
   __syncthreads();
@@ -425,7 +455,7 @@ template <int R>
 __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __restrict__ V,
                                                 double2* __restrict__ part) {
   __shared__ double s_al[CH];
-  __shared__ double2 red[4 * NV * 64];
+  __shared__ double2 red[kRedSize];
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
@@ -468,6 +498,7 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
     for (int r = 0; r < R; ++r) a |= (ex[r] == 0) && (tb + r < A.T);
     return (int)a;
   };
+  red_zero_tail(red, tid, nthr);
   int wg_active = __syncthreads_or(active());
   for (int l0 = m; l0 <= lmax; l0 += CH) {
     __syncthreads();
@@ -553,7 +584,7 @@ template <int R>
 __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __restrict__ V,
                                                 double2* __restrict__ part) {
   __shared__ double2 s_ag[CH];
-  __shared__ double2 red[4 * NV * 64];
+  __shared__ double2 red[kRedSize];
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
@@ -606,6 +637,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
     for (int r = 0; r < R; ++r) a |= ((ep[r] == 0) || (em[r] == 0)) && (tb + r < A.T);
     return (int)a;
   };
+  red_zero_tail(red, tid, nthr);
   int wg_active = __syncthreads_or(active());
   for (int l0 = lmin; l0 <= lmax; l0 += CH) {
     __syncthreads();
@@ -667,14 +699,6 @@ __device__ __forceinline__ void wave_sync64() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void rescale1(double& mu, double& mup, int& ex) {
-  if (ex < 0 && fabs(mu) > kBig) {
-    mu *= kRescale;
-    mup *= kRescale;
-    ex += 1;
-  }
 }
 
 __global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* __restrict__ V,
@@ -781,8 +805,8 @@ __global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* _
           mytile[(1 * 8 + jj) * kF64TS + lane] = nm * fm;
         }
         if (anydeep) {
-          rescale1(pp, ppp, ep);
-          rescale1(pm, pmp, em);
+          rescale_pair(pp, ppp, ep);
+          rescale_pair(pm, pmp, em);
         }
         wave_sync64();
 #pragma unroll
