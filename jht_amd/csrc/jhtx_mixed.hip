@@ -635,9 +635,11 @@ __device__ __forceinline__ void adj_tile_mfma(const float* tile, int lane, const
 // and opposite for the two signs.  The odd rows of the sign - tile are stored negated, which
 // makes the odd-parity accumulator equal sigma_+(column) times the true one; the sign is
 // applied when the tile result is reduced.
+// act: wave-uniform bit sg set when the sign-sg tile holds a nonzero value (an all-zero
+// sign tile contributes nothing and is skipped).
 template <int NCB>
 __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, const float (&Bv)[NCB][2][16],
-                                                 floatx4 (&D)[NCB][2]) {
+                                                 floatx4 (&D)[NCB][2], int act) {
   const int col = lane >> 4;
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
@@ -646,6 +648,7 @@ __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, co
     const int row = 2 * (lane & 15) + p;
 #pragma unroll
     for (int sg = 0; sg < 2; ++sg) {
+      if (!((act >> sg) & 1)) continue;
       const float* ts = tile + sg * TL * TS + row * TS + col;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -747,11 +750,14 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
     if (l0 + CH <= lmax) prefetch(l0 + CH);
 #pragma unroll 1
     for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
-      // recursion for TL steps, values to the wave's tile
+      // recursion for TL steps, values to the wave's tile; act collects which sign tiles
+      // hold a value above 2^-256 (wave-uniform)
+      int act = 0;
 #pragma unroll 1
       for (int j0 = 0; j0 < TL; j0 += 8) {
         const bool deep = valid && ((ep < 0) || (em < 0));
         const int anydeep = __ballot(deep) != 0ull;
+        act |= (__ballot(valid && ep == 0) != 0ull ? 1 : 0) | (__ballot(valid && em == 0) != 0ull ? 2 : 0);
         const double fp = (ep == 0 && valid) ? 1.0 : 0.0, fm = (em == 0 && valid) ? 1.0 : 0.0;
         double2 ag[8];
 #pragma unroll
@@ -777,9 +783,21 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
       }
       wave_sync();
       floatx4 D[NCB][2];
-      adj_tile_mfma_s2<NCB>(mytile, lane, Bv, D);
-      // publish per-wave sums (FP64) in the tile area after every wave has read its tile
-      __syncthreads();
+      adj_tile_mfma_s2<NCB>(mytile, lane, Bv, D, act);
+      // publish per-wave sums (FP64) in the tile area after every wave has read its tile;
+      // a work-group whose tiles are all zero writes zero partials without the reduction
+      if (!__syncthreads_or(act)) {
+        for (int r = tid; r < 2 * 16 * MG; r += RINGS) {
+          const int p = r / (16 * MG), i = (r / MG) % 16, g = r % MG;
+          const int l = l0 + jb + 2 * i + p;
+          const int b = mg * MG + g;
+          if (l <= lmax && b < batch) {
+            float2* pb = p_p2 + (long long)b * 2 * A.K;
+            pb[cb + l] = pb[A.K + cb + l] = make_float2(0.f, 0.f);
+          }
+        }
+        continue;  // no LDS was written after the barrier: the next tile may start at once
+      }
       double* myred = red + wave * 2 * 16 * NC;
 #pragma unroll
       for (int c = 0; c < NCB; ++c)
