@@ -129,6 +129,46 @@ def run(plan, alm, maps, back, B, steps, warmup):
     return dt, per
 
 
+def bench_offgrid(args):
+    """Off-grid pair (synthesis_general + adjoint_synthesis_general, FP64, epsilon 1e-10) on
+    random points: points/s of the pair and the per-stage split (HIP events)."""
+    import torch
+
+    import jht_amd
+    from jht_amd.offgrid import get_offgrid_plan
+
+    lmax, spin, npts = args.lmax, args.spin, args.offgrid
+    rng = np.random.default_rng(20261204)
+    if spin == 0:
+        a = gaussian_alm(lmax, rng)
+    else:
+        a = np.stack([gaussian_alm(lmax, rng, 2), gaussian_alm(lmax, rng, 2, 0.1)])
+    alm = torch.tensor(a, device="cuda")
+    loc = torch.tensor(np.stack([rng.uniform(0, np.pi, npts), rng.uniform(0, 2 * np.pi, npts)], axis=1), device="cuda")
+
+    def step():
+        f = jht_amd.synthesis_general(alm, loc, spin=spin, lmax=lmax)
+        return jht_amd.adjoint_synthesis_general(f, loc, spin=spin, lmax=lmax)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    plan = get_offgrid_plan(lmax, spin, 1e-10)
+    print(json.dumps({
+        "metric": f"off-grid synth+adjoint points/s lmax={lmax} (spin {spin}), epsilon 1e-10",
+        "value": round(npts / dt, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded Gaussian a_lm, uniform random points)",
+        "config": {"workload": "off-grid synthesis+adjoint", "lmax": lmax, "spin": spin, "npts": npts,
+                   "rings": plan.ns, "grid": [plan.nk, plan.nm], "W": plan.W},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +182,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=12)
+    ap.add_argument("--offgrid", type=int, default=0, metavar="NPTS",
+                    help="instead of the headline transform pair, time the off-grid pair "
+                         "synthesis_general + adjoint_synthesis_general at --lmax / --spin on "
+                         "NPTS uniformly random points and print its own JSON line")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -164,6 +208,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.offgrid > 0:
+        return bench_offgrid(args)
 
     nside, lmax, spin, B = args.nside, args.lmax, args.spin, args.batch
     prec = _lib.PREC_MIXED if args.precision == "mixed" else _lib.PREC_F64

@@ -118,8 +118,78 @@ int jhtx_host_rec_tables(int nside, int lmax, int spin, double* x, double* lsin,
                          double* alpha, double* gamma, double* G, double* seed);
 
 /* Plan introspection: 0 nside, 1 lmax, 2 spin, 3 precision, 4 max_batch, 5 ncomp,
- * 6 nrings, 7 npix (low 31 bits), 8 number of ring-FFT launch classes. -1 if unknown. */
+ * 6 nrings, 7 npix (low 31 bits), 8 number of ring-FFT launch classes, 9 1 for a ring-set
+ * plan. -1 if unknown. */
 int jhtx_plan_query(const jhtx_plan* plan, int what);
+
+/* ---------------------------------------------------------------------------------- *
+ * Off-grid synthesis (jht.synthesis_general / adjoint_synthesis_general, FP64, one field).
+ * The transform is Legendre step on a Clenshaw-Curtis ring set -> double-Fourier-sphere
+ * extension -> theta FFT -> deconvolved oversampled grid -> 2-D inverse FFT -> ES-kernel
+ * interpolation at the points (src/jht/offgrid.py:124-205, src/jht/_nufft.py:154-170),
+ * and its transpose.  The two plain FFTs are the caller's (library FFTs); these entry
+ * points are every other stage.
+ * ---------------------------------------------------------------------------------- */
+
+/* Replaces build_recursion_plan(x_cc, +-spin, lmax) as called by the off-grid _prepare
+ * (src/jht/offgrid.py:72-78): an FP64 plan over an arbitrary ring set symmetric about the
+ * equator.  theta: the (nrings + 1) / 2 north-half colatitudes, nondecreasing in [0, pi/2],
+ * the last the equator; rings on the pole are allowed.  Workspace: the adjoint's
+ * partials only.  jhtx_synth / jhtx_adjoint refuse such a plan (no HEALPix geometry). */
+int jhtx_plan_create_rings(int nrings, const double* theta, int lmax, int spin, int max_batch,
+                           jhtx_plan** plan);
+
+/* Replaces synth_contract (src/jht/_recursion.py:372-398) as used by the off-grid
+ * _alm_to_Fkm (offgrid.py:147-151): F[b][c][ring][m] = sum_l a^c_lm lambda^{s_c}_lm(ring),
+ * complex128, with c the +spin / -spin channel (spin 2: a^+ = -(E + iB), a^- = -(E - iB);
+ * spin 0: one channel).  Works on any plan (HEALPix plans give the coefficients before the
+ * ring FFT); no workspace. */
+int jhtx_legendre_synth(jhtx_plan* plan, const void* alm, void* F, int batch, void* stream);
+
+/* Replaces adjoint_contract (src/jht/_recursion.py:401-423) as used by the off-grid
+ * _Fkm_to_alm (offgrid.py:154-160): the transpose of jhtx_legendre_synth, p = sum_ring
+ * lambda^{+s} F^+, n = sum_ring lambda^{-s} F^-, returned as the HEALPix adjoint returns
+ * them: spin 0 p; spin 2 (E, B) = (-(p + n) / 2, i (p - n) / 2). */
+int jhtx_legendre_adjoint(jhtx_plan* plan, const void* F, void* alm, int batch, void* workspace,
+                          void* stream);
+
+/* Replaces the DFS construction of _coeffs_to_Fkm (src/jht/offgrid.py:124-134, without
+ * its FFT): from F[c][ring][m] (nrings Clenshaw-Curtis rings, both poles) to
+ * D[j][m' + lmax], j < nd = 2 nrings - 2, m' = -lmax..lmax: D[j] = Cfull[j] for j < nrings,
+ * (-1)^(|m'| + spin) Cfull[nd - j] beyond, Cfull[m' >= 0] = F^+[m'], Cfull[m' < 0] =
+ * conj(F^-[-m']). */
+int jhtx_dfs_extend(const void* F, int nrings, int lmax, int spin, void* D, void* stream);
+
+/* Replaces the transpose _Fkm_to_coeffs (src/jht/offgrid.py:138-153, without its inverse
+ * FFT): D[nd][2 lmax + 1] -> F[c][ring][m]; for spin 0 the two channels are summed. */
+int jhtx_dfs_fold(const void* D, int nrings, int lmax, int spin, void* F, void* stream);
+
+/* Replaces nufft2d2's mode placement (src/jht/_nufft.py:157-164): the Nk x (2 lmax + 1)
+ * centred modes X[k mod Nk][m' + lmax] (an unshifted FFT along k) onto the zeroed
+ * oversampled nk x nm grid, times sk[k + Nk/2] sm[m' + lmax] (device arrays: the
+ * deconvolution and normalisation factors). */
+int jhtx_nufft_grid_fill(const void* X, int Nk, int lmax, const double* sk, const double* sm, void* C,
+                         int nk, int nm, void* stream);
+
+/* Replaces nufft2d1's mode extraction (src/jht/_nufft.py:179-181): the transpose of
+ * jhtx_nufft_grid_fill. */
+int jhtx_nufft_grid_gather(const void* C, int Nk, int lmax, const double* sk, const double* sm, void* X,
+                           int nk, int nm, void* stream);
+
+/* Replaces _stencil + the stencil sum of nufft2d2 (src/jht/_nufft.py:140-170): f_j =
+ * sum of the W x W ES-weighted grid values around loc_j = (theta_j, phi_j) (grid
+ * complex128 nk x nm, the 2 pi torus).  re / im receive Re f, Im f (im may be NULL).
+ * dloc, when not NULL, receives [4][npts]: Re, Im of df/dtheta, then of df/dphi (the
+ * pointing derivative).  W in {6, 8, 10, 14, 16}.  npts = 0 is a no-op (point buffers may
+ * then be NULL). */
+int jhtx_nufft_interp(const void* grid, int nk, int nm, int W, double beta, const double* loc,
+                      long long npts, double* re, double* im, double* dloc, void* stream);
+
+/* Replaces the spreading of nufft2d1 (src/jht/_nufft.py:173-178): adds (re_j + i im_j) times
+ * the ES weights around loc_j into grid (which the caller zeroes); im may be NULL; npts = 0
+ * is a no-op. */
+int jhtx_nufft_spread(const double* re, const double* im, const double* loc, long long npts, void* grid,
+                      int nk, int nm, int W, double beta, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,5 @@
-"""jht_amd -- MI355X-native HEALPix on-grid synthesis / adjoint (drop-in for jht's pair).
+"""jht_amd -- MI355X-native HEALPix on-grid synthesis / adjoint (drop-in for jht's pair),
+plus the off-grid pair synthesis_general / adjoint_synthesis_general.
 
 The operators are hand-written HIP kernels for gfx950 in ``libjhtx.so`` (C ABI in
 ``include/jhtx.h``) driven from torch tensors.  Importing this package does not touch
@@ -22,6 +23,7 @@ from .healpix import (  # noqa: F401
     synthesis_batch,
 )
 from .graph import Instance, TransformGraph  # noqa: F401
+from .offgrid import adjoint_synthesis_general, synthesis_general  # noqa: F401
 from .tune import autotune  # noqa: F401
 
 from .callers import (  # noqa: F401

@@ -25,6 +25,15 @@ EXPORTS = (
     "jhtx_plan_set_profiling",
     "jhtx_plan_read_profile",
     "jhtx_host_rec_tables",
+    "jhtx_plan_create_rings",
+    "jhtx_legendre_synth",
+    "jhtx_legendre_adjoint",
+    "jhtx_dfs_extend",
+    "jhtx_dfs_fold",
+    "jhtx_nufft_grid_fill",
+    "jhtx_nufft_grid_gather",
+    "jhtx_nufft_interp",
+    "jhtx_nufft_spread",
 )
 
 JHTX_OK = 0
@@ -96,6 +105,22 @@ def load() -> ctypes.CDLL:
     dp = ctypes.POINTER(ctypes.c_double)
     lib.jhtx_host_rec_tables.argtypes = [ctypes.c_int] * 3 + [dp] * 7
     lib.jhtx_host_rec_tables.restype = ctypes.c_int
+    i, ll, d = ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+    sigs = {
+        "jhtx_plan_create_rings": [i, vp, i, i, i, ctypes.POINTER(vp)],
+        "jhtx_legendre_synth": [vp, vp, vp, i, vp],
+        "jhtx_legendre_adjoint": [vp, vp, vp, i, vp, vp],
+        "jhtx_dfs_extend": [vp, i, i, i, vp, vp],
+        "jhtx_dfs_fold": [vp, i, i, i, vp, vp],
+        "jhtx_nufft_grid_fill": [vp, i, i, vp, vp, vp, i, i, vp],
+        "jhtx_nufft_grid_gather": [vp, i, i, vp, vp, vp, i, i, vp],
+        "jhtx_nufft_interp": [vp, i, i, i, d, vp, ll, vp, vp, vp, vp],
+        "jhtx_nufft_spread": [vp, vp, vp, ll, vp, i, i, i, d, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
     _LIB = lib
     return lib
 

@@ -69,8 +69,13 @@ struct HostRec {
   std::vector<double> x, lsin, lcos, alpha, gamma, G, seed;
 };
 
-void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
-  const int M = lmax + 1, T = 2 * nside;
+// log of a non-negative factor; an exact zero (a ring on a pole) maps to -1e300, which
+// the seed split reads as an exact zero seed, while a zero power of it still gives 0.
+double pole_log(double v) { return v > 0.0 ? std::log(v) : -1.0e300; }
+
+// Tables over the north-half rings x[t] = cos(theta_t), t < T (the last one the equator).
+void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRec& h) {
+  const int M = lmax + 1, T = (int)xs.size();
   const long long K = (long long)(lmax + 1) * (lmax + 2) / 2;
   std::vector<double>& x = h.x;
   std::vector<double>& lsin = h.lsin;
@@ -79,14 +84,13 @@ void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
   std::vector<double>& gamma = h.gamma;
   std::vector<double>& G = h.G;
   std::vector<double>& seed = h.seed;
-  x.assign(T, 0.0);
+  x = xs;
   lsin.assign(T, 0.0);
   for (int t = 0; t < T; ++t) {
-    const double z = ring_z(nside, t);
-    x[t] = z;
+    const double z = x[t];
     if (spin == 0) {
       const double s2 = std::min(std::max(1.0 - z * z, 0.0), 1.0);
-      lsin[t] = std::log(std::sqrt(s2));
+      lsin[t] = pole_log(std::sqrt(s2));
     }
   }
   if (spin == 2) {
@@ -95,8 +99,8 @@ void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
       const double z = x[t];
       const double ch = std::sqrt(std::min(std::max((1.0 + z) / 2.0, 0.0), 1.0));
       const double sh = std::sqrt(std::min(std::max((1.0 - z) / 2.0, 0.0), 1.0));
-      lcos[t] = std::log(ch);
-      lsin[t] = std::log(sh);
+      lcos[t] = pole_log(ch);
+      lsin[t] = pole_log(sh);
     }
   }
   alpha.assign(K, 0.0);
@@ -170,9 +174,14 @@ void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
   }
 }
 
-int build_rec_tables(jhtx_plan* p) {
-  HostRec h;
-  host_rec_tables(p->nside, p->lmax, p->spin, h);
+// The HEALPix north-half rings t < 2 nside.
+void host_rec_tables(int nside, int lmax, int spin, HostRec& h) {
+  std::vector<double> xs(2 * nside);
+  for (int t = 0; t < 2 * nside; ++t) xs[t] = ring_z(nside, t);
+  host_rec_tables_x(xs, lmax, spin, h);
+}
+
+int upload_rec_tables(jhtx_plan* p, const HostRec& h) {
   int rc;
   if ((rc = upload(&p->rec.alpha, h.alpha))) return rc;
   if ((rc = upload(&p->rec.G, h.G))) return rc;
@@ -182,6 +191,12 @@ int build_rec_tables(jhtx_plan* p) {
   if ((rc = upload(&p->rec.lcos, h.lcos))) return rc;
   if ((rc = upload(&p->rec.seed, h.seed))) return rc;
   return JHTX_OK;
+}
+
+int build_rec_tables(jhtx_plan* p) {
+  HostRec h;
+  host_rec_tables(p->nside, p->lmax, p->spin, h);
+  return upload_rec_tables(p, h);
 }
 
 void free_rec_tables(jhtx_plan* p) {
@@ -251,6 +266,116 @@ int check_call(const jhtx_plan* p, const void* in, void* out, int batch, void* w
   return JHTX_OK;
 }
 
+// Kernel shapes (defaults, environment overrides, explicit tuning) and workspace sizes of
+// a plan whose geometry fields (M, T, nrings, ncomp, K) are set.
+int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
+  // rings per lane and work-group size of the recursion kernels; small grids use one
+  // ring per lane.  JHTX_SYNTH_R / JHTX_ADJ_R override the choice (tuning sweeps).
+  auto pick = [&](const char* env, int big, int small, int thresh, const int* allowed) {
+    int r = (p->T >= thresh) ? big : small;
+    if (const char* v = getenv(env)) {
+      const int want = atoi(v);
+      for (int i = 0; allowed[i]; ++i)
+        if (allowed[i] == want) r = want;
+    }
+    return r;
+  };
+  static const int s0_R[] = {1, 4, 8, 0};
+  static const int s2_R[] = {1, 2, 4, 0};
+  // defaults from sweeps at nside 2048 / lmax 4096 (synthesis: occupancy wins; adjoint:
+  // rings per lane amortise the per-lane ring reduction)
+  if (p->spin == 0) {
+    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 512, s0_R);
+    p->adj_R = pick("JHTX_ADJ_R", 4, 1, 1024, s0_R);
+  } else {
+    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
+    p->adj_R = pick("JHTX_ADJ_R", p->T >= 1024 ? 4 : 2, 1, 512, s2_R);
+  }
+  // mixed spin-2 kernels: maps per recursion (4 or 8) and the register budget of the
+  // 8-map variants (1 or 2 work-groups per CU); defaults from sweeps at nside 2048 / lmax 4096
+  auto pick2 = [](const char* env, int dflt, int a, int b) {
+    int r = dflt;
+    if (const char* v = getenv(env)) {
+      const int want = atoi(v);
+      if (want == a || want == b) r = want;
+    }
+    return r;
+  };
+  // maps per recursion follow the batch the plan is sized for (a partly empty group costs
+  // matrix-core time, a larger one amortises the FP64 recursion over more maps)
+  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4), 4, 8);
+  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
+    if (atoi(v) == 16) p->mx_synth_group = 16;  // the 16x16x4 tile kernel
+  }
+  p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
+  if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
+    if (atoi(v) == 3) p->mx_synth_occ = 3;
+  }
+  p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4), 4, 8);
+  if (const char* v = getenv("JHTX_MX_ADJ_GROUP")) {
+    if (atoi(v) == 16) p->mx_adj_group = 16;
+  }
+  p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
+  p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
+  p->f64_adj_mfma = (p->spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
+  int block_cap = 256;
+  if (const char* v = getenv("JHTX_REC_BLOCK")) {
+    const int b = atoi(v);
+    if (b == 64 || b == 128 || b == 192 || b == 256) block_cap = b;
+  }
+  if (tuning) {
+    // explicit shapes override the defaults and the environment
+    auto in = [](int v, std::initializer_list<int> ok) {
+      for (int o : ok)
+        if (v == o) return true;
+      return false;
+    };
+    const jhtx_tuning& t = *tuning;
+    const bool ok =
+        (t.synth_rings == 0 || (p->spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
+        (t.adj_rings == 0 || (p->spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
+        (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
+        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16})) &&
+        (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
+        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
+        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
+        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2}));
+    if (!ok) {
+      set_error("jhtx_tuning field out of range");
+      return JHTX_EINVAL;
+    }
+    if (t.synth_rings) p->rec_R = t.synth_rings;
+    if (t.adj_rings) p->adj_R = t.adj_rings;
+    if (t.block) block_cap = t.block;
+    if (t.mx_synth_group) p->mx_synth_group = t.mx_synth_group;
+    if (t.mx_synth_occ) p->mx_synth_occ = t.mx_synth_occ;
+    if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
+    if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
+    if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
+    if (t.f64_adj_mfma && p->spin == 2) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
+  }
+  p->block_cap = block_cap;
+  auto shape = [&](int R, int& block, int& nchunk) {
+    const int lanes = (p->T + R - 1) / R;
+    block = std::min(block_cap, std::max(64, ((lanes + 63) / 64) * 64));
+    nchunk = (p->T + block * R - 1) / (block * R);
+  };
+  shape(p->rec_R, p->rec_block, p->nchunk);
+  shape(p->adj_R, p->adj_block, p->adj_nchunk);
+  // ring coefficients F / V: complex128, or complex64 for the mixed path (FP32 ring FFT)
+  const size_t f_elem = (p->precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
+  p->ws_F_bytes = align256(f_elem * (size_t)p->max_batch * p->ncomp * p->nrings * p->M);
+  // the mixed kernels use one ring per lane and 256-lane work-groups
+  // the FP64 matrix-core adjoint writes one partial per 256 rings
+  const int part_chunks = (p->precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p)
+                          : p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256)
+                                            : p->adj_nchunk;
+  const size_t part_elem = (p->precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
+  p->ws_part_bytes = align256(part_elem * (size_t)part_chunks * p->max_batch * p->ncomp * p->K);
+  p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
+  return JHTX_OK;
+}
+
 }  // namespace
 }  // namespace jhtx
 
@@ -315,112 +440,12 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   p->ncomp = spin ? 2 : 1;
   p->npix = 12LL * nside * nside;
   p->K = (long long)(lmax + 1) * (lmax + 2) / 2;
-  // rings per lane and work-group size of the recursion kernels; small grids use one
-  // ring per lane.  JHTX_SYNTH_R / JHTX_ADJ_R override the choice (tuning sweeps).
-  auto pick = [&](const char* env, int big, int small, int thresh, const int* allowed) {
-    int r = (p->T >= thresh) ? big : small;
-    if (const char* v = getenv(env)) {
-      const int want = atoi(v);
-      for (int i = 0; allowed[i]; ++i)
-        if (allowed[i] == want) r = want;
-    }
-    return r;
-  };
-  static const int s0_R[] = {1, 4, 8, 0};
-  static const int s2_R[] = {1, 2, 4, 0};
-  // defaults from sweeps at nside 2048 / lmax 4096 (synthesis: occupancy wins; adjoint:
-  // rings per lane amortise the per-lane ring reduction)
-  if (spin == 0) {
-    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 512, s0_R);
-    p->adj_R = pick("JHTX_ADJ_R", 4, 1, 1024, s0_R);
-  } else {
-    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
-    p->adj_R = pick("JHTX_ADJ_R", p->T >= 1024 ? 4 : 2, 1, 512, s2_R);
+  int rc = configure_plan(p, tuning);
+  if (rc != JHTX_OK) {
+    delete p;
+    return rc;
   }
-  // mixed spin-2 kernels: maps per recursion (4 or 8) and the register budget of the
-  // 8-map variants (1 or 2 work-groups per CU); defaults from sweeps at nside 2048 / lmax 4096
-  auto pick2 = [](const char* env, int dflt, int a, int b) {
-    int r = dflt;
-    if (const char* v = getenv(env)) {
-      const int want = atoi(v);
-      if (want == a || want == b) r = want;
-    }
-    return r;
-  };
-  // maps per recursion follow the batch the plan is sized for (a partly empty group costs
-  // matrix-core time, a larger one amortises the FP64 recursion over more maps)
-  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", max_batch >= 16 ? 16 : (max_batch >= 5 ? 8 : 4), 4, 8);
-  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
-    if (atoi(v) == 16) p->mx_synth_group = 16;  // the 16x16x4 tile kernel
-  }
-  p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
-  if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
-    if (atoi(v) == 3) p->mx_synth_occ = 3;
-  }
-  p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", max_batch >= 16 ? 16 : (max_batch >= 5 ? 8 : 4), 4, 8);
-  if (const char* v = getenv("JHTX_MX_ADJ_GROUP")) {
-    if (atoi(v) == 16) p->mx_adj_group = 16;
-  }
-  p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
-  p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
-  p->f64_adj_mfma = (spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
-  int block_cap = 256;
-  if (const char* v = getenv("JHTX_REC_BLOCK")) {
-    const int b = atoi(v);
-    if (b == 64 || b == 128 || b == 192 || b == 256) block_cap = b;
-  }
-  if (tuning) {
-    // explicit shapes override the defaults and the environment
-    auto in = [](int v, std::initializer_list<int> ok) {
-      for (int o : ok)
-        if (v == o) return true;
-      return false;
-    };
-    const jhtx_tuning& t = *tuning;
-    const bool ok =
-        (t.synth_rings == 0 || (spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
-        (t.adj_rings == 0 || (spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
-        (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
-        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16})) &&
-        (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
-        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
-        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
-        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2}));
-    if (!ok) {
-      delete p;
-      set_error("jhtx_tuning field out of range");
-      return JHTX_EINVAL;
-    }
-    if (t.synth_rings) p->rec_R = t.synth_rings;
-    if (t.adj_rings) p->adj_R = t.adj_rings;
-    if (t.block) block_cap = t.block;
-    if (t.mx_synth_group) p->mx_synth_group = t.mx_synth_group;
-    if (t.mx_synth_occ) p->mx_synth_occ = t.mx_synth_occ;
-    if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
-    if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
-    if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
-    if (t.f64_adj_mfma && spin == 2) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
-  }
-  p->block_cap = block_cap;
-  auto shape = [&](int R, int& block, int& nchunk) {
-    const int lanes = (p->T + R - 1) / R;
-    block = std::min(block_cap, std::max(64, ((lanes + 63) / 64) * 64));
-    nchunk = (p->T + block * R - 1) / (block * R);
-  };
-  shape(p->rec_R, p->rec_block, p->nchunk);
-  shape(p->adj_R, p->adj_block, p->adj_nchunk);
-  // ring coefficients F / V: complex128, or complex64 for the mixed path (FP32 ring FFT)
-  const size_t f_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
-  p->ws_F_bytes = align256(f_elem * (size_t)max_batch * p->ncomp * p->nrings * p->M);
-  // the mixed kernels use one ring per lane and 256-lane work-groups
-  // the FP64 matrix-core adjoint writes one partial per 256 rings
-  const int part_chunks = (precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p)
-                          : p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256)
-                                            : p->adj_nchunk;
-  const size_t part_elem = (precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
-  p->ws_part_bytes = align256(part_elem * (size_t)part_chunks * max_batch * p->ncomp * p->K);
-  p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
-  int rc = build_rec_tables(p);
+  rc = build_rec_tables(p);
   if (rc == JHTX_OK) rc = build_fft_tables(p);
   if (rc != JHTX_OK) {
     free_rec_tables(p);
@@ -432,11 +457,120 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
   return JHTX_OK;
 }
 
+int jhtx_plan_create_rings(int nrings, const double* theta, int lmax, int spin, int max_batch,
+                           jhtx_plan** out) {
+  g_err.clear();
+  if (!out) {
+    set_error("null plan output pointer");
+    return JHTX_EINVAL;
+  }
+  *out = nullptr;
+  if (spin != 0 && spin != 2) {
+    set_error("spin=" + std::to_string(spin) + " unsupported (only 0 and 2)");
+    return JHTX_EUNSUPPORTED;
+  }
+  if (!theta || nrings < 1 || nrings % 2 == 0 || nrings > 65535 || lmax < 0 || lmax > 32767 ||
+      max_batch < 1) {
+    set_error("invalid ring set (nrings must be odd, 1..65535) or lmax/max_batch");
+    return JHTX_EINVAL;
+  }
+  const int T = (nrings + 1) / 2;
+  const double half_pi = 1.57079632679489661923;
+  for (int t = 0; t < T; ++t) {
+    const bool in_range = theta[t] >= 0.0 && theta[t] <= half_pi + 1e-12;
+    const bool sorted = t == 0 || theta[t] >= theta[t - 1];
+    if (!in_range || !sorted) {
+      set_error("theta must be nondecreasing in [0, pi/2]");
+      return JHTX_EINVAL;
+    }
+  }
+  if (std::fabs(theta[T - 1] - half_pi) > 1e-12) {
+    set_error("the last north-half ring must be the equator (theta = pi/2)");
+    return JHTX_EINVAL;
+  }
+  jhtx_plan* p = new jhtx_plan();
+  p->ring_set = 1;
+  p->lmax = lmax;
+  p->spin = spin;
+  p->precision = JHTX_PREC_F64;
+  p->max_batch = max_batch;
+  p->M = lmax + 1;
+  p->T = T;
+  p->nrings = nrings;
+  p->ncomp = spin ? 2 : 1;
+  p->K = (long long)(lmax + 1) * (lmax + 2) / 2;
+  int rc = configure_plan(p, nullptr);
+  if (rc != JHTX_OK) {
+    delete p;
+    return rc;
+  }
+  // the caller owns the ring coefficients: the workspace holds the adjoint's partials only
+  p->ws_F_bytes = 0;
+  p->ws_bytes = p->ws_part_bytes;
+  std::vector<double> xs(T);
+  for (int t = 0; t < T; ++t) xs[t] = std::cos(theta[t]);
+  HostRec h;
+  host_rec_tables_x(xs, lmax, spin, h);
+  rc = upload_rec_tables(p, h);
+  if (rc != JHTX_OK) {
+    free_rec_tables(p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return JHTX_OK;
+}
+
+int jhtx_legendre_synth(jhtx_plan* p, const void* alm, void* F, int batch, void* stream) {
+  static char dummy;
+  int rc = check_call(p, alm, F, batch, &dummy);
+  if (rc) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StageTimer t0(p, s, 0);
+  hipError_t e = (p->precision == JHTX_PREC_MIXED)
+                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<float2*>(F), batch, s)
+                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<double2*>(F), batch, s);
+  t0.done();
+  if (e != hipSuccess) {
+    set_error(std::string("Legendre synthesis launch failed: ") + hipGetErrorString(e));
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+int jhtx_legendre_adjoint(jhtx_plan* p, const void* F, void* alm, int batch, void* ws, void* stream) {
+  int rc = check_call(p, F, alm, batch, ws);
+  if (rc) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  void* part = reinterpret_cast<char*>(ws) + p->ws_F_bytes;
+  StageTimer t3(p, s, 3);
+  hipError_t e;
+  if (p->precision == JHTX_PREC_MIXED) {
+    float2* part32 = reinterpret_cast<float2*>(part);
+    e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, batch, s);
+    if (e == hipSuccess)
+      e = launch_adj_finish(p, part32, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
+  } else {
+    e = launch_adj_rec(p, reinterpret_cast<const double2*>(F), reinterpret_cast<double2*>(part),
+                       reinterpret_cast<double2*>(alm), batch, s);
+  }
+  t3.done();
+  if (e != hipSuccess) {
+    set_error(std::string("Legendre adjoint launch failed: ") + hipGetErrorString(e));
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
 size_t jhtx_plan_workspace_bytes(const jhtx_plan* p) { return p ? p->ws_bytes : 0; }
 
 int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, void* stream) {
   int rc = check_call(p, alm, map, batch, ws);
   if (rc) return rc;
+  if (p->ring_set) {
+    set_error("plan has no HEALPix geometry (a ring-set plan): use jhtx_legendre_synth");
+    return JHTX_EINVAL;
+  }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   void* F = ws;
   StageTimer t0(p, s, 0);
@@ -459,6 +593,10 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
 int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, void* stream) {
   int rc = check_call(p, map, alm, batch, ws);
   if (rc) return rc;
+  if (p->ring_set) {
+    set_error("plan has no HEALPix geometry (a ring-set plan): use jhtx_legendre_adjoint");
+    return JHTX_EINVAL;
+  }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   double2* V = reinterpret_cast<double2*>(ws);
   double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(ws) + p->ws_F_bytes);
@@ -570,6 +708,7 @@ int jhtx_plan_query(const jhtx_plan* p, int what) {
     case 6: return p->nrings;
     case 7: return (int)(p->npix & 0x7fffffff);
     case 8: return p->fft.ncls;
+    case 9: return p->ring_set;
     default: return -1;
   }
 }

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
       Fp[(long long)t * ringM + m] = f2(-pe_r + pb_i, -pe_i - pb_r);
       Fm[(long long)t * ringM + m] = f2(-me_r - mb_i, -me_i + mb_r);
       if (t < A.T - 1) {
-        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
         const double de_r = M0[0] - M1[0], de_i = M0[1] - M1[1], db_r = M0[2] - M1[2], db_i = M0[3] - M1[3];
         const double qe_r = P0[0] - P1[0], qe_i = P0[1] - P1[1], qb_r = P0[2] - P1[2], qb_i = P0[3] - P1[3];
         Fp[rs] = f2(-de_r + db_i, -de_i - db_r);
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
       float2* Fm = Fp + (long long)A.nrings * ringM;
       Fp[(long long)t * ringM + m] = Fm[(long long)t * ringM + m] = f2(0.0, 0.0);
       if (t < A.T - 1) {
-        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
         Fp[rs] = Fm[rs] = f2(0.0, 0.0);
       }
     }
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
       if (tt < A.T && b < batch) {
         float2* Fp = F + (long long)b * 2 * A.nrings * ringM;
         float2* Fm = Fp + (long long)A.nrings * ringM;
-        const long long rs = (long long)(4 * A.nside - 2 - tt) * ringM + m;
+        const long long rs = (long long)(A.nrings - 1 - tt) * ringM + m;
         if (lmin <= lmax) {
           double P0[4], P1[4], M0[4], M1[4];
 #pragma unroll
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
         const double c1r = s_ep[wave][1][lane][2 * h], c1i = s_ep[wave][1][lane][2 * h + 1];
         float2* Fb = F + (long long)b * A.nrings * ringM;
         Fb[(long long)t * ringM + m] = f2(c0r + c1r, c0i + c1i);
-        if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * ringM + m] = f2(c0r - c1r, c0i - c1i);
+        if (t < A.T - 1) Fb[(long long)(A.nrings - 1 - t) * ringM + m] = f2(c0r - c1r, c0i - c1i);
       }
     }
     wave_sync();
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
         if (tr < A.T && b < batch) {
           const long long rn = 2 * ((long long)tr * ringM + m) + ri;
           const bool south = tr < A.T - 1;  // the equator ring has no mirror
-          const long long rs = 2 * ((long long)(4 * A.nside - 2 - tr) * ringM + m) + ri;
+          const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
           if (lo) {
             bp = (double)Vp[rn];
             bm = south ? par0 * (double)Vp[rs] : 0.0;
@@ -868,7 +868,7 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __re
         const float2 n = Vb[(long long)tr * ringM + m];
         vn = c ? n.y : n.x;
         if (tr < A.T - 1) {
-          const float2 s = Vb[(long long)(4 * A.nside - 2 - tr) * ringM + m];
+          const float2 s = Vb[(long long)(A.nrings - 1 - tr) * ringM + m];
           vs = c ? s.y : s.x;
         }
       }

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_synth_s0(RecArgs A, const double2* __re
     const int t = tb + r;
     if (t < A.T) {
       Fb[(long long)t * A.M + m] = c0[r] + c1[r];
-      if (t < A.T - 1) Fb[(long long)(4 * A.nside - 2 - t) * A.M + m] = c0[r] - c1[r];
+      if (t < A.T - 1) Fb[(long long)(A.nrings - 1 - t) * A.M + m] = c0[r] - c1[r];
     }
   }
 }
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
     Fp[(long long)t * ringM + m] = make_double2(-PE.x + PB.y, -PE.y - PB.x);
     Fm[(long long)t * ringM + m] = make_double2(-ME.x - MB.y, -ME.y + MB.x);
     if (t < A.T - 1) {
-      const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+      const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
       const double2 dE = c0[r].me - c1[r].me, dB = c0[r].mb - c1[r].mb;  // from mu-
       const double2 qE = c0[r].pe - c1[r].pe, qB = c0[r].pb - c1[r].pb;  // from mu+
       Fp[rs] = make_double2(-dE.x + dB.y, -dE.y - dB.x);
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __rest
       mup[r] = -s;
       const double2 vn = Vb[(long long)t * ringM + m];
       double2 vs = make_double2(0.0, 0.0);
-      if (t < A.T - 1) vs = Vb[(long long)(4 * A.nside - 2 - t) * ringM + m];
+      if (t < A.T - 1) vs = Vb[(long long)(A.nrings - 1 - t) * ringM + m];
       U0[r] = vn + vs;  // (l - m) even
       U1[r] = vn - vs;  // (l - m) odd
     }
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
       if (t < A.T - 1) {
         // (-1)^(l-m) = (-1)^(l-lmin) (-1)^(lmin-m): fold the second factor into the data
         const double sg = ((lmin - m) & 1) ? -1.0 : 1.0;
-        const long long rs = (long long)(4 * A.nside - 2 - t) * ringM + m;
+        const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
         Vps[r] = make_double2(sg * Vp[rs].x, sg * Vp[rs].y);
         Vms[r] = make_double2(sg * Vm[rs].x, sg * Vm[rs].y);
       }
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* _
       if (tr < A.T) {
         const long long rn = 2 * ((long long)tr * ringM + m) + ri;
         const bool south = tr < A.T - 1;  // the equator ring has no mirror
-        const long long rs = 2 * ((long long)(4 * A.nside - 2 - tr) * ringM + m) + ri;
+        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
         if (lo) {
           bn = Vp[rn];
           cs = south ? Vp[rs] : 0.0;
