@@ -176,20 +176,32 @@ int jhtx_nufft_grid_fill(const void* X, int Nk, int lmax, const double* sk, cons
 int jhtx_nufft_grid_gather(const void* C, int Nk, int lmax, const double* sk, const double* sm, void* X,
                            int nk, int nm, void* stream);
 
+/* Bytes of the optional workspace of jhtx_nufft_interp / jhtx_nufft_spread for npts points
+ * on an nk x nm grid (no counterpart in the reference): a counting sort of the points
+ * into 32 x 32-cell grid tiles. */
+size_t jhtx_nufft_workspace_bytes(long long npts, int nk, int nm);
+
 /* Replaces _stencil + the stencil sum of nufft2d2 (src/jht/_nufft.py:140-170): f_j =
  * sum of the W x W ES-weighted grid values around loc_j = (theta_j, phi_j) (grid
  * complex128 nk x nm, the 2 pi torus).  re / im receive Re f, Im f (im may be NULL).
  * dloc, when not NULL, receives [4][npts]: Re, Im of df/dtheta, then of df/dphi (the
  * pointing derivative).  W in {6, 8, 10, 14, 16}.  npts = 0 is a no-op (point buffers may
- * then be NULL). */
+ * then be NULL).  With a workspace of jhtx_nufft_workspace_bytes bytes the points are
+ * sorted into grid tiles and each tile (plus halo) is read from the grid once into LDS;
+ * with workspace NULL each point gathers its stencil from the grid. */
 int jhtx_nufft_interp(const void* grid, int nk, int nm, int W, double beta, const double* loc,
-                      long long npts, double* re, double* im, double* dloc, void* stream);
+                      long long npts, double* re, double* im, double* dloc, void* workspace,
+                      size_t workspace_bytes, void* stream);
 
 /* Replaces the spreading of nufft2d1 (src/jht/_nufft.py:173-178): adds (re_j + i im_j) times
  * the ES weights around loc_j into grid (which the caller zeroes); im may be NULL; npts = 0
- * is a no-op. */
+ * is a no-op.  With a workspace of jhtx_nufft_workspace_bytes bytes the points are sorted
+ * into grid tiles and each tile is accumulated in LDS before one addition to the grid;
+ * with workspace NULL every stencil cell is a global atomic addition.  Either way the
+ * summation order, hence the last bits, may vary between runs. */
 int jhtx_nufft_spread(const double* re, const double* im, const double* loc, long long npts, void* grid,
-                      int nk, int nm, int W, double beta, void* stream);
+                      int nk, int nm, int W, double beta, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 #ifdef __cplusplus
 }

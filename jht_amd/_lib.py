@@ -34,6 +34,7 @@ EXPORTS = (
     "jhtx_nufft_grid_gather",
     "jhtx_nufft_interp",
     "jhtx_nufft_spread",
+    "jhtx_nufft_workspace_bytes",
 )
 
 JHTX_OK = 0
@@ -114,13 +115,15 @@ def load() -> ctypes.CDLL:
         "jhtx_dfs_fold": [vp, i, i, i, vp, vp],
         "jhtx_nufft_grid_fill": [vp, i, i, vp, vp, vp, i, i, vp],
         "jhtx_nufft_grid_gather": [vp, i, i, vp, vp, vp, i, i, vp],
-        "jhtx_nufft_interp": [vp, i, i, i, d, vp, ll, vp, vp, vp, vp],
-        "jhtx_nufft_spread": [vp, vp, vp, ll, vp, i, i, i, d, vp],
+        "jhtx_nufft_interp": [vp, i, i, i, d, vp, ll, vp, vp, vp, vp, ctypes.c_size_t, vp],
+        "jhtx_nufft_spread": [vp, vp, vp, ll, vp, i, i, i, d, vp, ctypes.c_size_t, vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    lib.jhtx_nufft_workspace_bytes.argtypes = [ll, i, i]
+    lib.jhtx_nufft_workspace_bytes.restype = ctypes.c_size_t
     _LIB = lib
     return lib
 

@@ -213,7 +213,215 @@ __global__ __launch_bounds__(256) void k_spread(const double* __restrict__ re, c
   }
 }
 
+// Binned spreading.  The grid is cut into kTile x kTile cell tiles; a point belongs to the
+// tile holding its stencil's first cell.  A counting sort groups the points by tile
+// (k_tile_count: per-tile counts and each point's rank; k_tile_offsets: exclusive prefix
+// sum; k_tile_scatter: the permutation), then one work-group per tile spreads its points
+// into an LDS copy of the tile plus a W - 1 halo with LDS FP64 atomics and adds the
+// non-zero cells to the grid once (tiles and halos of neighbours overlap: global atomics).
+constexpr int kTile = 32;
+
+__global__ void k_fill_int(int* __restrict__ p, long long n, int v) {
+  const long long i = gid();
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int tile_of(double x, int n, int W, int& first) {
+  first = wrap((long long)ceil(x - 0.5 * W), n);
+  return first / kTile;
+}
+
+__global__ void k_tile_count(const double* __restrict__ loc, long long npts, int nk, int nm, int W,
+                             int* __restrict__ counts, int* __restrict__ tile, int* __restrict__ rank) {
+  const long long j = gid();
+  if (j >= npts) return;
+  int r0, c0;
+  const int tr = tile_of(loc[2 * j] * (nk / kTwoPi), nk, W, r0);
+  const int tc = tile_of(loc[2 * j + 1] * (nm / kTwoPi), nm, W, c0);
+  const int t = tr * ((nm + kTile - 1) / kTile) + tc;
+  tile[j] = t;
+  rank[j] = atomicAdd(counts + t, 1);
+}
+
+// Exclusive prefix sum of counts[0, n) into offsets[0, n] by one work-group of 1024 threads:
+// each thread sums a contiguous run, the run totals are scanned in LDS, then each run is
+// written out.
+__global__ __launch_bounds__(1024) void k_tile_offsets(const int* __restrict__ counts, int n, int* __restrict__ offsets) {
+  __shared__ long long part[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = tid * per, e = min(n, b + per);
+  long long s = 0;
+  for (int i = b; i < e; ++i) s += counts[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const long long v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  long long run = part[tid] - s;  // exclusive start of this thread's run
+  for (int i = b; i < e; ++i) {
+    offsets[i] = (int)run;
+    run += counts[i];
+  }
+  if (tid == 1023) offsets[n] = (int)part[1023];
+}
+
+__global__ void k_tile_scatter(const int* __restrict__ tile, const int* __restrict__ rank, long long npts,
+                               const int* __restrict__ offsets, int* __restrict__ perm) {
+  const long long j = gid();
+  if (j >= npts) return;
+  perm[offsets[tile[j]] + rank[j]] = (int)j;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_spread_tiled(const double* __restrict__ re, const double* __restrict__ im,
+                                                      const double* __restrict__ loc, const int* __restrict__ perm,
+                                                      const int* __restrict__ offsets, double2* g, int nk, int nm,
+                                                      double beta) {
+  constexpr int E = kTile + W - 1;  // tile + halo, cells per side
+  __shared__ double s_re[E * E], s_im[E * E];
+  const int ntc = (nm + kTile - 1) / kTile;
+  const int t = blockIdx.x;
+  const int p0 = offsets[t], p1 = offsets[t + 1];
+  if (p0 == p1) return;
+  const int tr = t / ntc, tc = t - tr * ntc;
+  const int row0 = tr * kTile, col0 = tc * kTile;
+  for (int i = threadIdx.x; i < E * E; i += blockDim.x) s_re[i] = s_im[i] = 0.0;
+  __syncthreads();
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const int j = perm[q];
+    long long bk, bm;
+    double wk[W], wm[W], unused[W];
+    es_weights<W, false>(loc[2 * j] * (nk / kTwoPi), beta, bk, wk, unused);
+    es_weights<W, false>(loc[2 * j + 1] * (nm / kTwoPi), beta, bm, wm, unused);
+    // first stencil cell inside the tile: local offsets are in [0, kTile)
+    const int lr = wrap(bk, nk) - row0, lc = wrap(bm, nm) - col0;
+    const double vr = re[j], vi = im ? im[j] : 0.0;
+    for (int o1 = 0; o1 < W; ++o1) {
+      const int base = (lr + o1) * E + lc;
+      const double ar = wk[o1] * vr, ai = wk[o1] * vi;
+#pragma unroll
+      for (int o2 = 0; o2 < W; ++o2) {
+        unsafeAtomicAdd(s_re + base + o2, wm[o2] * ar);
+        if (vi != 0.0) unsafeAtomicAdd(s_im + base + o2, wm[o2] * ai);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < E * E; i += blockDim.x) {
+    const double vr = s_re[i], vi = s_im[i];
+    if (vr == 0.0 && vi == 0.0) continue;
+    const int r = i / E, c = i - r * E;
+    double* cell = reinterpret_cast<double*>(g + (long long)wrap(row0 + r, nk) * nm + wrap(col0 + c, nm));
+    unsafeAtomicAdd(cell, vr);
+    if (vi != 0.0) unsafeAtomicAdd(cell + 1, vi);
+  }
+}
+
+// Binned interpolation: one work-group per tile stages the tile plus its W - 1 halo from
+// the grid into LDS once (rows and columns wrap), then each thread evaluates the stencils
+// of the tile's points from LDS.
+template <int W, bool DERIV>
+__global__ __launch_bounds__(256) void k_interp_tiled(const double2* __restrict__ g, int nk, int nm, double beta,
+                                                      const double* __restrict__ loc, const int* __restrict__ perm,
+                                                      const int* __restrict__ offsets, long long npts,
+                                                      double* __restrict__ re, double* __restrict__ im,
+                                                      double* __restrict__ dloc) {
+  constexpr int E = kTile + W - 1;
+  __shared__ double2 s_g[E * E];
+  const int ntc = (nm + kTile - 1) / kTile;
+  const int t = blockIdx.x;
+  const int p0 = offsets[t], p1 = offsets[t + 1];
+  if (p0 == p1) return;
+  const int tr = t / ntc, tc = t - tr * ntc;
+  const int row0 = tr * kTile, col0 = tc * kTile;
+  for (int i = threadIdx.x; i < E * E; i += blockDim.x) {
+    const int r = i / E, c = i - r * E;
+    s_g[i] = g[(long long)wrap(row0 + r, nk) * nm + wrap(col0 + c, nm)];
+  }
+  __syncthreads();
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const int j = perm[q];
+    long long bk, bm;
+    double wk[W], dwk[W], wm[W], dwm[W];
+    es_weights<W, DERIV>(loc[2 * j] * (nk / kTwoPi), beta, bk, wk, dwk);
+    es_weights<W, DERIV>(loc[2 * j + 1] * (nm / kTwoPi), beta, bm, wm, dwm);
+    const int lr = wrap(bk, nk) - row0, lc = wrap(bm, nm) - col0;
+    double2 f = make_double2(0.0, 0.0), ft = f, fp = f;
+    for (int o1 = 0; o1 < W; ++o1) {
+      const double2* row = s_g + (lr + o1) * E + lc;
+      double2 sr = make_double2(0.0, 0.0), sd = sr;
+#pragma unroll
+      for (int o2 = 0; o2 < W; ++o2) {
+        const double2 v = row[o2];
+        sr.x = fma(wm[o2], v.x, sr.x);
+        sr.y = fma(wm[o2], v.y, sr.y);
+        if (DERIV) {
+          sd.x = fma(dwm[o2], v.x, sd.x);
+          sd.y = fma(dwm[o2], v.y, sd.y);
+        }
+      }
+      f.x = fma(wk[o1], sr.x, f.x);
+      f.y = fma(wk[o1], sr.y, f.y);
+      if (DERIV) {
+        ft.x = fma(dwk[o1], sr.x, ft.x);
+        ft.y = fma(dwk[o1], sr.y, ft.y);
+        fp.x = fma(wk[o1], sd.x, fp.x);
+        fp.y = fma(wk[o1], sd.y, fp.y);
+      }
+    }
+    re[j] = f.x;
+    if (im) im[j] = f.y;
+    if (DERIV) {
+      const double ck = nk / kTwoPi, cm = nm / kTwoPi;
+      dloc[j] = ck * ft.x;
+      dloc[npts + j] = ck * ft.y;
+      dloc[2 * npts + j] = cm * fp.x;
+      dloc[3 * npts + j] = cm * fp.y;
+    }
+  }
+}
+
 int blocks_for(long long n) { return (int)((n + 255) / 256); }
+
+long long tile_count(int nk, int nm) { return (long long)((nk + kTile - 1) / kTile) * ((nm + kTile - 1) / kTile); }
+
+size_t binning_bytes(long long npts, int nk, int nm) {
+  return sizeof(int) * (size_t)(2 * tile_count(nk, nm) + 1 + 3 * npts);
+}
+
+// Counting sort of the points by grid tile into the workspace: [counts nt][offsets nt + 1]
+// [tile npts][rank npts][perm npts] (int32).  perm lists the points of tile t at
+// offsets[t] .. offsets[t + 1].
+int bin_points(const double* loc, long long npts, int nk, int nm, int W, void* workspace, size_t bytes,
+               hipStream_t s, const char* who, const int** perm_out, const int** offsets_out, long long* nt_out) {
+  const long long nt = tile_count(nk, nm);
+  if (nt > 0x7fffffffLL) {
+    set_error(std::string(who) + ": grid too large for binning");
+    return JHTX_EINVAL;
+  }
+  if (bytes < binning_bytes(npts, nk, nm)) {
+    set_error(std::string(who) + ": workspace smaller than jhtx_nufft_workspace_bytes");
+    return JHTX_EINVAL;
+  }
+  int* counts = reinterpret_cast<int*>(workspace);
+  int* offsets = counts + nt;
+  int* tile = offsets + nt + 1;
+  int* rank = tile + npts;
+  int* perm = rank + npts;
+  const dim3 g(blocks_for(npts));
+  hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(nt)), dim3(256), 0, s, counts, nt, 0);
+  hipLaunchKernelGGL(k_tile_count, g, dim3(256), 0, s, loc, npts, nk, nm, W, counts, tile, rank);
+  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, s, counts, (int)nt, offsets);
+  hipLaunchKernelGGL(k_tile_scatter, g, dim3(256), 0, s, tile, rank, npts, offsets, perm);
+  *perm_out = perm;
+  *offsets_out = offsets;
+  *nt_out = nt;
+  return JHTX_OK;
+}
 
 bool valid_width(int W) { return W == 6 || W == 8 || W == 10 || W == 14 || W == 16; }
 
@@ -294,38 +502,83 @@ int jhtx_nufft_grid_gather(const void* C, int Nk, int lmax, const double* sk, co
 }
 
 int jhtx_nufft_interp(const void* grid, int nk, int nm, int W, double beta, const double* loc, long long npts,
-                      double* re, double* im, double* dloc, void* stream) {
+                      double* re, double* im, double* dloc, void* workspace, size_t workspace_bytes, void* stream) {
   if (npts == 0 && grid && valid_width(W)) return JHTX_OK;  // no points (their buffers may be NULL)
-  if (!grid || !loc || !re || npts < 0 || !valid_width(W) || nk < W || nm < W) {
+  if (!grid || !loc || !re || npts < 0 || npts > 0x7fffffffLL || !valid_width(W) || nk < W || nm < W) {
     set_error("jhtx_nufft_interp: invalid arguments");
     return JHTX_EINVAL;
   }
-  const dim3 g(blocks_for(npts));
   const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const double2* gr = reinterpret_cast<const double2*>(grid);
-  if (dloc)
-    launch_interp<true>(W, g, s, gr, nk, nm, beta, loc, npts, re, im, dloc);
-  else
-    launch_interp<false>(W, g, s, gr, nk, nm, beta, loc, npts, re, im, dloc);
+  if (!workspace) {
+    const dim3 g(blocks_for(npts));
+    if (dloc)
+      launch_interp<true>(W, g, s, gr, nk, nm, beta, loc, npts, re, im, dloc);
+    else
+      launch_interp<false>(W, g, s, gr, nk, nm, beta, loc, npts, re, im, dloc);
+    return launch_status("jhtx_nufft_interp");
+  }
+  const int* perm;
+  const int* offsets;
+  long long nt;
+  int rc = bin_points(loc, npts, nk, nm, W, workspace, workspace_bytes, s, "jhtx_nufft_interp", &perm, &offsets, &nt);
+  if (rc) return rc;
+  const dim3 gt((unsigned)nt);
+#define JHTX_INTERP_TILED(WW)                                                                                   \
+  if (dloc)                                                                                                    \
+    hipLaunchKernelGGL((k_interp_tiled<WW, true>), gt, dim3(256), 0, s, gr, nk, nm, beta, loc, perm, offsets, npts, \
+                       re, im, dloc);                                                                          \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_interp_tiled<WW, false>), gt, dim3(256), 0, s, gr, nk, nm, beta, loc, perm, offsets,     \
+                       npts, re, im, dloc);
+  switch (W) {
+    case 6: JHTX_INTERP_TILED(6) break;
+    case 8: JHTX_INTERP_TILED(8) break;
+    case 10: JHTX_INTERP_TILED(10) break;
+    case 14: JHTX_INTERP_TILED(14) break;
+    default: JHTX_INTERP_TILED(16) break;
+  }
+#undef JHTX_INTERP_TILED
   return launch_status("jhtx_nufft_interp");
 }
 
+size_t jhtx_nufft_workspace_bytes(long long npts, int nk, int nm) {
+  if (npts < 0 || nk < 1 || nm < 1) return 0;
+  return binning_bytes(npts, nk, nm);
+}
+
 int jhtx_nufft_spread(const double* re, const double* im, const double* loc, long long npts, void* grid, int nk,
-                      int nm, int W, double beta, void* stream) {
+                      int nm, int W, double beta, void* workspace, size_t workspace_bytes, void* stream) {
   if (npts == 0 && grid && valid_width(W)) return JHTX_OK;  // no points (their buffers may be NULL)
-  if (!grid || !loc || !re || npts < 0 || !valid_width(W) || nk < W || nm < W) {
+  if (!grid || !loc || !re || npts < 0 || npts > 0x7fffffffLL || !valid_width(W) || nk < W || nm < W) {
     set_error("jhtx_nufft_spread: invalid arguments");
     return JHTX_EINVAL;
   }
-  const dim3 g(blocks_for(npts));
   const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   double2* gr = reinterpret_cast<double2*>(grid);
+  const dim3 g(blocks_for(npts));
+  if (!workspace) {  // direct: W^2 global atomics per point
+    switch (W) {
+      case 6: hipLaunchKernelGGL(k_spread<6>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+      case 8: hipLaunchKernelGGL(k_spread<8>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+      case 10: hipLaunchKernelGGL(k_spread<10>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+      case 14: hipLaunchKernelGGL(k_spread<14>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+      default: hipLaunchKernelGGL(k_spread<16>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+    }
+    return launch_status("jhtx_nufft_spread");
+  }
+  const int* perm;
+  const int* offsets;
+  long long nt;
+  int rc = bin_points(loc, npts, nk, nm, W, workspace, workspace_bytes, s, "jhtx_nufft_spread", &perm, &offsets, &nt);
+  if (rc) return rc;
+  const dim3 gt((unsigned)nt);
   switch (W) {
-    case 6: hipLaunchKernelGGL(k_spread<6>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
-    case 8: hipLaunchKernelGGL(k_spread<8>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
-    case 10: hipLaunchKernelGGL(k_spread<10>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
-    case 14: hipLaunchKernelGGL(k_spread<14>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
-    default: hipLaunchKernelGGL(k_spread<16>, g, dim3(256), 0, s, re, im, loc, npts, gr, nk, nm, beta); break;
+    case 6: hipLaunchKernelGGL(k_spread_tiled<6>, gt, dim3(256), 0, s, re, im, loc, perm, offsets, gr, nk, nm, beta); break;
+    case 8: hipLaunchKernelGGL(k_spread_tiled<8>, gt, dim3(256), 0, s, re, im, loc, perm, offsets, gr, nk, nm, beta); break;
+    case 10: hipLaunchKernelGGL(k_spread_tiled<10>, gt, dim3(256), 0, s, re, im, loc, perm, offsets, gr, nk, nm, beta); break;
+    case 14: hipLaunchKernelGGL(k_spread_tiled<14>, gt, dim3(256), 0, s, re, im, loc, perm, offsets, gr, nk, nm, beta); break;
+    default: hipLaunchKernelGGL(k_spread_tiled<16>, gt, dim3(256), 0, s, re, im, loc, perm, offsets, gr, nk, nm, beta); break;
   }
   return launch_status("jhtx_nufft_spread");
 }

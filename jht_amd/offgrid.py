@@ -14,10 +14,12 @@ Pipeline (``src/jht/offgrid.py:124-160``, ``src/jht/_nufft.py:154-181``), FP64 t
 3. an FFT along theta (library FFT);
 4. the deconvolved oversampled grid (``jhtx_nufft_grid_fill``) and its 2-D inverse FFT
    (library FFT);
-5. the exponential-of-semicircle stencil sum at every point (``jhtx_nufft_interp``),
-   which also returns d/dtheta and d/dphi for the pointing gradient.
+5. the exponential-of-semicircle stencil sum at every point (``jhtx_nufft_interp``: points
+   sorted into grid tiles, each tile and its halo staged in LDS once), which also returns
+   d/dtheta and d/dphi for the pointing gradient.
 
-The adjoint runs the transposes in reverse (``jhtx_nufft_spread`` with FP64 atomics, 2-D
+The adjoint runs the transposes in reverse (``jhtx_nufft_spread``: points sorted into grid
+tiles, each tile accumulated in LDS, then added to the grid; 2-D
 FFT, ``jhtx_nufft_grid_gather``, inverse theta FFT, ``jhtx_dfs_fold``,
 ``jhtx_legendre_adjoint``).  Both functions are differentiable in ``alm`` / ``field`` and
 in ``loc`` through torch.autograd.  Spin 0 and 2 (the reference also has 1 and 3).
@@ -113,6 +115,11 @@ class OffgridPlan:
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    def _point_workspace(self, npts: int):
+        """Scratch for the tile sort of the points (the library allocates nothing)."""
+        nws = int(self._lib.jhtx_nufft_workspace_bytes(npts, self.nk, self.nm))
+        return torch.empty(max(nws, 8), dtype=torch.uint8, device=self.device), nws
+
     def grid(self, alm: torch.Tensor) -> torch.Tensor:
         """alm ([2,] K complex128) -> the oversampled grid g (nk x nm) of the type-2 NUFFT."""
         s, lib = self._stream(), self._lib
@@ -134,9 +141,11 @@ class OffgridPlan:
         re = torch.empty(npts, dtype=torch.float64, device=self.device)
         im = torch.empty(npts, dtype=torch.float64, device=self.device) if self.spin else None
         d = torch.empty((4, npts), dtype=torch.float64, device=self.device) if deriv else None
+        ws, nws = self._point_workspace(npts)
         _lib.check(self._lib.jhtx_nufft_interp(g.data_ptr(), self.nk, self.nm, self.W, self.beta, loc.data_ptr(), npts,
                                                re.data_ptr(), im.data_ptr() if im is not None else None,
-                                               d.data_ptr() if d is not None else None, self._stream()))
+                                               d.data_ptr() if d is not None else None, ws.data_ptr(), nws,
+                                               self._stream()))
         return re, im, d
 
     def adjoint(self, re: torch.Tensor, im, loc: torch.Tensor) -> torch.Tensor:
@@ -144,8 +153,11 @@ class OffgridPlan:
         the metric): the transposes of interp, grid and grid_fill, in reverse."""
         s, lib = self._stream(), self._lib
         g = torch.zeros((self.nk, self.nm), dtype=torch.complex128, device=self.device)
+        npts = loc.shape[0]
+        ws, nws = self._point_workspace(npts)
         _lib.check(lib.jhtx_nufft_spread(re.data_ptr(), im.data_ptr() if im is not None else None, loc.data_ptr(),
-                                         loc.shape[0], g.data_ptr(), self.nk, self.nm, self.W, self.beta, s))
+                                         npts, g.data_ptr(), self.nk, self.nm, self.W, self.beta,
+                                         ws.data_ptr(), nws, s))
         C = torch.fft.fft2(g).contiguous()
         del g
         Y = torch.empty((self.Nk, self.Nm), dtype=torch.complex128, device=self.device)

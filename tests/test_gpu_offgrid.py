@@ -214,3 +214,39 @@ def test_errors_match_reference():
     assert jht_amd.synthesis_general(a, empty, lmax=lmax).shape == (0,)
     assert float(jht_amd.adjoint_synthesis_general(torch.zeros(0, dtype=torch.float64, device="cuda"), empty,
                                                    lmax=lmax).abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("W", [6, 14, 16])
+def test_binned_and_direct_nufft_paths_agree(W):
+    """jhtx_nufft_interp / jhtx_nufft_spread with the tile-sort workspace against the direct
+    (per-point gather / global atomic) paths, through the C ABI, on a random grid with
+    points clustered near the poles and on tile borders."""
+    from jht_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(800 + W)
+    nk, nm, npts, beta = 200, 150, 20000, 2.3 * W
+    g = torch.tensor(rng.standard_normal((nk, nm)) + 1j * rng.standard_normal((nk, nm)), device="cuda")
+    th = np.concatenate([rng.uniform(0, np.pi, npts - 200), rng.uniform(0, 1e-3, 100),
+                         np.full(100, 2 * np.pi * 32 / nk)])
+    loc = torch.tensor(np.stack([th, rng.uniform(-1.0, 7.0, npts)], axis=1), device="cuda")
+    nws = int(lib.jhtx_nufft_workspace_bytes(npts, nk, nm))
+    ws = torch.empty(nws, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for w, n in ((None, 0), (ws.data_ptr(), nws)):
+        re = torch.empty(npts, dtype=torch.float64, device="cuda")
+        im = torch.empty_like(re)
+        d = torch.empty((4, npts), dtype=torch.float64, device="cuda")
+        _lib.check(lib.jhtx_nufft_interp(g.data_ptr(), nk, nm, W, beta, loc.data_ptr(), npts, re.data_ptr(),
+                                         im.data_ptr(), d.data_ptr(), w, n, s))
+        grid = torch.zeros((nk, nm), dtype=torch.complex128, device="cuda")
+        _lib.check(lib.jhtx_nufft_spread(re.data_ptr(), im.data_ptr(), loc.data_ptr(), npts, grid.data_ptr(),
+                                         nk, nm, W, beta, w, n, s))
+        outs.append((re, im, d, grid))
+    for a, b in zip(outs[0], outs[1]):
+        assert _rel(b, a) < 1e-13
+    # an undersized workspace is refused
+    with pytest.raises(_lib.JhtxError):
+        _lib.check(lib.jhtx_nufft_spread(outs[0][0].data_ptr(), None, loc.data_ptr(), npts, outs[0][3].data_ptr(),
+                                         nk, nm, W, beta, ws.data_ptr(), nws - 4, s))
