@@ -951,7 +951,7 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
 // waves per work-group of the mixed adjoint: 8 (512 rings per partial) for the 8-map
 // spin-2 kernel on large grids, else 4
 int mixed_adj_waves(const jhtx_plan* p) {
-  return (p->spin == 2 && p->mx_adj_group == 8 && p->mx_adj_wide && p->T >= 1024) ? 8 : 4;
+  return (p->spin == 2 && p->mx_adj_group >= 8 && p->mx_adj_wide && p->T >= 1024) ? 8 : 4;
 }
 
 int mixed_adj_chunks(const jhtx_plan* p) {
@@ -997,6 +997,8 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
+  else if (MG == 16 && mixed_adj_waves(p) == 8)
+    hipLaunchKernelGGL((k_adj_mx_s2<4, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 16)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 8 && mixed_adj_waves(p) == 8)

@@ -229,17 +229,18 @@ def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkey
     assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
-@pytest.mark.parametrize("wide", [1, 0])
-def test_mixed_adjoint_ring_chunks(wide, monkeypatch):
+@pytest.mark.parametrize("wide,batch", [(1, 9), (0, 9), (1, 18), (0, 18)])
+def test_mixed_adjoint_ring_chunks(wide, batch, monkeypatch):
     """Spin-2 mixed adjoint with 512-ring (wide) and 256-ring work-groups on a grid large
-    enough for the wide kernel (T = 1024 rings), against the FP64 adjoint."""
+    enough for the wide kernel (T = 1024 rings), 8-map (batch 9) and 16-map (batch 18, a
+    ragged last group) recursions, against the FP64 adjoint."""
     j = _jht()
     from jht_amd import healpix as H
 
     monkeypatch.setenv("JHTX_MX_ADJ_WIDE", str(wide))
     H.clear_plans()
     try:
-        nside, lmax, batch = 512, 640, 9
+        nside, lmax = 512, 640
         rng = np.random.default_rng(500 + wide)
         v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
         b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
@@ -247,7 +248,7 @@ def test_mixed_adjoint_ring_chunks(wide, monkeypatch):
     finally:
         H.clear_plans()
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
-    print(f"mixed adjoint wide={wide}: {ea:.2e}", flush=True)
+    print(f"mixed adjoint wide={wide} batch={batch}: {ea:.2e}", flush=True)
     assert ea <= MIXED_TOL
 
 
