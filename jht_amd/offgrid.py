@@ -270,10 +270,14 @@ def _check_spin(spin: int):
         raise NotImplementedError(f"spin={spin} unsupported by jht_amd's off-grid transforms (0, 2)")
 
 
-def _loc_tensor(loc) -> torch.Tensor:
+def _loc_npts(loc) -> int:
     shape = tuple(loc.shape) if hasattr(loc, "shape") else np.shape(loc)
     if len(shape) != 2 or shape[-1] != 2:
         raise ValueError(f"loc must have shape (npts, 2) [theta, phi]; got {shape}")
+    return int(shape[0])
+
+
+def _loc_tensor(loc) -> torch.Tensor:
     t, _ = _prep(loc, False)
     return t
 
@@ -288,7 +292,7 @@ def synthesis_general(alm, loc, *, spin: int = 0, lmax: int, epsilon: float = 1e
     ``(2, npts)`` (Q, U).  Differentiable in alm and in loc.
     """
     _check_spin(spin)
-    loc_t = _loc_tensor(loc)
+    _loc_npts(loc)
     shape = tuple(alm.shape) if hasattr(alm, "shape") else np.shape(alm)
     expect = (alm_size(lmax),) if spin == 0 else (2, alm_size(lmax))
     if tuple(shape) != expect:
@@ -296,6 +300,7 @@ def synthesis_general(alm, loc, *, spin: int = 0, lmax: int, epsilon: float = 1e
             f"alm shape {tuple(shape)} != expected {expect} for lmax={lmax}, spin={spin} "
             "(a wrong-size alm would otherwise be silently clamped by the gather)"
         )
+    loc_t = _loc_tensor(loc)
     a, single = _prep(alm, True)
     plan = get_offgrid_plan(lmax, spin, epsilon, a.device)
     out = _SynthGeneralFn.apply(a, loc_t, plan)
@@ -307,18 +312,18 @@ def adjoint_synthesis_general(field, loc, *, spin: int = 0, lmax: int, epsilon: 
     ``<S_g a, v>_pts == <a, S_g^T v>_G`` with G = 2 - delta_m0.  field: ``(npts,)`` for
     spin 0 or ``(2, npts)`` real (Q, U) planes.  Differentiable in field and in loc."""
     _check_spin(spin)
-    loc_t = _loc_tensor(loc)
+    npts = _loc_npts(loc)
     if (isinstance(field, torch.Tensor) and field.is_complex()) or (
             not isinstance(field, torch.Tensor) and np.iscomplexobj(field)):
         raise TypeError(
             "field must be real ((npts,) for spin=0; (2, npts) real Q/U planes for spin>0); "
             "a complex field would be silently mis-folded"
         )
-    npts = loc_t.shape[0]
     shape = tuple(field.shape) if hasattr(field, "shape") else np.shape(field)
     expect = (npts,) if spin == 0 else (2, npts)
     if tuple(shape) != expect:
         raise ValueError(f"field shape {tuple(shape)} != expected {expect} for {npts} points")
+    loc_t = _loc_tensor(loc)
     v, single = _prep(field, False)
     plan = get_offgrid_plan(lmax, spin, epsilon, v.device)
     out = _AdjGeneralFn.apply(v, loc_t, plan)

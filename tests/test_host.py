@@ -218,3 +218,56 @@ def test_tuning_validation_without_gpu():
     bad = L.Tuning(mx_adj_wide=3)
     rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL
+
+
+def test_offgrid_entry_validation_without_gpu():
+    """The off-grid C entry points reject bad arguments before any device work, and the
+    host-side NUFFT tables of jht_amd.offgrid match the oracle's (the reference's
+    epsilon table, 2,3,5-smooth grid sizes, ES deconvolution factors)."""
+    from jht_amd import _lib as L
+    from jht_amd import offgrid as OG
+    from oracle import offgrid_oracle as G
+
+    lib = _lib()
+    h = ctypes.c_void_p()
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def rings(theta, nrings, spin=0):
+        arr = (ctypes.c_double * len(theta))(*theta)
+        return lib.jhtx_plan_create_rings(nrings, ctypes.cast(arr, ctypes.c_void_p), 8, spin, 1, ctypes.byref(h))
+
+    half = np.pi / 2
+    assert rings([0.0, 0.5, half], 4) == L.JHTX_EINVAL  # even ring count
+    assert rings([0.0, 0.5, 1.0], 5) == L.JHTX_EINVAL   # last north ring is not the equator
+    assert b"equator" in lib.jhtx_last_error()
+    assert rings([0.5, 0.2, half], 5) == L.JHTX_EINVAL  # not sorted
+    assert rings([0.0, 0.5, half], 5, spin=1) == L.JHTX_EUNSUPPORTED
+    assert not h.value
+    assert lib.jhtx_dfs_extend(None, 5, 8, 0, None, None) == L.JHTX_EINVAL
+    assert lib.jhtx_nufft_interp(None, 64, 64, 14, 1.0, None, 10, None, None, None, None, 0, None) == L.JHTX_EINVAL
+    assert lib.jhtx_nufft_spread(None, None, None, 10, None, 64, 64, 7, 1.0, None, 0, None) == L.JHTX_EINVAL
+    assert lib.jhtx_nufft_workspace_bytes(1000, 64, 64) == 4 * (2 * 4 + 1 + 3000)
+    # host tables
+    for lmax, eps in ((12, 1e-10), (100, 1e-6), (7, 3e-9)):
+        ns = OG._smooth_even(lmax + 1) + 1
+        ns_o, _ = G.cc_rings(lmax)
+        assert ns == ns_o
+        W, sigma = OG._kernel_params(eps)
+        assert (W, sigma) == G.kernel_params(eps)
+        p = G.NufftPlan(2 * ns - 2, 2 * lmax + 1, eps)
+        assert (OG._smooth_even(sigma * (2 * ns - 2)), OG._smooth_even(sigma * (2 * lmax + 1))) == (p.nk, p.nm)
+        np.testing.assert_array_equal(OG._es_correction(p.Nk, p.nk, W, p.beta), p.corr_k)
+
+
+def test_offgrid_python_validation_without_gpu():
+    import jht_amd
+
+    K = jht_amd.alm_size(4)
+    with pytest.raises(NotImplementedError):
+        jht_amd.synthesis_general(np.zeros(K, complex), np.zeros((2, 2)), spin=7, lmax=4)
+    with pytest.raises(NotImplementedError, match="0, 2"):
+        jht_amd.synthesis_general(np.zeros((2, K), complex), np.zeros((2, 2)), spin=3, lmax=4)
+    with pytest.raises(ValueError, match="loc must have shape"):
+        jht_amd.synthesis_general(np.zeros(K, complex), np.zeros(3), lmax=4)
+    with pytest.raises(TypeError, match="field must be real"):
+        jht_amd.adjoint_synthesis_general(np.zeros(3, complex), np.zeros((3, 2)), lmax=4)
