@@ -313,15 +313,20 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
       if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // forward FFT, pointwise product with the kernel spectrum, backward FFT: one call site
+  // forward FFT, pointwise product with the kernel spectrum, backward FFT: one call site.
+  // tid passes through an empty asm register constraint (no instruction) in every
+  // iteration, so the FFT's thread-index arithmetic is recomputed per pass instead of
+  // being hoisted out of the loop and held (spilled) across both passes.
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass) {
-    fft_pow2<E, LGLO, LGHI, T>(x, lgL, tid, pass ? +1 : -1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
-    if (pass == 0 && tid < nt) {
+    int tp = tid;
+    asm volatile("" : "+v"(tp));
+    fft_pow2<E, LGLO, LGHI, T>(x, lgL, tp, pass ? +1 : -1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
+    if (pass == 0 && tp < nt) {
       const cx<T>* B = bspec_of<T>(A) + bofs;  // spectrum of e^{-i pi n^2/N}; conj for s < 0
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const cx<T> bv = B[tid + nt * e];
+        const cx<T> bv = B[tp + nt * e];
         x[e] = cmul(x[e], s > 0 ? bv : cconj(bv));
         if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bound in-flight loads
       }
@@ -439,12 +444,15 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
   constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
   cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
-  const int p = A.pairs[blockIdx.x];
+  // spin 2: one work-group per ring (blockIdx.x = 2 pair slot + member); spin 0 packs the
+  // pair into one complex transform
+  const int p = A.pairs[SPIN == 2 ? (blockIdx.x >> 1) : blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int rn = p;
   const bool pair = p < 2 * A.nside - 1;
   const int rs = 4 * A.nside - 2 - p;
+  if (SPIN == 2 && (blockIdx.x & 1) && !pair) return;  // the equator has no mirror
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
@@ -472,19 +480,15 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
     const cx<T>* Fmb = Fpb + (long long)A.nrings * ringM;
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
-#pragma unroll 1
-    for (int h = 0; h < (pair ? 2 : 1); ++h) {
-      const int r = h ? rs : rn;
-      ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
-      const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
-      const cx<T>* out = l2 + o;
-      const long long st = ring_start(A.nside, r);
-      for (int j = tid; j < N; j += BLOCK) {
-        const cx<T> z = out[j];
-        Q[st + j] = (double)z.x;
-        U[st + j] = (double)z.y;
-      }
-      __syncthreads();
+    const int r = (blockIdx.x & 1) ? rs : rn;
+    ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
+    const cx<T>* out = l2 + o;
+    const long long st = ring_start(A.nside, r);
+    for (int j = tid; j < N; j += BLOCK) {
+      const cx<T> z = out[j];
+      Q[st + j] = (double)z.x;
+      U[st + j] = (double)z.y;
     }
   }
 }
@@ -495,12 +499,13 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
   extern __shared__ double lds[];
   cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
-  const int p = A.pairs[blockIdx.x];
+  const int p = A.pairs[SPIN == 2 ? (blockIdx.x >> 1) : blockIdx.x];  // spin 2: one ring each
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int rn = p;
   const bool pair = p < 2 * A.nside - 1;
   const int rs = 4 * A.nside - 2 - p;
+  if (SPIN == 2 && (blockIdx.x & 1) && !pair) return;  // the equator has no mirror
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
@@ -534,26 +539,23 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   } else {
     const double* Q = map + (long long)b * 2 * A.npix;
     const double* U = Q + A.npix;
-#pragma unroll 1
-    for (int h = 0; h < (pair ? 2 : 1); ++h) {
-      const int r = h ? rs : rn;
-      const long long st = h ? ss : sn;
-      __syncthreads();
-      for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)Q[st + j], (T)U[st + j]);
-      __syncthreads();
-      const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
-      const cx<T>* W = l2 + o;
-      cx<T>* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
-      cx<T>* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
-      for (int m = tid; m < A.M; m += BLOCK) {
-        const int k = m % N, q = m / N;
-        const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-        double2 t64 = make_double2(sg, 0.0);
-        if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
-        const cx<T> t = to_cx<T>(t64);
-        Vp[m] = cmul(W[k], t);
-        Vm[m] = cmul(cconj(W[(N - k) % N]), t);
-      }
+    const bool h = blockIdx.x & 1;
+    const int r = h ? rs : rn;
+    const long long st = h ? ss : sn;
+    for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)Q[st + j], (T)U[st + j]);
+    __syncthreads();
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
+    const cx<T>* W = l2 + o;
+    cx<T>* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
+    cx<T>* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
+    for (int m = tid; m < A.M; m += BLOCK) {
+      const int k = m % N, q = m / N;
+      const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
+      double2 t64 = make_double2(sg, 0.0);
+      if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
+      const cx<T> t = to_cx<T>(t64);
+      Vp[m] = cmul(W[k], t);
+      Vm[m] = cmul(cconj(W[(N - k) % N]), t);
     }
   }
 }
@@ -677,7 +679,7 @@ inline int class_block(int lc, bool f32) { return (lc == 3 && f32 && fft_wide_bi
 template <int SPIN, typename T>
 void launch_class(bool synth, int lc, int kind, const FftArgs& a, dim3 grid, size_t sh,
                   hipStream_t s, const void* in, void* out) {
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4) {  // FP64 at 1024 x 16 spills (128-register cap): 512 x 32
     if (lc == 3 && fft_wide_big()) {
       launch_one<1024, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out);
       return;
@@ -857,7 +859,7 @@ static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, vo
     if (!fc.npairs) continue;
     const FftArgs a = make_fft_args(p, fc);
     const size_t sh = fc.lds_bytes;
-    const dim3 grid(fc.npairs, batch);
+    const dim3 grid(p->spin == 2 ? 2 * fc.npairs : fc.npairs, batch);  // spin 2: a work-group per ring
     if (p->spin == 0) {
       if (f32)
         launch_class<0, float>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
