@@ -134,22 +134,23 @@ int jhtx_plan_query(const jhtx_plan* plan, int what);
 /* Replaces build_recursion_plan(x_cc, +-spin, lmax) as called by the off-grid _prepare
  * (src/jht/offgrid.py:72-78): an FP64 plan over an arbitrary ring set symmetric about the
  * equator.  theta: the (nrings + 1) / 2 north-half colatitudes, nondecreasing in [0, pi/2],
- * the last the equator; rings on the pole are allowed.  Workspace: the adjoint's
- * partials only.  jhtx_synth / jhtx_adjoint refuse such a plan (no HEALPix geometry). */
+ * the last the equator; rings on the pole are allowed.  spin 0, 1, 2 or 3 (HEALPix plans:
+ * 0, 2).  Workspace: the adjoint's partials only.  jhtx_synth / jhtx_adjoint refuse such
+ * a plan (no HEALPix geometry). */
 int jhtx_plan_create_rings(int nrings, const double* theta, int lmax, int spin, int max_batch,
                            jhtx_plan** plan);
 
 /* Replaces synth_contract (src/jht/_recursion.py:372-398) as used by the off-grid
  * _alm_to_Fkm (offgrid.py:147-151): F[b][c][ring][m] = sum_l a^c_lm lambda^{s_c}_lm(ring),
- * complex128, with c the +spin / -spin channel (spin 2: a^+ = -(E + iB), a^- = -(E - iB);
- * spin 0: one channel).  Works on any plan (HEALPix plans give the coefficients before the
+ * complex128, with c the +spin / -spin channel (spin s > 0: a^+ = -(E + iB),
+ * a^- = -(-1)^s (E - iB); spin 0: one channel).  Works on any plan (HEALPix plans give the coefficients before the
  * ring FFT); no workspace. */
 int jhtx_legendre_synth(jhtx_plan* plan, const void* alm, void* F, int batch, void* stream);
 
 /* Replaces adjoint_contract (src/jht/_recursion.py:401-423) as used by the off-grid
  * _Fkm_to_alm (offgrid.py:154-160): the transpose of jhtx_legendre_synth, p = sum_ring
  * lambda^{+s} F^+, n = sum_ring lambda^{-s} F^-, returned as the HEALPix adjoint returns
- * them: spin 0 p; spin 2 (E, B) = (-(p + n) / 2, i (p - n) / 2). */
+ * them: spin 0 p; spin s (E, B) = (-(p + t n) / 2, i (p - t n) / 2), t = (-1)^s. */
 int jhtx_legendre_adjoint(jhtx_plan* plan, const void* F, void* alm, int batch, void* workspace,
                           void* stream);
 

@@ -93,7 +93,7 @@ void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRe
       lsin[t] = pole_log(std::sqrt(s2));
     }
   }
-  if (spin == 2) {
+  if (spin != 0) {
     lcos.resize(T);
     for (int t = 0; t < T; ++t) {
       const double z = x[t];
@@ -105,7 +105,7 @@ void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRe
   }
   alpha.assign(K, 0.0);
   G.assign(K, 0.0);
-  if (spin == 2) gamma.assign(K, 0.0);
+  if (spin != 0) gamma.assign(K, 0.0);
   const long double four_pi = 4.0L * 3.14159265358979323846264338327950288L;
   if (spin == 0) {
     seed.resize(M);
@@ -135,22 +135,25 @@ void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRe
       }
     }
   } else {
+    // spin-weighted (|s| = 1, 2, 3; on-grid plans use 2): the +|s| and -|s| recursions
+    const int sa = spin < 0 ? -spin : spin;
+    const long double ss2 = (long double)sa * sa;
     seed.resize(8 * (size_t)M);
     std::vector<long double> g(lmax + 2);
     for (int m = 0; m < M; ++m) {
-      const int lmin = std::max(m, 2);
-      wigner_seed_params(-m, 2, lmin, &seed[8 * m]);
-      wigner_seed_params(-m, -2, lmin, &seed[8 * m + 4]);
+      const int lmin = std::max(m, sa);
+      wigner_seed_params(-m, sa, lmin, &seed[8 * m]);
+      wigner_seed_params(-m, -sa, lmin, &seed[8 * m + 4]);
       if (lmin > lmax) continue;
       const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
       const long double mm = m;
       // d^{l'+1} = (A x + C) d^{l'} - B d^{l'-1}; coefficients of the produced degree l = l'+1
       auto coefs = [&](int l, long double& A, long double& B, long double& C) {
         const long double lp = l - 1;
-        const long double D = lp * sqrtl(((lp + 1) * (lp + 1) - mm * mm) * ((lp + 1) * (lp + 1) - 4.0L));
+        const long double D = lp * sqrtl(((lp + 1) * (lp + 1) - mm * mm) * ((lp + 1) * (lp + 1) - ss2));
         A = (2 * lp + 1) * lp * (lp + 1) / D;
-        C = 2.0L * mm * (2 * lp + 1) / D;  // -(2l'+1) M M' / D with M = -m, M' = +2
-        B = (lp + 1) * sqrtl((lp * lp - mm * mm) * (lp * lp - 4.0L)) / D;
+        C = (long double)sa * mm * (2 * lp + 1) / D;  // -(2l'+1) M M' / D with M = -m, M' = +|s|
+        B = (lp + 1) * sqrtl((lp * lp - mm * mm) * (lp * lp - ss2)) / D;
       };
       g[lmin] = 1.0L;
       if (lmin + 1 <= lmax) g[lmin + 1] = 1.0L;
@@ -465,8 +468,8 @@ int jhtx_plan_create_rings(int nrings, const double* theta, int lmax, int spin, 
     return JHTX_EINVAL;
   }
   *out = nullptr;
-  if (spin != 0 && spin != 2) {
-    set_error("spin=" + std::to_string(spin) + " unsupported (only 0 and 2)");
+  if (spin < 0 || spin > 3) {
+    set_error("spin=" + std::to_string(spin) + " unsupported (0, 1, 2, 3)");
     return JHTX_EUNSUPPORTED;
   }
   if (!theta || nrings < 1 || nrings % 2 == 0 || nrings > 65535 || lmax < 0 || lmax > 32767 ||

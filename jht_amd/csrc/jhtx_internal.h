@@ -59,6 +59,8 @@ struct RecArgs {
   const double* seed;
   int lmax, M, T, nside, nrings, nchunk;
   long long K;
+  int sabs;      // |spin| (the spin-weighted recursions start at l = max(m, |spin|))
+  double ssign;  // (-1)^spin: the -spin channel's factor, -(-1)^s (E - iB)
 };
 
 // Ring-FFT launch class: ring pairs whose transform length fits one work-group shape.

@@ -945,6 +945,8 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
   a.nrings = p->nrings;
   a.nchunk = (p->T + 255) / 256;
   a.K = p->K;
+  a.sabs = p->spin;  // 0 or 2 (HEALPix plans)
+  a.ssign = 1.0;
   return a;
 }
 

@@ -454,7 +454,7 @@ using namespace jhtx;
 extern "C" {
 
 int jhtx_dfs_extend(const void* F, int nrings, int lmax, int spin, void* D, void* stream) {
-  if (!F || !D || nrings < 3 || nrings % 2 == 0 || lmax < 0 || (spin != 0 && spin != 2)) {
+  if (!F || !D || nrings < 3 || nrings % 2 == 0 || lmax < 0 || spin < 0 || spin > 3) {
     set_error("jhtx_dfs_extend: invalid arguments");
     return JHTX_EINVAL;
   }
@@ -466,7 +466,7 @@ int jhtx_dfs_extend(const void* F, int nrings, int lmax, int spin, void* D, void
 }
 
 int jhtx_dfs_fold(const void* D, int nrings, int lmax, int spin, void* F, void* stream) {
-  if (!F || !D || nrings < 3 || nrings % 2 == 0 || lmax < 0 || (spin != 0 && spin != 2)) {
+  if (!F || !D || nrings < 3 || nrings % 2 == 0 || lmax < 0 || spin < 0 || spin > 3) {
     set_error("jhtx_dfs_fold: invalid arguments");
     return JHTX_EINVAL;
   }

@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lmax = A.lmax;
-  const int lmin = max(m, 2);
+  const int lmin = max(m, A.sabs);
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const double2* aE = alm + (long long)b * 2 * A.K;
   const double2* aB = aE + A.K;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
       }
     }
   }
-  // bp = -(aE + i aB), bm = -(aE - i aB); south values carry (-1)^(l-m)
+  // bp = -(aE + i aB), bm = -(-1)^s (aE - i aB); south values carry (-1)^(l-m)
   const long long ringM = (long long)A.M;
   double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
   double2* Fm = Fp + (long long)A.nrings * ringM;
@@ -343,14 +343,15 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
     const double2 PE = c0[r].pe + c1[r].pe, PB = c0[r].pb + c1[r].pb;
     const double2 ME = c0[r].me + c1[r].me, MB = c0[r].mb + c1[r].mb;
     // -PE - i PB  and  -ME + i MB
+    const double sg = A.ssign;
     Fp[(long long)t * ringM + m] = make_double2(-PE.x + PB.y, -PE.y - PB.x);
-    Fm[(long long)t * ringM + m] = make_double2(-ME.x - MB.y, -ME.y + MB.x);
+    Fm[(long long)t * ringM + m] = make_double2(sg * (-ME.x - MB.y), sg * (-ME.y + MB.x));
     if (t < A.T - 1) {
       const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
       const double2 dE = c0[r].me - c1[r].me, dB = c0[r].mb - c1[r].mb;  // from mu-
       const double2 qE = c0[r].pe - c1[r].pe, qB = c0[r].pb - c1[r].pb;  // from mu+
       Fp[rs] = make_double2(-dE.x + dB.y, -dE.y - dB.x);
-      Fm[rs] = make_double2(-qE.x - qB.y, -qE.y + qB.x);
+      Fm[rs] = make_double2(sg * (-qE.x - qB.y), sg * (-qE.y + qB.x));
     }
   }
 }
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int lmax = A.lmax;
-  const int lmin = max(m, 2);
+  const int lmin = max(m, A.sabs);
   if (lmin > lmax) return;
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const long long ringM = A.M;
@@ -850,8 +851,9 @@ __global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* _
   }
 }
 
-// Sum the ring-chunk partials, apply G_l (and for spin 2 the E/B algebra
-// aE = (-p2 - m2)/2, aB = i (p2 - m2)/2, healpix.py:353-354), write packed a_lm.
+// Sum the ring-chunk partials, apply G_l (and for spin s != 0 the E/B algebra
+// aE = (-p2 - (-1)^s m2)/2, aB = i (p2 - (-1)^s m2)/2, healpix.py:353-354 for s = 2),
+// write packed a_lm.
 // P: the chunk partials' type -- double2 (FP64 path) or float2 (mixed path, whose partials
 // are sums of FP32 matrix-core tiles); the chunk sum is taken in FP64 in a fixed order
 template <typename P>
@@ -877,7 +879,7 @@ __global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restric
     } else {
       double2* aE = alm + (long long)b * 2 * A.K;
       double2* aB = aE + A.K;
-      if (l < 2) {
+      if (l < A.sabs) {
         aE[idx] = make_double2(0.0, 0.0);
         aB[idx] = make_double2(0.0, 0.0);
         continue;
@@ -892,7 +894,7 @@ __global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restric
         m2.y += (double)v.y;
       }
       p2 = make_double2(p2.x * g, p2.y * g);
-      m2 = make_double2(m2.x * g, m2.y * g);
+      m2 = make_double2(m2.x * g * A.ssign, m2.y * g * A.ssign);  // the -spin channel's (-1)^s
       aE[idx] = make_double2(0.5 * (-p2.x - m2.x), 0.5 * (-p2.y - m2.y));
       const double2 d = p2 - m2;
       aB[idx] = make_double2(-0.5 * d.y, 0.5 * d.x);
@@ -916,6 +918,8 @@ RecArgs make_args(const jhtx_plan* p, bool adjoint) {
   a.nrings = p->nrings;
   a.nchunk = adjoint ? p->adj_nchunk : p->nchunk;
   a.K = p->K;
+  a.sabs = p->spin < 0 ? -p->spin : p->spin;
+  a.ssign = (p->spin & 1) ? -1.0 : 1.0;
   return a;
 }
 

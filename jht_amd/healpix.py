@@ -319,7 +319,7 @@ def _synth_backward(b: torch.Tensor, lmax: int, spin: int) -> torch.Tensor:
     """torch gradient w.r.t. a from b = S^T(g) (see the module docstring)."""
     m0, G = _metric_tensors(lmax, str(b.device))
     grad = b * G
-    if spin == 2:
+    if spin != 0:  # E, B at m = 0 (any spin weight: the off-grid pair also takes 1 and 3)
         bE = b[..., 0, :].real
         bB = b[..., 1, :].real
         gE = torch.where(m0, torch.complex(bE, bB), grad[..., 0, :])

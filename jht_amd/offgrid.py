@@ -22,7 +22,8 @@ The adjoint runs the transposes in reverse (``jhtx_nufft_spread``: points sorted
 tiles, each tile accumulated in LDS, then added to the grid; 2-D
 FFT, ``jhtx_nufft_grid_gather``, inverse theta FFT, ``jhtx_dfs_fold``,
 ``jhtx_legendre_adjoint``).  Both functions are differentiable in ``alm`` / ``field`` and
-in ``loc`` through torch.autograd.  Spin 0 and 2 (the reference also has 1 and 3).
+in ``loc`` through torch.autograd.  Spins 0, 1, 2 and 3; for s > 0 the -s channel carries
+(-1)^s (``src/jht/offgrid.py:98-121``).
 """
 
 from __future__ import annotations
@@ -208,8 +209,8 @@ def _loc_grad(plan: OffgridPlan, w: torch.Tensor, d: torch.Tensor) -> torch.Tens
 
 def _adjoint_raw(plan: OffgridPlan, field: torch.Tensor, loc: torch.Tensor) -> torch.Tensor:
     """The strict adjoint in the (2 - delta_m0) metric: the library's Legendre adjoint
-    returns spin 0 sum lambda V and spin 2 (-(p + n) / 2, i (p - n) / 2); the off-grid
-    transpose is spin 0 (p + n) / G and spin 2 (-(p + n), i (p - n)) / G
+    returns spin 0 sum lambda V and spin s (-(p + t n) / 2, i (p - t n) / 2), t = (-1)^s;
+    the off-grid transpose is spin 0 (p + n) / G and spin s (-(p + t n), i (p - t n)) / G
     (src/jht/offgrid.py:113-121, :240-243)."""
     if plan.spin == 0:
         b = plan.adjoint(field, None, loc)[0]
@@ -266,8 +267,6 @@ class _AdjGeneralFn(torch.autograd.Function):
 def _check_spin(spin: int):
     if spin not in (0, 1, 2, 3):
         raise NotImplementedError(f"spin={spin} unsupported (0, 1, 2, 3)")
-    if spin not in (0, 2):
-        raise NotImplementedError(f"spin={spin} unsupported by jht_amd's off-grid transforms (0, 2)")
 
 
 def _loc_npts(loc) -> int:

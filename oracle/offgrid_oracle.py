@@ -217,16 +217,17 @@ def Fkm_to_coeffs(G, lmax: int, spin: int, ns: int):
 
 
 def _channels(alm, lmax: int, spin: int):
-    """(+spin, -spin) channel coefficients: a itself for spin 0; -(E + iB), -(E - iB) for
-    spin 2 (``offgrid.py:98-110``)."""
+    """(+spin, -spin) channel coefficients: a itself for spin 0; -(E + iB) and
+    -(-1)^s (E - iB) otherwise (``offgrid.py:98-110``)."""
     if spin == 0:
         return alm, alm
-    return -(alm[0] + 1j * alm[1]), -(alm[0] - 1j * alm[1])
+    s_sign = (-1.0) ** spin
+    return -(alm[0] + 1j * alm[1]), -s_sign * (alm[0] - 1j * alm[1])
 
 
 def _check_spin(spin: int):
-    if spin not in (0, 2):
-        raise NotImplementedError(f"spin={spin} unsupported here (0, 2)")
+    if spin not in (0, 1, 2, 3):
+        raise NotImplementedError(f"spin={spin} unsupported (0, 1, 2, 3)")
 
 
 def synthesis_general(alm, loc, spin: int, lmax: int, epsilon: float = 1e-10):
@@ -261,14 +262,16 @@ def adjoint_synthesis_general(field, loc, spin: int, lmax: int, epsilon: float =
     g = O.alm_metric_weight(lmax)
     if spin == 0:
         return (p + n) / g
-    return np.stack([-(p + n), 1j * (p - n)]) / g
+    s_sign = (-1.0) ** spin
+    return np.stack([-(p + s_sign * n), 1j * (p - s_sign * n)]) / g
 
 
 def brute_force_field(alm, loc, spin: int, lmax: int):
     """Direct sum of spin-weighted harmonics at each point, negative m included:
     spin 0: f = sum_{l, m>=0} G_m Re(a_lm lambda_lm(theta) e^{i m phi});
-    spin 2: Q + iU = sum_{l, all m} 2a_lm lambda^2_{l,m}(theta) e^{i m phi} with
-    2a_{l,-m} = (-1)^m conj(-2a_lm) (the reality condition of Q, U)."""
+    spin s > 0: Q + iU = sum_{l, all m} sa_lm lambda^s_{l,m}(theta) e^{i m phi} with
+    sa_{l,-m} = (-1)^(s+m) conj(-sa_lm) (the reality condition of Q, U, from
+    conj(sY_lm) = (-1)^(s+m) -sY_{l,-m})."""
     loc = np.asarray(loc, dtype=np.float64)
     th, ph = loc[:, 0], loc[:, 1]
     x = np.cos(th)
@@ -280,20 +283,20 @@ def brute_force_field(alm, loc, spin: int, lmax: int):
             s = lam.T @ alm[cb + m : cb + lmax + 1]
             f += (1.0 if m == 0 else 2.0) * np.real(s * np.exp(1j * m * ph))
         return f
-    ap, am = _channels(np.asarray(alm), lmax, spin)  # +2 a_lm and -2 a_lm, m >= 0
+    ap, am = _channels(np.asarray(alm), lmax, spin)  # +s a_lm and -s a_lm, m >= 0
     z = np.zeros(th.size, dtype=np.complex128)
     for m in range(-lmax, lmax + 1):
         mm = abs(m)
-        lmin = max(mm, 2)
+        lmin = max(mm, spin)
         if lmin > lmax:
             continue
         cb = _cb(mm, lmax)
         if m >= 0:
             c = ap[cb + lmin : cb + lmax + 1]
         else:
-            c = ((-1.0) ** mm) * np.conj(am[cb + lmin : cb + lmax + 1])
-        # lambda^{+2}_{l,m} = (-1)^m sqrt((2l+1)/4pi) d^l_{-m,2}, for either sign of m
-        d = O.wigner_d_column(x, -m, 2, lmax)
+            c = ((-1.0) ** (mm + spin)) * np.conj(am[cb + lmin : cb + lmax + 1])
+        # lambda^{s}_{l,m} = (-1)^m sqrt((2l+1)/4pi) d^l_{-m,s}, for either sign of m
+        d = O.wigner_d_column(x, -m, spin, lmax)
         l = np.arange(lmin, lmax + 1, dtype=np.float64)
         lam = ((-1.0) ** mm) * np.sqrt((2.0 * l + 1.0) / O.FOUR_PI)[:, None] * d
         z += (lam.T @ c) * np.exp(1j * m * ph)

@@ -2,7 +2,7 @@
 adjoint_synthesis_general against the oracle (oracle/offgrid_oracle.py, a restatement of
 src/jht/offgrid.py + src/jht/_nufft.py pinned in tests/test_offgrid_oracle.py).
 
-* parity with the oracle, spin 0 and 2, random points plus the poles and grid nodes, at
+* parity with the oracle, spins 0-3, random points plus the poles and grid nodes, at
   the 1e-10 and 1e-12 kernel tiers (the two run the same algorithm: they agree far below
   the tier);
 * at HEALPix pixel centres the field equals the on-grid GPU synthesis (nside 128,
@@ -36,7 +36,7 @@ def _rel(a, b):
 def _alm(lmax, spin, rng):
     if spin == 0:
         return O.random_alm(lmax, rng)
-    return np.stack([O.random_alm(lmax, rng, 2), O.random_alm(lmax, rng, 2)])
+    return np.stack([O.random_alm(lmax, rng, spin), O.random_alm(lmax, rng, spin)])
 
 
 def _loc(n, rng, extra=True):
@@ -46,7 +46,7 @@ def _loc(n, rng, extra=True):
     return pts
 
 
-@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("spin", [0, 1, 2, 3])
 @pytest.mark.parametrize("lmax,eps", [(12, 1e-10), (40, 1e-12), (7, 1e-6)])
 def test_synthesis_matches_oracle(spin, lmax, eps):
     import jht_amd
@@ -61,7 +61,7 @@ def test_synthesis_matches_oracle(spin, lmax, eps):
     assert got.shape == want.shape and e < 1e-12
 
 
-@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("spin", [0, 1, 2, 3])
 @pytest.mark.parametrize("lmax", [12, 40])
 def test_adjoint_matches_oracle(spin, lmax):
     import jht_amd
@@ -111,7 +111,7 @@ def test_adjoint_identity_large(spin):
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
 
 
-@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("spin", [0, 1, 2, 3])
 def test_alm_gradient_bridge(spin):
     """d/d alm of 0.5 |S a - d|^2 equals G (S^T r) with the spin-2 m = 0 convention of the
     on-grid operators; checked against central differences along a random direction."""
@@ -139,7 +139,7 @@ def test_alm_gradient_bridge(spin):
     assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd))
 
 
-@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("spin", [0, 1, 2])
 def test_pointing_gradients(spin):
     """d/d loc of <w, S a>(loc) and of <c, S^T v>(loc) against central differences."""
     import jht_amd
@@ -204,7 +204,7 @@ def test_errors_match_reference():
         jht_amd.synthesis_general(a, loc, spin=5, lmax=lmax)
     with pytest.raises(NotImplementedError):
         jht_amd.synthesis_general(torch.zeros((2, O.alm_size(lmax)), dtype=torch.complex128, device="cuda"), loc,
-                                  spin=1, lmax=lmax)
+                                  spin=4, lmax=lmax)
     with pytest.raises(TypeError, match="field must be real"):
         jht_amd.adjoint_synthesis_general(torch.zeros(3, dtype=torch.complex128, device="cuda"), loc, lmax=lmax)
     with pytest.raises(ValueError, match="field shape"):

@@ -121,4 +121,22 @@ def test_poles_grid_nodes_and_periodic_theta():
 
 def test_unsupported_spin_raises():
     with pytest.raises(NotImplementedError):
-        G.synthesis_general(np.zeros(O.alm_size(4), complex), np.zeros((1, 2)), 1, 4)
+        G.synthesis_general(np.zeros(O.alm_size(4), complex), np.zeros((1, 2)), 4, 4)
+
+
+@pytest.mark.parametrize("spin", [1, 3])
+def test_odd_spin_brute_force_and_adjoint(spin):
+    """Spins 1 and 3 (offgrid.py:64-67, the -spin channel carries (-1)^s): the DFS path
+    against the direct harmonic sum, and the adjoint identity."""
+    lmax = 9
+    rng = np.random.default_rng(60 + spin)
+    alm = np.stack([O.random_alm(lmax, rng, spin), O.random_alm(lmax, rng, spin)])
+    loc = _points(60, rng)
+    got = G.synthesis_general(alm, loc, spin, lmax, 1e-12)
+    assert _rel(got, G.brute_force_field(alm, loc, spin, lmax)) < 1e-10
+    v = rng.standard_normal((2, 60))
+    STv = G.adjoint_synthesis_general(v, loc, spin, lmax)
+    g = O.alm_metric_weight(lmax)
+    lhs = float(np.sum(G.synthesis_general(alm, loc, spin, lmax) * v))
+    rhs = float(np.sum(g * np.real(np.conj(alm) * STv)))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
