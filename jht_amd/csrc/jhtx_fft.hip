@@ -129,17 +129,49 @@ __device__ __forceinline__ void fft_pass(cx<T> (&x)[E], int tid, int nt, int Ns,
 #pragma unroll
     for (int t = 0; t < R; ++t) v[t] = x[bb + t * NB];
     if (Ns > 1) {
-      // one table load per butterfly; w^2, w^3 by complex products (<= 2 ulp)
+      // one table load per butterfly; w^2, w^3, ... by complex products (<= 2 ulp each)
       cx<T> w = tw[k * step];
       if (dir > 0) w.y = -w.y;
       v[1] = cmul(v[1], w);
-      if (R == 4) {
+      if (R >= 4) {
         const cx<T> w2 = cmul(w, w);
+        const cx<T> w3 = cmul(w2, w);
         v[2] = cmul(v[2], w2);
-        v[3] = cmul(v[3], cmul(w2, w));
+        v[3] = cmul(v[3], w3);
+        if (R == 8) {
+          const cx<T> w4 = cmul(w2, w2);
+          v[4] = cmul(v[4], w4);
+          v[5] = cmul(v[5], cmul(w4, w));
+          v[6] = cmul(v[6], cmul(w4, w2));
+          v[7] = cmul(v[7], cmul(w4, w3));
+        }
       }
     }
-    if (R == 2) {
+    if (R == 8) {
+      // two 4-point DFTs (even, odd inputs), combined with W8^k = e^{dir i pi k / 4}
+      const cx<T> e0 = cadd(v[0], v[4]), e1 = csub(v[0], v[4]);
+      const cx<T> e2 = cadd(v[2], v[6]), ed = csub(v[2], v[6]);
+      const cx<T> e3 = dir < 0 ? mk2(ed.y, -ed.x) : mk2(-ed.y, ed.x);
+      const cx<T> E0 = cadd(e0, e2), E1 = cadd(e1, e3), E2 = csub(e0, e2), E3 = csub(e1, e3);
+      const cx<T> o0 = cadd(v[1], v[5]), o1 = csub(v[1], v[5]);
+      const cx<T> o2 = cadd(v[3], v[7]), od = csub(v[3], v[7]);
+      const cx<T> o3 = dir < 0 ? mk2(od.y, -od.x) : mk2(-od.y, od.x);
+      const cx<T> O0 = cadd(o0, o2), O1 = cadd(o1, o3), O2 = csub(o0, o2), O3 = csub(o1, o3);
+      const T c = (T)0.70710678118654752440;  // cos(pi/4)
+      const T sd = dir < 0 ? -c : c;          // sin(dir pi/4)
+      // W8^1 O1, W8^2 O2 = dir i O2, W8^3 O3
+      const cx<T> t1 = mk2(c * O1.x - sd * O1.y, c * O1.y + sd * O1.x);
+      const cx<T> t2 = dir < 0 ? mk2(O2.y, -O2.x) : mk2(-O2.y, O2.x);
+      const cx<T> t3 = mk2(-c * O3.x - sd * O3.y, -c * O3.y + sd * O3.x);
+      x[bb] = cadd(E0, O0);
+      x[bb + NB] = cadd(E1, t1);
+      x[bb + 2 * NB] = cadd(E2, t2);
+      x[bb + 3 * NB] = cadd(E3, t3);
+      x[bb + 4 * NB] = csub(E0, O0);
+      x[bb + 5 * NB] = csub(E1, t1);
+      x[bb + 6 * NB] = csub(E2, t2);
+      x[bb + 7 * NB] = csub(E3, t3);
+    } else if (R == 2) {
       x[bb] = cadd(v[0], v[1]);
       x[bb + NB] = csub(v[0], v[1]);
     } else {
@@ -160,7 +192,7 @@ template <int R, int E, typename T>
 __device__ __forceinline__ void fft_exchange(cx<T> (&x)[E], int tid, int nt, int Ns, int lgNs,
                                              void* lds, bool split) {
   constexpr int NB = E / R;
-  constexpr int lgR = (R == 2) ? 1 : 2;
+  constexpr int lgR = (R == 2) ? 1 : (R == 4 ? 2 : 3);
   if (!split) {
     cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
     if (tid < nt) {
@@ -206,15 +238,17 @@ __device__ __forceinline__ void fft_exchange(cx<T> (&x)[E], int tid, int nt, int
   }
 }
 
-// Compile-time pass sequence for L = 2^LG: a radix-2 pass first when LG is odd, then
-// radix-4 passes; every pass but the last ends with an exchange through LDS.
+// Compile-time pass sequence for L = 2^LG with radix 8 where it fits: with REM = log2 of
+// the radix still to apply, radix 8 when REM = 0 mod 3, radix 4 when REM = 2 or 1 mod 3
+// (radix 2 for REM = 1), so 2^13 and 2^14 take 5 passes instead of 7; every pass but the
+// last ends with an exchange through LDS.  Radix 8 needs E >= 8 elements per thread.
 template <int E, int LG, int LGNS, typename T>
 __device__ __forceinline__ void fft_passes(cx<T> (&x)[E], int tid, int dir, const cx<T>* tw,
                                            int twL, void* lds, bool split) {
   constexpr int REM = LG - LGNS;
   if constexpr (REM > 0) {
-    constexpr int R = (REM & 1) ? 2 : 4;
-    constexpr int lgR = (R == 2) ? 1 : 2;
+    constexpr int R = (REM == 1) ? 2 : ((REM % 3 == 0 && E % 8 == 0) ? 8 : 4);
+    constexpr int lgR = (R == 2) ? 1 : (R == 4 ? 2 : 3);
     constexpr int nt = (1 << LG) / E;
     constexpr int Ns = 1 << LGNS;
     fft_pass<R, E, T>(x, tid, nt, Ns, dir, tw, twL);
