@@ -23,7 +23,13 @@ from .healpix import (  # noqa: F401
     synthesis_batch,
 )
 from .graph import Instance, TransformGraph  # noqa: F401
-from .offgrid import adjoint_synthesis_general, synthesis_general  # noqa: F401
+from .offgrid import (  # noqa: F401
+    adjoint_synthesis_general,
+    adjoint_synthesis_general_real,
+    synthesis_general,
+    synthesis_general_real,
+)
+from .cache import enable_compilation_cache  # noqa: F401
 from .tune import autotune  # noqa: F401
 
 from .callers import (  # noqa: F401

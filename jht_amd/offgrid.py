@@ -329,5 +329,25 @@ def adjoint_synthesis_general(field, loc, *, spin: int = 0, lmax: int, epsilon: 
     return out.to(torch.complex64) if single else out
 
 
-__all__ = ["synthesis_general", "adjoint_synthesis_general", "OffgridPlan", "get_offgrid_plan",
-           "clear_offgrid_plans"]
+def synthesis_general_real(x, loc, *, spin: int = 0, lmax: int, epsilon: float = 1e-10) -> torch.Tensor:
+    """``S_g o T^-1``: real DOF vector -> field at the points (``src/jht/diff.py:79-92``).
+
+    ``x`` has length ``n_dof(lmax, spin)``; the result is that of
+    :func:`synthesis_general`.  A plain real-linear map, differentiable in ``x`` and ``loc``.
+    """
+    from .callers import real_to_alm
+
+    return synthesis_general(real_to_alm(x, lmax, spin), loc, spin=spin, lmax=lmax, epsilon=epsilon)
+
+
+def adjoint_synthesis_general_real(field, loc, *, spin: int = 0, lmax: int, epsilon: float = 1e-10) -> torch.Tensor:
+    """``T o S_g^T``: field at the points -> real DOF vector (``src/jht/diff.py:95-107``), the
+    exact Euclidean transpose of :func:`synthesis_general_real` (T is an isometry in the
+    (2 - delta_m0) metric)."""
+    from .callers import alm_to_real
+
+    return alm_to_real(adjoint_synthesis_general(field, loc, spin=spin, lmax=lmax, epsilon=epsilon), lmax, spin)
+
+
+__all__ = ["synthesis_general", "adjoint_synthesis_general", "synthesis_general_real",
+           "adjoint_synthesis_general_real", "OffgridPlan", "get_offgrid_plan", "clear_offgrid_plans"]

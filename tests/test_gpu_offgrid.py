@@ -250,3 +250,22 @@ def test_binned_and_direct_nufft_paths_agree(W):
     with pytest.raises(_lib.JhtxError):
         _lib.check(lib.jhtx_nufft_spread(outs[0][0].data_ptr(), None, loc.data_ptr(), npts, outs[0][3].data_ptr(),
                                          nk, nm, W, beta, ws.data_ptr(), nws - 4, s))
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_real_dof_offgrid_pair_is_a_transpose(spin):
+    """synthesis_general_real / adjoint_synthesis_general_real (src/jht/diff.py:79-107):
+    <S x, v> = <x, S^T v> in plain real inner products."""
+    import jht_amd
+
+    lmax = 20
+    rng = np.random.default_rng(900 + spin)
+    n = jht_amd.n_dof(lmax, spin)
+    x = torch.tensor(rng.standard_normal(n), device="cuda")
+    loc = torch.tensor(_loc(300, rng), device="cuda")
+    v = torch.tensor(rng.standard_normal(loc.shape[0] if spin == 0 else (2, loc.shape[0])), device="cuda")
+    Sx = jht_amd.synthesis_general_real(x, loc, spin=spin, lmax=lmax)
+    STv = jht_amd.adjoint_synthesis_general_real(v, loc, spin=spin, lmax=lmax)
+    lhs, rhs = float((Sx * v).sum()), float((x * STv).sum())
+    print(f"real-DOF off-grid spin {spin}: {lhs:.12e} vs {rhs:.12e}", flush=True)
+    assert STv.shape == (n,) and abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))

@@ -271,3 +271,29 @@ def test_offgrid_python_validation_without_gpu():
         jht_amd.synthesis_general(np.zeros(K, complex), np.zeros(3), lmax=4)
     with pytest.raises(TypeError, match="field must be real"):
         jht_amd.adjoint_synthesis_general(np.zeros(3, complex), np.zeros((3, 2)), lmax=4)
+
+
+def test_reference_public_names_exist():
+    """Every public name of the reference package (src/jht/__init__.py:66-106) has a
+    counterpart in jht_amd."""
+    import jht_amd
+
+    names = ["synthesis", "adjoint_synthesis", "synthesis_general", "adjoint_synthesis_general",
+             "analysis", "map2alm", "bare_analysis", "ring_weights", "pixel_weights", "pseudo_alm",
+             "deconvolve", "wiener", "constrained_realization", "synthesis_real", "analysis_real",
+             "synthesis_general_real", "adjoint_synthesis_general_real", "bandpower", "alm_to_real",
+             "real_to_alm", "n_dof", "alm_size", "alm_metric_weight", "enable_compilation_cache",
+             "set_azimuth_fft_mode", "get_azimuth_fft_mode", "enable_looped_fft", "__version__"]
+    missing = [n for n in names if not hasattr(jht_amd, n)]
+    assert not missing, missing
+
+
+def test_enable_compilation_cache(tmp_path, monkeypatch):
+    import jht_amd
+
+    monkeypatch.delenv("ROCFFT_RTC_CACHE_PATH", raising=False)
+    jht_amd.enable_compilation_cache(str(tmp_path / "cache"), min_compile_time_secs=2.0)
+    assert os.environ["ROCFFT_RTC_CACHE_PATH"].startswith(str(tmp_path / "cache"))
+    assert (tmp_path / "cache").is_dir()
+    with pytest.raises(ValueError):
+        jht_amd.enable_compilation_cache(str(tmp_path), min_compile_time_secs=-1)
