@@ -654,27 +654,46 @@ __device__ __forceinline__ void adj_tile_mfma(const float* tile, int lane, const
 // and opposite for the two signs.  The odd rows of the sign - tile are stored negated, which
 // makes the odd-parity accumulator equal sigma_+(column) times the true one; the sign is
 // applied when the tile result is reduced.
+//
+// Tile layout [sign][row][TS2]: row 16 p + i holds step 2 i + p of the tile (parity p), and
+// ring r of the wave sits in column 16 (r % 4) + r / 4.  The A operand of lane (i, k) for
+// the 16 MFMAs of one (parity, sign) is then rings 4 q + k, q = 0..15: 16 contiguous floats,
+// four 16-byte reads.  With TS2 = 68 the rows of the 16 lanes of a read group start 4 banks
+// apart (conflict-free), and a recursion step writes one row of 64 consecutive floats.
+constexpr int TS2 = 68;
+__device__ __forceinline__ int s2_tile_row(int j) { return 16 * (j & 1) + (j >> 1); }
+__device__ __forceinline__ int s2_tile_col(int r) { return 16 * (r & 3) + (r >> 2); }
+
 // act: wave-uniform bit sg set when the sign-sg tile holds a nonzero value (an all-zero
-// sign tile contributes nothing and is skipped).
+// sign tile contributes nothing and is skipped).  The operands of the next (parity, sign)
+// step are read while the MFMAs of the current one run.
 template <int NCB>
 __device__ __forceinline__ void adj_tile_mfma_s2(const float* tile, int lane, const float (&Bv)[NCB][2][16],
                                                  floatx4 (&D)[NCB][2], int act) {
-  const int col = lane >> 4;
+  const float* base = tile + (lane & 15) * TS2 + 16 * (lane >> 4);
+  floatx4 a[2][4];
+  // step s = 2 p + sg
+  auto load = [&](int s, floatx4(&dst)[4]) {
+    const floatx4* src = reinterpret_cast<const floatx4*>(base + ((s & 1) * TL + 16 * (s >> 1)) * TS2);
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
+    for (int n = 0; n < 4; ++n) dst[n] = src[n];
+  };
+  load(0, a[0]);
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) D[c][p] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int row = 2 * (lane & 15) + p;
+  for (int s = 0; s < 4; ++s) {
+    if (s + 1 < 4) load(s + 1, a[(s + 1) & 1]);
+    const int p = s >> 1, sg = s & 1;
+    if (sg == 0) {
 #pragma unroll
-    for (int sg = 0; sg < 2; ++sg) {
-      if (!((act >> sg) & 1)) continue;
-      const float* ts = tile + sg * TL * TS + row * TS + col;
+      for (int c = 0; c < NCB; ++c) D[c][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    if ((act >> sg) & 1) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const float a = ts[4 * q];
+        const float av = a[s & 1][q >> 2][q & 3];
 #pragma unroll
         for (int c = 0; c < NCB; ++c)
-          D[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[c][sg][q], D[c][p], 0, 0, 0);
+          D[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Bv[c][sg][q], D[c][p], 0, 0, 0);
       }
     }
   }
@@ -688,9 +707,11 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   constexpr int MG = 4 * NCB;
   constexpr int RINGS = 64 * NW;
   __shared__ double2 s_ag[CH];
-  __shared__ float tiles[NW][2][TL][TS];  // [wave][sign][l][ring]; reused for the FP64 sums
-  static_assert(sizeof(double) * NW * 2 * 16 * 16 * NCB <= sizeof(float) * NW * 2 * TL * TS,
-                "the FP64 reduction buffer aliases the tiles");
+  __shared__ __align__(16) float tiles[NW][2][TL][TS2];  // [wave][sign][row][col]; reused for the wave sums
+  __shared__ int s_act[NW];
+  // the tile results of a wave, [2 i + p][16 NCB cols + 2 pad], overwrite the wave's own tile
+  constexpr int NC = 16 * NCB + 2;
+  static_assert(2 * 16 * NC <= 2 * TL * TS2, "the tile results fit in the wave's tile");
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -752,8 +773,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   }
   float2* p_p2 = part + (((long long)chunk * batch) * 2) * A.K;  // offset per map below
   float* mytile = &tiles[wave][0][0][0];
-  double* red = reinterpret_cast<double*>(&tiles[0][0][0][0]);  // [wave][p][16 l][16 NCB cols] FP64
-  constexpr int NC = 16 * NCB;
+  const int tcol = s2_tile_col(lane);
   // the next chunk's (alpha, gamma) are loaded into registers while this chunk computes
   double2 pf_ag = make_double2(0.0, 0.0);
   auto prefetch = [&](int l0n) {
@@ -791,9 +811,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
           pp = np;
           pmp = pm;
           pm = nm;
-          mytile[(0 * TL + j0 + jj) * TS + lane] = (float)(np * fp);
+          mytile[(0 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = (float)(np * fp);
           // odd rows (odd l - lmin) of the sign - tile negated: one B serves both parities
-          mytile[(1 * TL + j0 + jj) * TS + lane] = (float)(nm * ((jj & 1) ? -fm : fm));
+          mytile[(1 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = (float)(nm * ((jj & 1) ? -fm : fm));
         }
         if (anydeep) {
           rescale_pair(pp, ppp, ep);
@@ -803,21 +823,10 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
       wave_sync();
       floatx4 D[NCB][2];
       adj_tile_mfma_s2<NCB>(mytile, lane, Bv, D, act);
-      // publish per-wave sums (FP64) in the tile area after every wave has read its tile;
-      // a work-group whose tiles are all zero writes zero partials without the reduction
-      if (!__syncthreads_or(act)) {
-        for (int r = tid; r < 2 * 16 * MG; r += RINGS) {
-          const int p = r / (16 * MG), i = (r / MG) % 16, g = r % MG;
-          const int l = l0 + jb + 2 * i + p;
-          const int b = mg * MG + g;
-          if (l <= lmax && b < batch) {
-            float2* pb = p_p2 + (long long)b * 2 * A.K;
-            pb[cb + l] = pb[A.K + cb + l] = make_float2(0.f, 0.f);
-          }
-        }
-        continue;  // no LDS was written after the barrier: the next tile may start at once
-      }
-      double* myred = red + wave * 2 * 16 * NC;
+      // the wave's tile results replace its tile (its own reads of the tile are complete);
+      // rows 2 i + p with stride NC = 16 NCB + 2 make the stores below and the 8-byte reads
+      // of the reduction conflict-free
+      if (lane == 0) s_act[wave] = act;
 #pragma unroll
       for (int c = 0; c < NCB; ++c)
 #pragma unroll
@@ -826,23 +835,30 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
           for (int r = 0; r < 4; ++r) {
             const int i = 4 * (lane >> 4) + r, jc = lane & 15;
             // odd parity: sigma_+ = -1 for the m2 columns (component lane%4 >= 2)
-            const double sig = (p == 1 && (lane & 2)) ? -1.0 : 1.0;
-            myred[(p * 16 + i) * NC + 16 * c + jc] = sig * (double)D[c][p][r];
+            const float sig = (p == 1 && (lane & 2)) ? -1.0f : 1.0f;
+            mytile[(2 * i + p) * NC + 16 * c + jc] = sig * D[c][p][r];
           }
       __syncthreads();
-      // thread -> (p, i, g): sums the waves for the 4 components of map g at l = l0+jb+2i+p
-      for (int r = tid; r < 2 * 16 * MG; r += RINGS) {
-        const int p = r / (16 * MG), i = (r / MG) % 16, g = r % MG;
+      int any = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) any |= s_act[w];
+      // thread -> (p, i, g, h) with l = l0 + jb + 2 i + p fastest: sums the waves in FP64 for
+      // components 2h, 2h + 1 (p2 for h = 0, m2 for h = 1) of map g; a work-group whose
+      // tiles are all zero stores zeros
+      for (int r = tid; r < 2 * 16 * MG * 2; r += RINGS) {
+        const int p = r & 1, i = (r >> 1) & 15, g = (r >> 5) % MG, h = r / (32 * MG);
         const int l = l0 + jb + 2 * i + p;
         const int b = mg * MG + g;
         if (l <= lmax && b < batch) {
-          double s4[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int w = 0; w < NW; ++w)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) s4[c] += red[((w * 2 + p) * 16 + i) * NC + 4 * g + c];
-          float2* pb = p_p2 + (long long)b * 2 * A.K;
-          pb[cb + l] = make_float2((float)s4[0], (float)s4[1]);
-          pb[A.K + cb + l] = make_float2((float)s4[2], (float)s4[3]);
+          double s0 = 0.0, s1 = 0.0;
+          if (any) {
+            for (int w = 0; w < NW; ++w) {
+              const float2 v = *reinterpret_cast<const float2*>(&tiles[w][0][0][0] + (2 * i + p) * NC + 4 * g + 2 * h);
+              s0 += (double)v.x;
+              s1 += (double)v.y;
+            }
+          }
+          p_p2[(long long)b * 2 * A.K + (long long)h * A.K + cb + l] = make_float2((float)s0, (float)s1);
         }
       }
       __syncthreads();
