@@ -320,6 +320,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   }
   p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
   p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
+  p->mx_adj_cb = pick2("JHTX_MX_ADJ_CB", 1, 0, 1);
   p->f64_adj_mfma = (p->spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {

@@ -105,6 +105,7 @@ struct jhtx_plan {
   int mx_adj_group;    // the same for the mixed spin-2 adjoint
   int mx_adj_occ;      // register budget of the 8-map adjoint: 1 or 2 work-groups per CU
   int mx_adj_wide;     // 8-map adjoint with 512-ring work-groups on large grids (0 or 1)
+  int mx_adj_cb;       // 16-map adjoint in column-block form (0 or 1)
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)

@@ -866,6 +866,169 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   }
 }
 
+// Column-block form of the 16-map spin-2 adjoint (4 waves, 256 rings per work-group).
+// Wave w runs the recursion of rings 64 w .. 64 w + 63 of the chunk into its tile, as in
+// k_adj_mx_s2; after a barrier it contracts column block w (maps 4 w .. 4 w + 3 of the
+// group) over all 256 rings, reading the four tiles.  Every wave then does the same matrix
+// work and skips the same all-zero (ring block, sign) tiles, and its accumulators are the
+// chunk partials of its columns: no cross-wave reduction.  The sum over the chunk's rings
+// is the FP32 accumulation of the matrix instructions.
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
+                                                         float2* __restrict__ part, int batch) {
+  constexpr int NW = 4, MG = 16, RINGS = 256;
+  __shared__ double2 s_ag[CH];
+  __shared__ __align__(16) float tiles[NW][2][TL][TS2];  // [ring block][sign][row][col]
+  __shared__ int s_act[NW];
+  int m, chunk;
+  if (!decode_block<true>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  if (lmin > lmax) return;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
+  int ep = 0, em = 0;
+  if (valid) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m;
+    const double lc = A.lcos[t], ls = A.lsin[t];
+    double s;
+    seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep);
+    ppp = -s;
+    seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
+    pmp = -s;
+  }
+  // B operands of column block `wave`: chunk ring 4 q + lane/16 (q = 0..63), column
+  // lane%16 -> map 4 wave + (lane%16)/4, component lane%4 (see k_adj_mx_s2)
+  const int g = (lane & 15) >> 2, cc = lane & 3;
+  const int bcol = mg * MG + 4 * wave + g;  // the map of this lane's column
+  float Bv[2][64];                          // [sign][q]
+  {
+    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const bool lo = cc < 2;
+    const int ri = cc & 1;
+    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
+    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
+      double bp = 0.0, bm = 0.0;
+      if (tr < A.T && bcol < batch) {
+        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
+        const bool south = tr < A.T - 1;  // the equator ring has no mirror
+        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
+        if (lo) {
+          bp = (double)Vp[rn];
+          bm = south ? par0 * (double)Vp[rs] : 0.0;
+        } else {
+          bp = south ? par0 * (double)Vm[rs] : 0.0;
+          bm = (double)Vm[rn];
+        }
+      }
+      Bv[0][q] = (float)bp;
+      Bv[1][q] = (float)bm;
+    }
+  }
+  float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
+  // odd parity: sigma_+ = -1 for the m2 columns
+  const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
+  float* mytile = &tiles[wave][0][0][0];
+  const int tcol = s2_tile_col(lane);
+  const float* abase = &tiles[0][0][0][0] + (lane & 15) * TS2 + 16 * (lane >> 4);
+  double2 pf_ag = make_double2(0.0, 0.0);
+  auto prefetch = [&](int l0n) {
+    const int l = l0n + tid;
+    pf_ag = make_double2(0.0, 0.0);
+    if (tid < CH && l <= lmax && l != lmin) pf_ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+  };
+  prefetch(lmin);
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) s_ag[tid] = pf_ag;
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+#pragma unroll 1
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+      int act = 0;
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const bool deep = valid && ((ep < 0) || (em < 0));
+        const int anydeep = __ballot(deep) != 0ull;
+        act |= (__ballot(valid && ep == 0) != 0ull ? 1 : 0) | (__ballot(valid && em == 0) != 0ull ? 2 : 0);
+        const double fp = (ep == 0 && valid) ? 1.0 : 0.0, fm = (em == 0 && valid) ? 1.0 : 0.0;
+        double2 ag[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[jb + j0 + jj];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double cp = fma(ag[jj].x, x, ag[jj].y);
+          const double cm = fma(ag[jj].x, x, -ag[jj].y);
+          const double np = fma(cp, pp, -ppp);
+          const double nm = fma(cm, pm, -pmp);
+          ppp = pp;
+          pp = np;
+          pmp = pm;
+          pm = nm;
+          mytile[(0 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = (float)(np * fp);
+          // odd rows (odd l - lmin) of the sign - tile negated: one B serves both parities
+          mytile[(1 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = (float)(nm * ((jj & 1) ? -fm : fm));
+        }
+        if (anydeep) {
+          rescale_pair(pp, ppp, ep);
+          rescale_pair(pm, pmp, em);
+        }
+      }
+      if (lane == 0) s_act[wave] = act;
+      __syncthreads();
+      int actv[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) actv[w] = s_act[w];
+      // D[p][h]: parity p, half h of the 16 MFMAs of a (sign, ring block) step (independent
+      // accumulation chains)
+      floatx4 D[2][2];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) D[p][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg)
+#pragma unroll
+        for (int rb = 0; rb < NW; ++rb) {
+          if (!((actv[rb] >> sg) & 1)) continue;
+          floatx4 a[2][4];
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const floatx4* src = reinterpret_cast<const floatx4*>(abase + ((rb * 2 + sg) * TL + 16 * p) * TS2);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) a[p][n] = src[n];
+          }
+#pragma unroll
+          for (int qq = 0; qq < 16; ++qq)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+              D[p][qq & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][qq >> 2][qq & 3], Bv[sg][16 * rb + qq],
+                                                                  D[p][qq & 1], 0, 0, 0);
+        }
+      // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + jb + 2 i + p
+      if (bcol < batch) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + p;
+            const float v = D[p][0][r] + D[p][1][r];
+            if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * v : v;
+          }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
 __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __restrict__ V,
                                                    float2* __restrict__ part, int batch) {
@@ -1036,6 +1199,8 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16 && mixed_adj_waves(p) == 8)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
+  else if (MG == 16 && p->mx_adj_cb)
+    hipLaunchKernelGGL(k_adj_mx_s2_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 8 && mixed_adj_waves(p) == 8)

@@ -229,15 +229,17 @@ def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkey
     assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
-@pytest.mark.parametrize("wide,batch", [(1, 9), (0, 9), (1, 18), (0, 18)])
-def test_mixed_adjoint_ring_chunks(wide, batch, monkeypatch):
+@pytest.mark.parametrize("wide,batch,cb", [(1, 9, 1), (0, 9, 1), (1, 18, 1), (0, 18, 1), (0, 18, 0), (0, 32, 1)])
+def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
     """Spin-2 mixed adjoint with 512-ring (wide) and 256-ring work-groups on a grid large
     enough for the wide kernel (T = 1024 rings), 8-map (batch 9) and 16-map (batch 18, a
-    ragged last group) recursions, against the FP64 adjoint."""
+    ragged last group; batch 32, two full groups) recursions, the 16-map one in column-block
+    form (cb 1) and with the cross-wave reduction (cb 0), against the FP64 adjoint."""
     j = _jht()
     from jht_amd import healpix as H
 
     monkeypatch.setenv("JHTX_MX_ADJ_WIDE", str(wide))
+    monkeypatch.setenv("JHTX_MX_ADJ_CB", str(cb))
     H.clear_plans()
     try:
         nside, lmax = 512, 640
@@ -248,7 +250,7 @@ def test_mixed_adjoint_ring_chunks(wide, batch, monkeypatch):
     finally:
         H.clear_plans()
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
-    print(f"mixed adjoint wide={wide} batch={batch}: {ea:.2e}", flush=True)
+    print(f"mixed adjoint wide={wide} batch={batch} cb={cb}: {ea:.2e}", flush=True)
     assert ea <= MIXED_TOL
 
 
