@@ -871,8 +871,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
 // k_adj_mx_s2; after a barrier it contracts column block w (maps 4 w .. 4 w + 3 of the
 // group) over all 256 rings, reading the four tiles.  Every wave then does the same matrix
 // work and skips the same all-zero (ring block, sign) tiles, and its accumulators are the
-// chunk partials of its columns: no cross-wave reduction.  The sum over the chunk's rings
-// is the FP32 accumulation of the matrix instructions.
+// chunk partials of its columns: no cross-wave reduction.
 __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
                                                          float2* __restrict__ part, int batch) {
   constexpr int NW = 4, MG = 16, RINGS = 256;
@@ -987,17 +986,19 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2
       int actv[NW];
 #pragma unroll
       for (int w = 0; w < NW; ++w) actv[w] = s_act[w];
-      // D[p][h]: parity p, half h of the 16 MFMAs of a (sign, ring block) step (independent
-      // accumulation chains)
-      floatx4 D[2][2];
+      // D[p][rb]: parity p, ring block rb.  Each ring block accumulates in FP32 over its 64
+      // rings (both signs) and the four are summed in FP64, in ring-block order: the
+      // arithmetic of the per-wave tiles and FP64 cross-wave sum of k_adj_mx_s2 (a single
+      // FP32 accumulation over the 256 rings loses up to 8x accuracy in the high-l bands).
+      floatx4 D[2][NW];
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) D[p][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int rb = 0; rb < NW; ++rb) D[p][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int sg = 0; sg < 2; ++sg)
+      for (int rb = 0; rb < NW; ++rb)
 #pragma unroll
-        for (int rb = 0; rb < NW; ++rb) {
+        for (int sg = 0; sg < 2; ++sg) {
           if (!((actv[rb] >> sg) & 1)) continue;
           floatx4 a[2][4];
 #pragma unroll
@@ -1010,8 +1011,8 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2
           for (int qq = 0; qq < 16; ++qq)
 #pragma unroll
             for (int p = 0; p < 2; ++p)
-              D[p][qq & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][qq >> 2][qq & 3], Bv[sg][16 * rb + qq],
-                                                                  D[p][qq & 1], 0, 0, 0);
+              D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][qq >> 2][qq & 3], Bv[sg][16 * rb + qq],
+                                                              D[p][rb], 0, 0, 0);
         }
       // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + jb + 2 i + p
       if (bcol < batch) {
@@ -1020,8 +1021,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + p;
-            const float v = D[p][0][r] + D[p][1][r];
-            if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * v : v;
+            double v = 0.0;
+#pragma unroll
+            for (int rb = 0; rb < NW; ++rb) v += (double)D[p][rb][r];
+            if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * (float)v : (float)v;
           }
       }
       __syncthreads();

@@ -254,6 +254,26 @@ def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
     assert ea <= MIXED_TOL
 
 
+def test_mixed_adjoint_column_block_bitwise(monkeypatch):
+    """The column-block 16-map adjoint keeps the arithmetic of the cross-wave form (FP32
+    over each 64-ring block, FP64 over the blocks in order): identical bits, ragged batch."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    nside, lmax, batch = 256, 400, 18
+    rng = np.random.default_rng(808)
+    v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
+    out = {}
+    for cb in (1, 0):
+        monkeypatch.setenv("JHTX_MX_ADJ_CB", str(cb))
+        H.clear_plans()
+        try:
+            out[cb] = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="mixed").cpu()
+        finally:
+            H.clear_plans()
+    assert torch.equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("nside,lmax", [(8, 12), (64, 128), (128, 300), (256, 200)])
 def test_f64_adjoint_kernels_agree(nside, lmax, monkeypatch):
     """Spin-2 FP64 adjoint on the matrix cores against the vector-ALU kernel and the oracle
