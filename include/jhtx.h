@@ -64,12 +64,16 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *   mx_adj_group, mx_adj_occ the same for the mixed spin-2 adjoint {4,8,16}, {1,2}
  *   mx_adj_wide              mixed spin-2 adjoint with 512-ring work-groups {1 on, 2 off}
  *   f64_adj_mfma             FP64 spin-2 adjoint contracted on the FP64 matrix cores
- *                            {1 on, 2 off (default): the vector-ALU kernel with adj_rings} */
+ *                            {1 on, 2 off (default): the vector-ALU kernel with adj_rings}
+ *   mx_adj_cb                16-map mixed spin-2 adjoint in column-block form (each wave
+ *                            contracts 4 maps over the work-group's rings; the same bits)
+ *                            {1 on (default), 2 off: per-wave tiles and a cross-wave sum} */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
   int mx_adj_group, mx_adj_occ, mx_adj_wide;
   int f64_adj_mfma;
+  int mx_adj_cb;
 } jhtx_tuning;
 
 /* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An

@@ -320,6 +320,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   }
   p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
   p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
+  // the 16-map adjoint in column-block form (1) or with per-wave tiles summed across waves (0)
   p->mx_adj_cb = pick2("JHTX_MX_ADJ_CB", 1, 0, 1);
   p->f64_adj_mfma = (p->spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
   int block_cap = 256;
@@ -343,7 +344,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
-        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2}));
+        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2})) && (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2}));
     if (!ok) {
       set_error("jhtx_tuning field out of range");
       return JHTX_EINVAL;
@@ -357,6 +358,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
     if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
     if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
     if (t.f64_adj_mfma && p->spin == 2) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
+    if (t.mx_adj_cb) p->mx_adj_cb = (t.mx_adj_cb == 1);
   }
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
@@ -408,6 +410,7 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->mx_adj_occ = p->mx_adj_occ;
   t->mx_adj_wide = p->mx_adj_wide ? 1 : 2;
   t->f64_adj_mfma = p->f64_adj_mfma ? 1 : 2;
+  t->mx_adj_cb = p->mx_adj_cb ? 1 : 2;
   return JHTX_OK;
 }
 
