@@ -856,50 +856,72 @@ __global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* _
 // write packed a_lm.
 // P: the chunk partials' type -- double2 (FP64 path) or float2 (mixed path, whose partials
 // are sums of FP32 matrix-core tiles); the chunk sum is taken in FP64 in a fixed order
+// Sum the ring-chunk partials in FP64 (chunk order, fixed), apply G_l and the E/B
+// combination.  One thread per (map, packed index): the grid runs over the flat triangle,
+// so short high-m columns waste no threads, and the chunk loads are issued 8 at a time.
+// Entries with l < |s| are zero; they sit in the columns m < |s|.
 template <typename P>
-__global__ void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
-                             double2* __restrict__ alm) {
-  const int m = blockIdx.x;
+__global__ __launch_bounds__(256) void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
+                                                    double2* __restrict__ alm) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   const int batch = gridDim.y;
-  const int lmax = A.lmax;
-  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  if (idx >= A.K) return;
   const int ncomp = spin ? 2 : 1;
-  for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
-    const long long idx = cb + l;
-    const double g = A.G[idx];
-    if (spin == 0) {
-      double2 s = make_double2(0.0, 0.0);
-      for (int c = 0; c < nchunk; ++c) {
-        const P v = part[((long long)c * batch + b) * A.K + idx];
-        s.x += (double)v.x;
-        s.y += (double)v.y;
+  const double g = A.G[idx];
+  // part[((c * batch + b) * ncomp + comp) * K + idx]
+  const P* p0 = part + ((long long)b * ncomp) * A.K + idx;
+  const long long cstride = (long long)batch * ncomp * A.K;
+  double2 s0 = make_double2(0.0, 0.0), s1 = s0;
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {
+    P u[8], v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      u[k] = p0[(c + k) * cstride];
+      if (spin) v[k] = p0[(c + k) * cstride + A.K];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      s0.x += (double)u[k].x;
+      s0.y += (double)u[k].y;
+      if (spin) {
+        s1.x += (double)v[k].x;
+        s1.y += (double)v[k].y;
       }
-      alm[(long long)b * A.K + idx] = make_double2(s.x * g, s.y * g);
-    } else {
-      double2* aE = alm + (long long)b * 2 * A.K;
-      double2* aB = aE + A.K;
-      if (l < A.sabs) {
-        aE[idx] = make_double2(0.0, 0.0);
-        aB[idx] = make_double2(0.0, 0.0);
-        continue;
-      }
-      double2 p2 = make_double2(0.0, 0.0), m2 = p2;
-      for (int c = 0; c < nchunk; ++c) {
-        const P u = part[(((long long)c * batch + b) * ncomp + 0) * A.K + idx];
-        const P v = part[(((long long)c * batch + b) * ncomp + 1) * A.K + idx];
-        p2.x += (double)u.x;
-        p2.y += (double)u.y;
-        m2.x += (double)v.x;
-        m2.y += (double)v.y;
-      }
-      p2 = make_double2(p2.x * g, p2.y * g);
-      m2 = make_double2(m2.x * g * A.ssign, m2.y * g * A.ssign);  // the -spin channel's (-1)^s
-      aE[idx] = make_double2(0.5 * (-p2.x - m2.x), 0.5 * (-p2.y - m2.y));
-      const double2 d = p2 - m2;
-      aB[idx] = make_double2(-0.5 * d.y, 0.5 * d.x);
     }
   }
+  for (; c < nchunk; ++c) {
+    const P u = p0[c * cstride];
+    s0.x += (double)u.x;
+    s0.y += (double)u.y;
+    if (spin) {
+      const P v = p0[c * cstride + A.K];
+      s1.x += (double)v.x;
+      s1.y += (double)v.y;
+    }
+  }
+  if (spin == 0) {
+    alm[(long long)b * A.K + idx] = make_double2(s0.x * g, s0.y * g);
+    return;
+  }
+  double2* aE = alm + (long long)b * 2 * A.K;
+  double2* aB = aE + A.K;
+  bool below = false;  // l < |s|: column m < |s| at l = m .. |s| - 1
+  for (int m = 0; m < A.sabs && m <= A.lmax; ++m) {
+    const long long cb = (long long)m * (2 * A.lmax + 1 - m) / 2;
+    below |= (idx >= cb + m) && (idx < cb + A.sabs);
+  }
+  if (below) {
+    aE[idx] = make_double2(0.0, 0.0);
+    aB[idx] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 p2 = make_double2(s0.x * g, s0.y * g);
+  const double2 m2 = make_double2(s1.x * g * A.ssign, s1.y * g * A.ssign);  // the -spin channel's (-1)^s
+  aE[idx] = make_double2(0.5 * (-p2.x - m2.x), 0.5 * (-p2.y - m2.y));
+  const double2 d = p2 - m2;
+  aB[idx] = make_double2(-0.5 * d.y, 0.5 * d.x);
 }
 
 RecArgs make_args(const jhtx_plan* p, bool adjoint) {
@@ -981,14 +1003,16 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s) {
   const RecArgs a = make_args(p, true);
-  hipLaunchKernelGGL(k_adj_finish<double2>, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
+  hipLaunchKernelGGL(k_adj_finish<double2>, dim3((unsigned)((p->K + 255) / 256), batch), dim3(256), 0, s, a, p->spin,
+                     nchunk, part, alm);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s) {
   const RecArgs a = make_args(p, true);
-  hipLaunchKernelGGL(k_adj_finish<float2>, dim3(p->M, batch), dim3(256), 0, s, a, p->spin, nchunk, part, alm);
+  hipLaunchKernelGGL(k_adj_finish<float2>, dim3((unsigned)((p->K + 255) / 256), batch), dim3(256), 0, s, a, p->spin,
+                     nchunk, part, alm);
   return hipGetLastError();
 }
 
