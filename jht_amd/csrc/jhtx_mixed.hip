@@ -615,6 +615,189 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
 }
 
 // ---------------------------------------------------------------------------------- //
+// spin 0 synthesis on 16x16x4 tiles, 16 maps per work-group (grid.y = map group)
+// ---------------------------------------------------------------------------------- //
+// Lane = ring: a wave runs the recursion of 64 rings.  Every 8 l-steps the wave's FP32
+// values go to an LDS tile [step][lane]; v_mfma_f32_16x16x4f32 contracts 4 same-parity
+// steps (K) of 16 rings (M) against 8 maps x (re, im) (N): 2 parities x 4 row blocks x 2
+// column blocks = 16 instructions per 8 steps for 64 rings x 16 maps, a quarter of the
+// instructions of the rank-1 4x4x1 form and the recursion shared by twice the maps.
+// Same software pipeline as k_synth_tile_s2: the matrix instructions of group j and the
+// recursion of group j + 1 (into the other tile buffer) form one straight-line block.
+constexpr int kS0AS = 40;  // s_a row stride (floats): B reads at rows 2k+p, bank 16k+j
+
+// G_l a_lm of 16 maps for l0..l0+CH-1: thread t takes l = l0 + t % CH and maps t / CH + 4u
+// (a wave reads 64 consecutive coefficients of one column), all loads issued at once
+__device__ __forceinline__ void load_alm_s0(const RecArgs& A, const double2* __restrict__ alm, long long cb, int l0,
+                                            int lmax, int mg, int batch, int tid, double2 (&a)[4], double& g) {
+  const int j = tid % CH, b0 = tid / CH;
+  const int l = l0 + j;
+  const bool lok = l <= lmax;
+  g = lok ? A.G[cb + l] : 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int b = mg * 16 + b0 + 4 * u;
+    a[u] = (lok && b < batch) ? alm[(long long)b * A.K + cb + l] : make_double2(0.0, 0.0);
+  }
+}
+
+// s_a[j * kS0AS + 2 map + (re, im)]
+__device__ __forceinline__ void store_alm_s0(const double2 (&a)[4], double g, int tid, float* s_a) {
+  const int j = tid % CH, b0 = tid / CH;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float* d = s_a + j * kS0AS + 2 * (b0 + 4 * u);
+    d[0] = (float)(a[u].x * g);
+    d[1] = (float)(a[u].y * g);
+  }
+}
+
+template <int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const double2* __restrict__ alm,
+                                                             float2* __restrict__ F, int batch) {
+  constexpr int MG = 16, AS = kS0AS, RINGS = 256;
+  constexpr int EPS = 17;  // epilogue staging row stride (floats)
+  constexpr int LOOP_FLOATS = CH * AS + 4 * 2 * 8 * kTileTS;  // s_a + double-buffered wave tiles
+  constexpr int EPI_FLOATS = 4 * 2 * 64 * EPS;                // per wave [parity][ring][column]
+  __shared__ double s_al[CH];
+  __shared__ __align__(16) float s_buf[LOOP_FLOATS > EPI_FLOATS ? LOOP_FLOATS : EPI_FLOATS];
+  float* s_a = s_buf;              // [l][2 map + (re, im)]
+  float* tiles = s_buf + CH * AS;  // [wave][buffer][step][lane]
+  int m, chunk;
+  if (!decode_block<false>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ii = lane & 15, kk = lane >> 4;  // MFMA lane decomposition
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double sv;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, sv, ex);
+    mup = -sv;
+  }
+  floatx4 D[2][4][2];  // [parity][row block][column block]
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float* const tile0 = tiles + wave * 2 * 8 * kTileTS;
+  float* const tile1 = tile0 + 8 * kTileTS;
+  // the next chunk's coefficients are loaded into registers while this chunk computes
+  double2 pf_a[4];
+  double pf_g = 0.0, pf_al = 0.0;
+  auto prefetch = [&](int l0n) {
+    if (tid < CH) {
+      const int l = l0n + tid;
+      pf_al = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    load_alm_s0(A, alm, cb, l0n, lmax, mg, batch, tid, pf_a, pf_g);
+  };
+  prefetch(m);
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) s_al[tid] = pf_al;
+    store_alm_s0(pf_a, pf_g, tid, s_a);
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+    const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
+    int j0 = 0;
+    // every lane below 2^-256: recursion only (a lane's exponent never decreases)
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+      double al[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) al[jj] = s_al[j0 + jj];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double n = fma(al[jj] * x, mu, -mup);
+        mup = mu;
+        mu = n;
+      }
+      rescale_pair(mu, mup, ex);
+      j0 += 8;
+    }
+    auto rec_group = [&](int jg, float* tl) {
+      const double f = (ex == 0) ? 1.0 : 0.0;
+      double al[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) al[jj] = s_al[jg + jj];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double n = fma(al[jj] * x, mu, -mup);
+        mup = mu;
+        mu = n;
+        // the 1/0 mask in FP64 first: a lane below 2^-256 may hold a mantissa above FP32 range
+        tl[jj * kTileTS + lane] = (float)(n * f);
+      }
+      rescale_pair(mu, mup, ex);
+    };
+    auto mfma_group = [&](int jg, const float* tl) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        float bv[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) bv[c] = s_a[(jg + 2 * kk + p) * AS + 16 * c + ii];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const float av = tl[(2 * kk + p) * kTileTS + 16 * rb + ii];
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            D[p][rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], D[p][rb][c], 0, 0, 0);
+        }
+      }
+    };
+    if (j0 < nst) {
+      rec_group(j0, tile0);
+      wave_sync();
+      bool odd = false;  // which buffer holds group j0
+      for (; j0 + 8 < nst; j0 += 8) {
+        mfma_group(j0, odd ? tile1 : tile0);
+        rec_group(j0 + 8, odd ? tile0 : tile1);
+        wave_sync();
+        odd = !odd;
+      }
+      mfma_group(j0, odd ? tile1 : tile0);
+      wave_sync();
+    }
+  }
+  // epilogue, one column block (8 maps) at a time: D[i][j] sits in lane j + 16 (i / 4),
+  // register i % 4; row i of block rb is ring 16 rb + i of the wave, column j = 2 map + re/im
+  __syncthreads();  // the staging below aliases s_a and the tiles
+  float* ep = s_buf + wave * 2 * 64 * EPS;  // [parity][ring][column]
+  const long long ringM = A.M;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep[(p * 64 + 16 * rb + 4 * kk + r) * EPS + ii] = D[p][rb][c][r];
+    wave_sync();
+    if (valid) {
+#pragma unroll
+      for (int h = 0; h < 8; ++h) {
+        const int b = mg * MG + 8 * c + h;
+        if (b >= batch) continue;
+        const double c0r = ep[lane * EPS + 2 * h], c0i = ep[lane * EPS + 2 * h + 1];
+        const double c1r = ep[(64 + lane) * EPS + 2 * h], c1i = ep[(64 + lane) * EPS + 2 * h + 1];
+        float2* Fb = F + (long long)b * A.nrings * ringM;
+        Fb[(long long)t * ringM + m] = f2(c0r + c1r, c0i + c1i);
+        if (t < A.T - 1) Fb[(long long)(A.nrings - 1 - t) * ringM + m] = f2(c0r - c1r, c0i - c1i);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
 // adjoint: per-wave FP32 tile of 32 l x 64 rings (per recursion sign), 16x16x4 MFMA
 // ---------------------------------------------------------------------------------- //
 // 16x16x4 f32 layouts: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
@@ -1163,7 +1346,7 @@ int mixed_adj_chunks(const jhtx_plan* p) {
 }
 
 int mixed_group(const jhtx_plan* p, bool adjoint) {
-  if (p->spin == 0) return 8;
+  if (p->spin == 0) return (!adjoint && p->mx_synth_group >= 16) ? 16 : 8;
   return adjoint ? p->mx_adj_group : p->mx_synth_group;
 }
 
@@ -1178,7 +1361,9 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     at.nchunk = (p->T + 127) / 128;
     dim3 gt(Mpad * at.nchunk, (batch + 15) / 16);
     hipLaunchKernelGGL((k_synth_tile_s2<4, 2>), gt, dim3(256), 0, s, at, alm, F, batch);
-  } else if (p->spin == 0)
+  } else if (p->spin == 0 && MG == 16)
+    hipLaunchKernelGGL((k_synth_tile_s0<2>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8 && p->mx_synth_occ == 3)
     hipLaunchKernelGGL((k_synth_mx_s2<2, 3>), grid, dim3(256), 0, s, a, alm, F, batch);

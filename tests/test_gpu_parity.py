@@ -254,6 +254,30 @@ def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
     assert ea <= MIXED_TOL
 
 
+@pytest.mark.parametrize("group", [16, 8])
+@pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
+def test_mixed_spin0_synthesis_groups(nside, lmax, group, monkeypatch):
+    """Spin-0 mixed synthesis with 16 maps per recursion (the 16x16x4 tile kernel) and 8
+    (the rank-1 kernel), batch 18 (a ragged second group), against the FP64 path."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_SYNTH_GROUP", str(group))
+    H.clear_plans()
+    try:
+        batch = 18
+        rng = np.random.default_rng(90 + group + nside)
+        a = np.stack([O.gaussian_alm(lmax, rng, 0) for _ in range(batch)])
+        at = torch.tensor(a, device="cuda")
+        s64 = j.synthesis_batch(at, nside, lmax, 0, precision="f64").cpu().numpy()
+        smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed").cpu().numpy()
+    finally:
+        H.clear_plans()
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    print(f"mixed spin-0 synthesis group {group} nside={nside} lmax={lmax}: {es:.2e}", flush=True)
+    assert es <= MIXED_TOL
+
+
 def test_mixed_adjoint_column_block_bitwise(monkeypatch):
     """The column-block 16-map adjoint keeps the arithmetic of the cross-wave form (FP32
     over each 64-ring block, FP64 over the blocks in order): identical bits, ragged batch."""
