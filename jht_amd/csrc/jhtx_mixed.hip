@@ -1311,6 +1311,128 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __re
   }
 }
 
+// Column-block form of the spin-0 adjoint for 16 maps (4 waves, 256 rings per work-group).
+// Wave w runs the recursion of rings 64 w .. 64 w + 63 of the chunk into its tile (layout
+// of the spin-2 tiles: row 16 p + i holds step 2 i + p, ring r in column 16 (r % 4) + r / 4);
+// after a barrier it contracts parity p = w / 2 of column block c = w % 2 (maps 8 c .. 8 c + 7,
+// re / im) over all 256 rings against U_p = V_north +- V_south.  Every wave does the same
+// matrix work and skips the same all-zero ring-block tiles; no cross-wave reduction.  Each
+// 64-ring block accumulates in FP32 (two chains of 32 consecutive rings) and the chains
+// are summed in FP64.
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cb(RecArgs A, const float2* __restrict__ V,
+                                                         float2* __restrict__ part, int batch) {
+  constexpr int NW = 4, MG = 16, RINGS = 256;
+  __shared__ double s_al[CH];
+  __shared__ __align__(16) float tiles[NW][TL][TS2];  // [ring block][row][col]
+  __shared__ int s_act[NW];
+  int m, chunk;
+  if (!decode_block<true>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double sv;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, sv, ex);
+    mup = -sv;
+  }
+  // this wave's output: parity pw, column block cw; lane column j = lane % 16 -> map
+  // 8 cw + j / 2, re / im = j % 2; B rows: chunk ring 4 q + lane / 16, q = 0..63
+  const int pw = wave >> 1, cw = wave & 1;
+  const int j = lane & 15;
+  const int bcol = mg * MG + 8 * cw + (j >> 1);
+  float Bv[64];
+  {
+    const float* Vb = reinterpret_cast<const float*>(V + (long long)bcol * A.nrings * ringM);
+    const int ri = j & 1;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
+      double u = 0.0;
+      if (tr < A.T && bcol < batch) {
+        const double vn = (double)Vb[2 * ((long long)tr * ringM + m) + ri];
+        // the equator ring has no mirror
+        const double vs = (tr < A.T - 1) ? (double)Vb[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri] : 0.0;
+        u = pw ? vn - vs : vn + vs;
+      }
+      Bv[q] = (float)u;
+    }
+  }
+  float* pcol = reinterpret_cast<float*>(part + ((long long)chunk * batch + bcol) * A.K) + (j & 1);
+  float* mytile = &tiles[wave][0][0];
+  const int tcol = s2_tile_col(lane);
+  const float* abase = &tiles[0][0][0] + (16 * pw + (lane & 15)) * TS2 + 16 * (lane >> 4);
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+      int act = 0;
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const int anydeep = __ballot(valid && ex < 0) != 0ull;
+        act |= __ballot(valid && ex == 0) != 0ull;
+        const double f = (ex == 0 && valid) ? 1.0 : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double c = s_al[jb + j0 + jj] * x;
+          const double mn = fma(c, mu, -mup);
+          mup = mu;
+          mu = mn;
+          mytile[s2_tile_row(j0 + jj) * TS2 + tcol] = (float)(mn * f);
+        }
+        if (anydeep) rescale_pair(mu, mup, ex);
+      }
+      if (lane == 0) s_act[wave] = act;
+      __syncthreads();
+      floatx4 D[NW][2];
+#pragma unroll
+      for (int rb = 0; rb < NW; ++rb) D[rb][0] = D[rb][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int rb = 0; rb < NW; ++rb) {
+        if (!s_act[rb]) continue;
+        floatx4 a[4];
+        const floatx4* src = reinterpret_cast<const floatx4*>(abase + rb * TL * TS2);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) a[n] = src[n];
+        // two chains over the contiguous halves of the block (rings 0-31, 32-63), issued
+        // alternately: a strided split (every other group of 4 rings) sums nearly in-phase
+        // terms at high l, where consecutive rings advance the recursion by about pi/2,
+        // and measured 2-3x the error in the l >= 1024 bands
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+          D[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[qq >> 2][qq & 3], Bv[16 * rb + qq], D[rb][0], 0, 0, 0);
+          D[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2 + (qq >> 2)][qq & 3], Bv[16 * rb + 8 + qq], D[rb][1],
+                                                          0, 0, 0);
+        }
+      }
+      // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + jb + 2 i + pw
+      if (bcol < batch) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + pw;
+          double v = 0.0;
+#pragma unroll
+          for (int rb = 0; rb < NW; ++rb) v += (double)D[rb][0][r] + (double)D[rb][1][r];
+          if (l <= lmax) pcol[2 * (cb + l)] = (float)v;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace
 
 RecArgs make_args_mixed(const jhtx_plan* p) {
@@ -1346,7 +1468,7 @@ int mixed_adj_chunks(const jhtx_plan* p) {
 }
 
 int mixed_group(const jhtx_plan* p, bool adjoint) {
-  if (p->spin == 0) return (!adjoint && p->mx_synth_group >= 16) ? 16 : 8;
+  if (p->spin == 0) return ((adjoint ? p->mx_adj_group : p->mx_synth_group) >= 16) ? 16 : 8;
   return adjoint ? p->mx_adj_group : p->mx_synth_group;
 }
 
@@ -1383,7 +1505,9 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, true);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 0)
+  if (p->spin == 0 && MG == 16)
+    hipLaunchKernelGGL(k_adj_mx_s0_cb, grid, dim3(256), 0, s, a, V, part, batch);
+  else if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16 && mixed_adj_waves(p) == 8)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);

@@ -256,26 +256,32 @@ def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
 
 @pytest.mark.parametrize("group", [16, 8])
 @pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
-def test_mixed_spin0_synthesis_groups(nside, lmax, group, monkeypatch):
-    """Spin-0 mixed synthesis with 16 maps per recursion (the 16x16x4 tile kernel) and 8
-    (the rank-1 kernel), batch 18 (a ragged second group), against the FP64 path."""
+def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
+    """Spin-0 mixed synthesis and adjoint with 16 maps per recursion (the 16x16x4 tile
+    synthesis, the column-block adjoint) and 8 (the rank-1 synthesis, the per-wave-tile
+    adjoint), batch 18 (a ragged second group), against the FP64 path."""
     j = _jht()
     from jht_amd import healpix as H
 
     monkeypatch.setenv("JHTX_MX_SYNTH_GROUP", str(group))
+    monkeypatch.setenv("JHTX_MX_ADJ_GROUP", str(group))
     H.clear_plans()
     try:
         batch = 18
         rng = np.random.default_rng(90 + group + nside)
         a = np.stack([O.gaussian_alm(lmax, rng, 0) for _ in range(batch)])
         at = torch.tensor(a, device="cuda")
-        s64 = j.synthesis_batch(at, nside, lmax, 0, precision="f64").cpu().numpy()
-        smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed").cpu().numpy()
+        s64 = j.synthesis_batch(at, nside, lmax, 0, precision="f64")
+        smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed")
+        b64 = j.adjoint_synthesis_batch(s64, nside, lmax, 0, precision="f64")
+        bmx = j.adjoint_synthesis_batch(s64, nside, lmax, 0, precision="mixed")
     finally:
         H.clear_plans()
+    s64, smx, b64, bmx = (x.cpu().numpy() for x in (s64, smx, b64, bmx))
     es = max(_rel(smx[i], s64[i]) for i in range(batch))
-    print(f"mixed spin-0 synthesis group {group} nside={nside} lmax={lmax}: {es:.2e}", flush=True)
-    assert es <= MIXED_TOL
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"mixed spin-0 group {group} nside={nside} lmax={lmax}: synth {es:.2e} adj {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
 def test_mixed_adjoint_column_block_bitwise(monkeypatch):
