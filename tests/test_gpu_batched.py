@@ -75,7 +75,7 @@ def test_graph_ragged_mixed_batch():
             assert worst_s <= 1e-5 and worst_a <= 1e-5
 
 
-@pytest.mark.parametrize("precision,spin", [("f64", 0), ("f64", 2), ("mixed", 2)])
+@pytest.mark.parametrize("precision,spin", [("f64", 0), ("f64", 2), ("mixed", 0), ("mixed", 2)])
 def test_autotune_choice_is_exact(precision, spin, tmp_path):
     import jht_amd
     from jht_amd import healpix as H
