@@ -284,6 +284,40 @@ def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
     assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("nside,lmax", [(1, 2), (2, 5), (4, 8), (8, 23)])
+def test_mixed_16map_kernels_tiny_grids(nside, lmax, spin):
+    """The 16-map mixed kernels (tile syntheses, column-block adjoints) on grids where a
+    work-group holds only a few valid rings (T = 2 .. 16), batch 17 (one full group and a
+    one-map ragged group), against the FP64 path."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    H.clear_plans()
+    try:
+        batch = 17
+        rng = np.random.default_rng(300 + 10 * nside + spin)
+        if spin:
+            a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                          for _ in range(batch)])
+        else:
+            a = np.stack([O.gaussian_alm(lmax, rng, 0) for _ in range(batch)])
+        at = torch.tensor(a, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")  # lmax above the 3 nside - 1 ceiling
+            s64 = j.synthesis_batch(at, nside, lmax, spin, precision="f64")
+            smx = j.synthesis_batch(at, nside, lmax, spin, precision="mixed")
+            b64 = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="f64")
+            bmx = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="mixed")
+    finally:
+        H.clear_plans()
+    s64, smx, b64, bmx = (x.cpu().numpy() for x in (s64, smx, b64, bmx))
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"16-map kernels nside={nside} lmax={lmax} spin={spin}: synth {es:.2e} adj {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
+
+
 def test_mixed_adjoint_column_block_bitwise(monkeypatch):
     """The column-block 16-map adjoint keeps the arithmetic of the cross-wave form (FP32
     over each 64-ring block, FP64 over the blocks in order): identical bits, ragged batch."""
