@@ -59,8 +59,9 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *   synth_rings, adj_rings   rings per lane of the FP64 recursion kernels: spin 0 {1,4,8},
  *                            spin 2 {1,2,4}
  *   block                    FP64 recursion work-group size cap {64,128,192,256}
- *   mx_synth_group           maps per recursion, mixed synthesis {4,8,16}: spin 2 uses
- *                            the value; spin 0 runs 16 (tile kernel) for 16, else 8
+ *   mx_synth_group           maps per recursion, mixed synthesis {4,8,16} (spin 0 also
+ *                            32): spin 2 uses the value; spin 0 runs tile kernels for 16
+ *                            and 32, else 8
  *   mx_synth_occ             its register budget in work-groups per CU {1,2,3} (spin 2)
  *   mx_adj_group, mx_adj_occ the same for the mixed adjoint {4,8,16}, {1,2}
  *   mx_adj_wide              mixed spin-2 adjoint with 512-ring work-groups {1 on, 2 off}

@@ -306,9 +306,12 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   };
   // maps per recursion follow the batch the plan is sized for (a partly empty group costs
   // matrix-core time, a larger one amortises the FP64 recursion over more maps)
+  // (spin 0 can also run 32 maps per recursion, by tuning: at 2 waves per SIMD instead of 3
+  // it measured +3 % at nside 2048 / batch 32 and -2 % on the ragged nside-512 batch)
   p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4), 4, 8);
   if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
     if (atoi(v) == 16) p->mx_synth_group = 16;  // the 16x16x4 tile kernel
+    if (atoi(v) == 32 && p->spin == 0) p->mx_synth_group = 32;
   }
   p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
   if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
@@ -340,7 +343,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.synth_rings == 0 || (p->spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
         (t.adj_rings == 0 || (p->spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
         (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
-        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16})) &&
+        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16}) || (p->spin == 0 && t.mx_synth_group == 32)) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&

@@ -615,47 +615,50 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
 }
 
 // ---------------------------------------------------------------------------------- //
-// spin 0 synthesis on 16x16x4 tiles, 16 maps per work-group (grid.y = map group)
+// spin 0 synthesis on 16x16x4 tiles, 16 or 32 maps per work-group (grid.y = map group)
 // ---------------------------------------------------------------------------------- //
 // Lane = ring: a wave runs the recursion of 64 rings.  Every 8 l-steps the wave's FP32
 // values go to an LDS tile [step][lane]; v_mfma_f32_16x16x4f32 contracts 4 same-parity
-// steps (K) of 16 rings (M) against 8 maps x (re, im) (N): 2 parities x 4 row blocks x 2
-// column blocks = 16 instructions per 8 steps for 64 rings x 16 maps, a quarter of the
-// instructions of the rank-1 4x4x1 form and the recursion shared by twice the maps.
+// steps (K) of 16 rings (M) against 8 maps x (re, im) (N): 2 parities x 4 row blocks x NCB
+// column blocks = 8 NCB instructions per 8 steps for 64 rings x 8 NCB maps, a quarter of
+// the instructions of the rank-1 4x4x1 form, with the recursion shared by 16 or 32 maps.
 // Same software pipeline as k_synth_tile_s2: the matrix instructions of group j and the
 // recursion of group j + 1 (into the other tile buffer) form one straight-line block.
-constexpr int kS0AS = 40;  // s_a row stride (floats): B reads at rows 2k+p, bank 16k+j
 
-// G_l a_lm of 16 maps for l0..l0+CH-1: thread t takes l = l0 + t % CH and maps t / CH + 4u
+// G_l a_lm of MG maps for l0..l0+CH-1: thread t takes l = l0 + t % CH and maps t / CH + 4u
 // (a wave reads 64 consecutive coefficients of one column), all loads issued at once
+template <int MG>
 __device__ __forceinline__ void load_alm_s0(const RecArgs& A, const double2* __restrict__ alm, long long cb, int l0,
-                                            int lmax, int mg, int batch, int tid, double2 (&a)[4], double& g) {
+                                            int lmax, int mg, int batch, int tid, double2 (&a)[MG / 4], double& g) {
   const int j = tid % CH, b0 = tid / CH;
   const int l = l0 + j;
   const bool lok = l <= lmax;
   g = lok ? A.G[cb + l] : 0.0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int b = mg * 16 + b0 + 4 * u;
+  for (int u = 0; u < MG / 4; ++u) {
+    const int b = mg * MG + b0 + 4 * u;
     a[u] = (lok && b < batch) ? alm[(long long)b * A.K + cb + l] : make_double2(0.0, 0.0);
   }
 }
 
-// s_a[j * kS0AS + 2 map + (re, im)]
-__device__ __forceinline__ void store_alm_s0(const double2 (&a)[4], double g, int tid, float* s_a) {
+// s_a[j * AS + 2 map + (re, im)]
+template <int MG, int AS>
+__device__ __forceinline__ void store_alm_s0(const double2 (&a)[MG / 4], double g, int tid, float* s_a) {
   const int j = tid % CH, b0 = tid / CH;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    float* d = s_a + j * kS0AS + 2 * (b0 + 4 * u);
+  for (int u = 0; u < MG / 4; ++u) {
+    float* d = s_a + j * AS + 2 * (b0 + 4 * u);
     d[0] = (float)(a[u].x * g);
     d[1] = (float)(a[u].y * g);
   }
 }
 
-template <int MINB>
+// NCB column blocks of 8 maps: 2 (16 maps, 64 accumulator registers) or 4 (32 maps, 128)
+template <int NCB, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const double2* __restrict__ alm,
                                                              float2* __restrict__ F, int batch) {
-  constexpr int MG = 16, AS = kS0AS, RINGS = 256;
+  constexpr int MG = 8 * NCB, AS = 16 * NCB + 8, RINGS = 256;
+  static_assert(AS % 64 == 8 || AS % 64 == 40, "B reads at rows 2k+p land on banks 16k+j");
   constexpr int EPS = 17;  // epilogue staging row stride (floats)
   constexpr int LOOP_FLOATS = CH * AS + 4 * 2 * 8 * kTileTS;  // s_a + double-buffered wave tiles
   constexpr int EPI_FLOATS = 4 * 2 * 64 * EPS;                // per wave [parity][ring][column]
@@ -681,30 +684,30 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
     seed_split(L, (m & 1) ? -1.0 : 1.0, sv, ex);
     mup = -sv;
   }
-  floatx4 D[2][4][2];  // [parity][row block][column block]
+  floatx4 D[2][4][NCB];  // [parity][row block][column block]
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-      for (int c = 0; c < 2; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int c = 0; c < NCB; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
   float* const tile0 = tiles + wave * 2 * 8 * kTileTS;
   float* const tile1 = tile0 + 8 * kTileTS;
   // the next chunk's coefficients are loaded into registers while this chunk computes
-  double2 pf_a[4];
+  double2 pf_a[MG / 4];
   double pf_g = 0.0, pf_al = 0.0;
   auto prefetch = [&](int l0n) {
     if (tid < CH) {
       const int l = l0n + tid;
       pf_al = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
     }
-    load_alm_s0(A, alm, cb, l0n, lmax, mg, batch, tid, pf_a, pf_g);
+    load_alm_s0<MG>(A, alm, cb, l0n, lmax, mg, batch, tid, pf_a, pf_g);
   };
   prefetch(m);
   for (int l0 = m; l0 <= lmax; l0 += CH) {
     __syncthreads();
     if (tid < CH) s_al[tid] = pf_al;
-    store_alm_s0(pf_a, pf_g, tid, s_a);
+    store_alm_s0<MG, AS>(pf_a, pf_g, tid, s_a);
     __syncthreads();
     if (l0 + CH <= lmax) prefetch(l0 + CH);
     const int nst = min(CH, (lmax - l0 + 1 + 7) & ~7);
@@ -741,14 +744,14 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
     auto mfma_group = [&](int jg, const float* tl) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        float bv[2];
+        float bv[NCB];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) bv[c] = s_a[(jg + 2 * kk + p) * AS + 16 * c + ii];
+        for (int c = 0; c < NCB; ++c) bv[c] = s_a[(jg + 2 * kk + p) * AS + 16 * c + ii];
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) {
           const float av = tl[(2 * kk + p) * kTileTS + 16 * rb + ii];
 #pragma unroll
-          for (int c = 0; c < 2; ++c)
+          for (int c = 0; c < NCB; ++c)
             D[p][rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], D[p][rb][c], 0, 0, 0);
         }
       }
@@ -773,7 +776,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
   float* ep = s_buf + wave * 2 * 64 * EPS;  // [parity][ring][column]
   const long long ringM = A.M;
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < NCB; ++c) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -1468,7 +1471,10 @@ int mixed_adj_chunks(const jhtx_plan* p) {
 }
 
 int mixed_group(const jhtx_plan* p, bool adjoint) {
-  if (p->spin == 0) return ((adjoint ? p->mx_adj_group : p->mx_synth_group) >= 16) ? 16 : 8;
+  if (p->spin == 0) {
+    const int g = adjoint ? p->mx_adj_group : p->mx_synth_group;
+    return (!adjoint && g >= 32) ? 32 : (g >= 16 ? 16 : 8);
+  }
   return adjoint ? p->mx_adj_group : p->mx_synth_group;
 }
 
@@ -1483,8 +1489,10 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     at.nchunk = (p->T + 127) / 128;
     dim3 gt(Mpad * at.nchunk, (batch + 15) / 16);
     hipLaunchKernelGGL((k_synth_tile_s2<4, 2>), gt, dim3(256), 0, s, at, alm, F, batch);
-  } else if (p->spin == 0 && MG == 16)
-    hipLaunchKernelGGL((k_synth_tile_s0<2>), grid, dim3(256), 0, s, a, alm, F, batch);
+  } else if (p->spin == 0 && MG == 32)
+    hipLaunchKernelGGL((k_synth_tile_s0<4, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
+  else if (p->spin == 0 && MG == 16)
+    hipLaunchKernelGGL((k_synth_tile_s0<2, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (MG == 8 && p->mx_synth_occ == 3)

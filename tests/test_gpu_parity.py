@@ -254,12 +254,13 @@ def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
     assert ea <= MIXED_TOL
 
 
-@pytest.mark.parametrize("group", [16, 8])
+@pytest.mark.parametrize("group", [32, 16, 8])
 @pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
 def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
-    """Spin-0 mixed synthesis and adjoint with 16 maps per recursion (the 16x16x4 tile
-    synthesis, the column-block adjoint) and 8 (the rank-1 synthesis, the per-wave-tile
-    adjoint), batch 18 (a ragged second group), against the FP64 path."""
+    """Spin-0 mixed synthesis and adjoint with 32 maps per recursion (the tile synthesis
+    with 4 column blocks; the adjoint takes 16), 16 (the 16x16x4 tile synthesis, the
+    column-block adjoint) and 8 (the rank-1 synthesis, the per-wave-tile adjoint), batch 18
+    (a ragged group), against the FP64 path."""
     j = _jht()
     from jht_amd import healpix as H
 
