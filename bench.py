@@ -341,7 +341,7 @@ def main():
         flops, peak, bound = tri * (16 if spin else 4) * B, PEAK_FP32_MFMA_TFLOPS, "mfma"
     achieved = flops / (per[dom][0] * 1e-3) / 1e12
     out = {
-        "metric": "maps/s synth+adjoint nside=2048 lmax=4096 (spin 2): mixed vs FP64 GPU at rel err <= 1e-5",
+        "metric": f"maps/s synth+adjoint nside={nside} lmax={lmax} (spin {spin}): mixed vs FP64 GPU at rel err <= 1e-5",
         "value": round(value, 4),
         "unit": "maps/s",
         "n_gpus": 1,
