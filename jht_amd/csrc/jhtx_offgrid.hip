@@ -351,6 +351,8 @@ __global__ __launch_bounds__(256) void k_interp_tiled(const double2* __restrict_
     es_weights<W, DERIV>(loc[2 * j + 1] * (nm / kTwoPi), beta, bm, wm, dwm);
     const int lr = wrap(bk, nk) - row0, lc = wrap(bm, nm) - col0;
     double2 f = make_double2(0.0, 0.0), ft = f, fp = f;
+    // bounded unroll: a full unroll of the stencil rows hoists all W x W cell loads
+#pragma unroll 2
     for (int o1 = 0; o1 < W; ++o1) {
       const double2* row = s_g + (lr + o1) * E + lc;
       double2 sr = make_double2(0.0, 0.0), sd = sr;
