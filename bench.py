@@ -12,8 +12,9 @@ and the mixed outputs of the first map are compared with the FP64 GPU outputs
                     [--nside N --lmax L --spin S] [--no-cpu] [--no-f64]
                     [--offgrid NPTS | --c4]
 
-Roofline: the dominant recursion + contraction kernel (synthesis or adjoint) is timed live
-with HIP events recorded by the library on the call's stream.  Work unit: one (l, m,
+Roofline: the dominant recursion + contraction kernel (synthesis or adjoint; the adjoint's
+chunk reduction is the separate stage adj_finish) is timed live with HIP events recorded by
+the library on the call's stream around that kernel alone.  Work unit: one (l, m,
 north-half ring) triple of the active triangle l >= max(m, |s|).  FP64 path: 26 flops per
 triple per map (spin 2; 8 for spin 0) against the 78.6 TFLOP/s FP64 peak.  Mixed path: the
 contraction that runs on the FP32 matrix cores, 16 flops per triple per map (spin 2; 4 for
@@ -36,11 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-# HBM bytes per call of the dominant stage, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of
+# HBM bytes per call of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of
 # this script (each counter in its own run, scaled by the counters of a 1 GiB device copy:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax, spin, batch, stage)
 MEASURED_TRAFFIC = {
-    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 99.4e9,
+    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 57.1e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0

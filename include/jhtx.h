@@ -110,9 +110,10 @@ const char* jhtx_last_error(void);
 /* Stage timing (no counterpart in the reference; for benchmarks and profilers): when
  * enabled, every call brackets each kernel stage with HIP events on the call's stream.
  * Stages: 0 synthesis recursion, 1 synthesis ring FFT, 2 adjoint ring FFT,
- * 3 adjoint recursion (+ chunk reduction).  jhtx_plan_read_profile synchronises the
- * recorded events, writes the summed milliseconds and launch counts per stage (arrays of
- * 4) and clears them.  Up to 4096 calls are recorded between reads. */
+ * 3 adjoint recursion, 4 adjoint chunk reduction (the sum of the recursion's ring-chunk
+ * partials).  jhtx_plan_read_profile synchronises the recorded events, writes the summed
+ * milliseconds and launch counts per stage (arrays of 5) and clears them.  Up to 4096
+ * calls are recorded between reads. */
 int jhtx_plan_set_profiling(jhtx_plan* plan, int enable);
 int jhtx_plan_read_profile(jhtx_plan* plan, double* ms, int* counts);
 

@@ -229,6 +229,8 @@ struct Profile {
   }
 };
 
+constexpr int kStages = 5;  // profiled stages (include/jhtx.h)
+
 // Bracket one stage with events when profiling is on.
 struct StageTimer {
   Profile* pr;
@@ -236,7 +238,7 @@ struct StageTimer {
   int stage;
   hipEvent_t a = nullptr;
   StageTimer(const jhtx_plan* p, hipStream_t s_, int st) : pr(nullptr), s(s_), stage(st) {
-    if (p->profiling && p->prof && ((Profile*)p->prof)->recs.size() < 4 * 4096) {
+    if (p->profiling && p->prof && ((Profile*)p->prof)->recs.size() < kStages * 4096) {
       pr = (Profile*)p->prof;
       a = pr->get();
       (void)hipEventRecord(a, s);
@@ -252,6 +254,35 @@ struct StageTimer {
 };
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// The adjoint recursion kernel (stage 3), then the sum of its ring-chunk partials into
+// alm (stage 4).  F: ring coefficients of the precision's type; part: the workspace's
+// partials.
+hipError_t launch_adjoint_rec_finish(const jhtx_plan* p, const void* F, void* part, void* alm, int batch,
+                                     hipStream_t s) {
+  hipError_t e;
+  double2* out = reinterpret_cast<double2*>(alm);
+  if (p->precision == JHTX_PREC_MIXED) {
+    float2* part32 = reinterpret_cast<float2*>(part);
+    StageTimer t3(p, s, 3);
+    e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, batch, s);
+    t3.done();
+    if (e != hipSuccess) return e;
+    StageTimer t4(p, s, 4);
+    e = launch_adj_finish(p, part32, out, batch, mixed_adj_chunks(p), s);
+    t4.done();
+  } else {
+    double2* part64 = reinterpret_cast<double2*>(part);
+    StageTimer t3(p, s, 3);
+    e = launch_adj_rec(p, reinterpret_cast<const double2*>(F), part64, batch, s);
+    t3.done();
+    if (e != hipSuccess) return e;
+    StageTimer t4(p, s, 4);
+    e = launch_adj_finish(p, part64, out, batch, adj_rec_chunks(p), s);
+    t4.done();
+  }
+  return e;
+}
 
 int check_call(const jhtx_plan* p, const void* in, void* out, int batch, void* ws) {
   if (!p) {
@@ -553,18 +584,7 @@ int jhtx_legendre_adjoint(jhtx_plan* p, const void* F, void* alm, int batch, voi
   if (rc) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   void* part = reinterpret_cast<char*>(ws) + p->ws_F_bytes;
-  StageTimer t3(p, s, 3);
-  hipError_t e;
-  if (p->precision == JHTX_PREC_MIXED) {
-    float2* part32 = reinterpret_cast<float2*>(part);
-    e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, batch, s);
-    if (e == hipSuccess)
-      e = launch_adj_finish(p, part32, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
-  } else {
-    e = launch_adj_rec(p, reinterpret_cast<const double2*>(F), reinterpret_cast<double2*>(part),
-                       reinterpret_cast<double2*>(alm), batch, s);
-  }
-  t3.done();
+  hipError_t e = launch_adjoint_rec_finish(p, F, part, alm, batch, s);
   if (e != hipSuccess) {
     set_error(std::string("Legendre adjoint launch failed: ") + hipGetErrorString(e));
     return JHTX_EDEVICE;
@@ -614,16 +634,7 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   hipError_t e = launch_ring_adjoint(p, reinterpret_cast<const double*>(map), V, batch, s);
   t2.done();
   if (e == hipSuccess) {
-    StageTimer t3(p, s, 3);
-    if (p->precision == JHTX_PREC_MIXED) {
-      float2* part32 = reinterpret_cast<float2*>(part);
-      e = launch_adj_mixed(p, reinterpret_cast<const float2*>(V), part32, batch, s);
-      if (e == hipSuccess)
-        e = launch_adj_finish(p, part32, reinterpret_cast<double2*>(alm), batch, mixed_adj_chunks(p), s);
-    } else {
-      e = launch_adj_rec(p, V, part, reinterpret_cast<double2*>(alm), batch, s);
-    }
-    t3.done();
+    e = launch_adjoint_rec_finish(p, V, part, alm, batch, s);
   }
   if (e != hipSuccess) {
     set_error(std::string("adjoint launch failed: ") + hipGetErrorString(e));
@@ -668,7 +679,7 @@ int jhtx_plan_read_profile(jhtx_plan* p, double* ms, int* counts) {
     set_error("null argument");
     return JHTX_EINVAL;
   }
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < kStages; ++i) {
     ms[i] = 0.0;
     counts[i] = 0;
   }

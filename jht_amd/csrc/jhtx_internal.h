@@ -123,8 +123,8 @@ void set_error(const std::string& msg);
 // launchers (return hipError_t)
 hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                             hipStream_t s);
-hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
-                          int batch, hipStream_t s);
+hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s);
+int adj_rec_chunks(const jhtx_plan* p);
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);
 hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,

@@ -969,8 +969,9 @@ hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, 
   return hipGetLastError();
 }
 
-hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, double2* alm,
-                          int batch, hipStream_t s) {
+// the recursion + contraction kernel of the FP64 adjoint; its chunk partials are summed by
+// launch_adj_finish (adj_rec_chunks chunks)
+hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s) {
   const RecArgs a = make_args(p, true);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   dim3 grid(Mpad * p->adj_nchunk, batch);
@@ -985,9 +986,6 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
     RecArgs am = a;
     am.nchunk = (p->T + 255) / 256;
     hipLaunchKernelGGL(k_adj_s2_mfma, dim3(Mpad * am.nchunk, batch), dim3(256), 0, s, am, V, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_adj_finish(p, part, alm, batch, am.nchunk, s);
   } else {
     switch (p->adj_R) {
       case 4: hipLaunchKernelGGL(k_adj_s2<4>, grid, block, 0, s, a, V, part); break;
@@ -995,9 +993,12 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, d
       default: hipLaunchKernelGGL(k_adj_s2<1>, grid, block, 0, s, a, V, part); break;
     }
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return launch_adj_finish(p, part, alm, batch, p->adj_nchunk, s);
+  return hipGetLastError();
+}
+
+// ring chunks (partials per map and component) of the FP64 adjoint
+int adj_rec_chunks(const jhtx_plan* p) {
+  return (p->spin == 2 && p->f64_adj_mfma) ? (p->T + 255) / 256 : p->adj_nchunk;
 }
 
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,

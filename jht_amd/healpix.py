@@ -175,15 +175,15 @@ class Plan:
         _lib.check(self._lib.jhtx_plan_get_tuning(self._h, _lib.ctypes.byref(t)))
         return {k: getattr(t, k) for k in _lib.TUNING_FIELDS}
 
-    STAGES = ("synth_rec", "synth_fft", "adj_fft", "adj_rec")
+    STAGES = ("synth_rec", "synth_fft", "adj_fft", "adj_rec", "adj_finish")
 
     def set_profiling(self, on: bool) -> None:
         _lib.check(self._lib.jhtx_plan_set_profiling(self._h, 1 if on else 0))
 
     def read_profile(self) -> dict:
         """{stage: (summed ms, launches)} of the calls since the last read (HIP events)."""
-        ms = (_lib.ctypes.c_double * 4)()
-        cnt = (_lib.ctypes.c_int * 4)()
+        ms = (_lib.ctypes.c_double * len(self.STAGES))()
+        cnt = (_lib.ctypes.c_int * len(self.STAGES))()
         _lib.check(self._lib.jhtx_plan_read_profile(self._h, ms, cnt))
         return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
 

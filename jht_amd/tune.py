@@ -53,7 +53,7 @@ def _time_plan(plan: Plan, a, maps, back, batch, steps):
         plan.adjoint(maps, back, batch)
     prof = plan.read_profile()
     plan.set_profiling(False)
-    return prof["synth_rec"][0] / steps, prof["adj_rec"][0] / steps
+    return prof["synth_rec"][0] / steps, (prof["adj_rec"][0] + prof["adj_finish"][0]) / steps
 
 
 def autotune(nside: int, lmax: int, spin: int = 0, batch: int = 1, precision: str = "f64",
