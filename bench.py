@@ -10,7 +10,7 @@ and the mixed outputs of the first map are compared with the FP64 GPU outputs
 
     python bench.py [--steps K] [--warmup W] [--precision f64|mixed] [--batch B]
                     [--nside N --lmax L --spin S] [--no-cpu] [--no-f64]
-                    [--offgrid NPTS | --c4]
+                    [--offgrid NPTS | --c4 | --peaks]
 
 Roofline: the dominant recursion + contraction kernel (synthesis or adjoint; the adjoint's
 chunk reduction is the separate stage adj_finish) is timed live with HIP events recorded by
@@ -171,6 +171,38 @@ def bench_offgrid(args):
     }), flush=True)
 
 
+def bench_peaks(args):
+    """Measured device rates beside the published peaks the roofline uses: FP32 matrix
+    (v_mfma_f32_16x16x4f32 loops), FP64 vector FMA and device-to-device copy (jhtx_probe_peaks);
+    best of --steps runs."""
+    import ctypes
+
+    import torch
+
+    from jht_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    buf = torch.empty(1 << 29, dtype=torch.uint8, device=dev)
+    best = [0.0, 0.0, 0.0]
+    for _ in range(max(1, args.steps)):
+        out = (ctypes.c_double * 3)()
+        _lib.check(lib.jhtx_probe_peaks(ctypes.c_void_p(buf.data_ptr()), buf.numel(), out,
+                                        ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)))
+        best = [max(b, o) for b, o in zip(best, out)]
+    print(json.dumps({
+        "metric": "measured device rates (best of runs) against the published MI355X peaks",
+        "steps": args.steps,
+        "fp32_mfma_tflops": {"measured": round(best[0], 2), "published": PEAK_FP32_MFMA_TFLOPS,
+                             "frac": round(best[0] / PEAK_FP32_MFMA_TFLOPS, 4)},
+        "fp64_fma_tflops": {"measured": round(best[1], 2), "published": PEAK_FP64_TFLOPS,
+                            "frac": round(best[1] / PEAK_FP64_TFLOPS, 4)},
+        "hbm_copy_gbs": {"measured": round(best[2], 1), "published": HBM_PEAK_GBS,
+                         "frac": round(best[2] / HBM_PEAK_GBS, 4)},
+    }), flush=True)
+
+
 C4_INSTANCES = [(512, 1024, 0, 64), (128, 256, 0, 6), (256, 512, 0, 5), (512, 1024, 0, 5)]
 
 
@@ -249,6 +281,10 @@ def main():
                     help="instead of the headline transform pair, time the ragged batch (64 spin-0 "
                          "maps at nside 512 / lmax 1024 + 16 over nside 128/256/512) captured in "
                          "one graph, against the unbatched FP64 path, and print its own JSON line")
+    ap.add_argument("--peaks", action="store_true",
+                    help="instead of the headline transform pair, measure the device's FP32 matrix, "
+                         "FP64 FMA and copy rates (the roofline denominators) and print them beside "
+                         "the published peaks")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -276,6 +312,8 @@ def main():
         return bench_offgrid(args)
     if args.c4:
         return bench_c4(args)
+    if args.peaks:
+        return bench_peaks(args)
 
     nside, lmax, spin, B = args.nside, args.lmax, args.spin, args.batch
     prec = _lib.PREC_MIXED if args.precision == "mixed" else _lib.PREC_F64

@@ -211,6 +211,12 @@ int jhtx_nufft_spread(const double* re, const double* im, const double* loc, lon
                       int nk, int nm, int W, double beta, void* workspace, size_t workspace_bytes,
                       void* stream);
 
+/* Measured device rates for roofline denominators (no counterpart in the reference; for
+ * benchmarks): out[0] FP32 matrix TFLOP/s (v_mfma_f32_16x16x4f32 loops), out[1] FP64
+ * vector-FMA TFLOP/s, out[2] device-to-device copy GB/s (bytes read + written).  buf is
+ * device scratch of at least 512 MiB, overwritten; the call synchronises the stream. */
+int jhtx_probe_peaks(void* buf, size_t bytes, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ EXPORTS = (
     "jhtx_nufft_interp",
     "jhtx_nufft_spread",
     "jhtx_nufft_workspace_bytes",
+    "jhtx_probe_peaks",
 )
 
 JHTX_OK = 0
@@ -117,6 +118,7 @@ def load() -> ctypes.CDLL:
         "jhtx_nufft_grid_gather": [vp, i, i, vp, vp, vp, i, i, vp],
         "jhtx_nufft_interp": [vp, i, i, i, d, vp, ll, vp, vp, vp, vp, ctypes.c_size_t, vp],
         "jhtx_nufft_spread": [vp, vp, vp, ll, vp, i, i, i, d, vp, ctypes.c_size_t, vp],
+        "jhtx_probe_peaks": [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["jhtx_api.hip", "jhtx_rec.hip", "jhtx_fft.hip", "jhtx_mixed.hip", "jhtx_offgrid.hip"]
+SOURCES = ["jhtx_api.hip", "jhtx_rec.hip", "jhtx_fft.hip", "jhtx_mixed.hip", "jhtx_offgrid.hip", "jhtx_peaks.hip"]
 OUT = os.path.join(HERE, "libjhtx.so")
 
 FLAGS = [
