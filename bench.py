@@ -330,7 +330,11 @@ def main():
     mshape = (B, npix) if spin == 0 else (B, 2, npix)
     maps = torch.empty(mshape, dtype=torch.float64, device=dev)
     back = torch.empty_like(alm)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter()
     plan = get_plan(nside, lmax, spin, B, precision=prec)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t_plan
     dt, per = run(plan, alm, maps, back, B, args.steps, args.warmup)
     value = B * args.steps / dt
 
@@ -401,6 +405,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((args.precision, nside, lmax, spin, B, dom))},
         "stages_ms": {k: round(v[0], 3) for k, v in per.items()},
         "adjoint_identity": {"lhs": lhs, "rhs": rhs, "rel": ident},
+        "plan_create_s": round(t_plan, 3),
     }
     out.update(out_extra)
     if not args.no_cpu:
