@@ -109,7 +109,7 @@ __device__ __forceinline__ void rescale(double (&mu)[R], double (&mup)[R], int (
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_synth_s0(RecArgs A, const double2* __restrict__ alm,
+__global__ __launch_bounds__(256, 2) void k_synth_s0(RecArgs A, const double2* __restrict__ alm,
                                                   double2* __restrict__ F) {
   __shared__ double s_al[CH];
   __shared__ double2 s_a[CH];
@@ -453,7 +453,7 @@ __device__ __forceinline__ void s0_adj_block(const double* s_al, int j0, const d
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_adj_s0(RecArgs A, const double2* __restrict__ V,
+__global__ __launch_bounds__(256, 2) void k_adj_s0(RecArgs A, const double2* __restrict__ V,
                                                 double2* __restrict__ part) {
   __shared__ double s_al[CH];
   __shared__ double2 red[kRedSize];
