@@ -18,7 +18,8 @@ the library on the call's stream around that kernel alone.  Work unit: one (l, m
 north-half ring) triple of the active triangle l >= max(m, |s|).  FP64 path: 26 flops per
 triple per map (spin 2; 8 for spin 0) against the 78.6 TFLOP/s FP64 peak.  Mixed path: the
 contraction that runs on the FP32 matrix cores, 16 flops per triple per map (spin 2; 4 for
-spin 0), against the 157.3 TFLOP/s dense FP32 MFMA peak.  CPU baseline: the reference's
+spin 0), against the 157.3 TFLOP/s dense FP32 MFMA peak; a one-map mixed call runs the FP64
+kernels and is priced as the FP64 path.  CPU baseline: the reference's
 fused masked scan restated in numpy (oracle/jht_oracle.py) on a sample of orders m over
 every l and ring, scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs.
 """
@@ -378,7 +379,9 @@ def main():
     ms_step = dt / args.steps * 1e3
     dom = max(("synth_rec", "adj_rec"), key=lambda k: per[k][0])
     tri = triples(nside, lmax, spin)
-    if prec == _lib.PREC_F64:
+    # one-map calls of a mixed plan run the FP64 vector-FMA recursion kernels (mx_single)
+    fp64_kernels = prec == _lib.PREC_F64 or (B == 1 and plan.tuning["mx_single"] == 1)
+    if fp64_kernels:
         flops, peak, bound = tri * (26 if spin else 8) * B, PEAK_FP64_TFLOPS, "mfma"
     else:
         flops, peak, bound = tri * (16 if spin else 4) * B, PEAK_FP32_MFMA_TFLOPS, "mfma"
@@ -394,7 +397,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == _lib.PREC_F64 else "f64 recursion / f32 mfma contraction",
+        "dtype": ("f64" if prec == _lib.PREC_F64 else
+                  "f64 recursion and contraction / f32 ring fft" if fp64_kernels else
+                  "f64 recursion / f32 mfma contraction"),
         "data": "synthetic (seeded Gaussian E/B a_lm, C_l=(l+1)^-2, B=0.1 E, l<2 zero)",
         "config": {"workload": f"spin-{spin} synthesis+adjoint, nside={nside}, lmax={lmax}",
                    "nside": nside, "lmax": lmax, "spin": spin, "batch": B,

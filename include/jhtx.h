@@ -69,13 +69,18 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            {1 on, 2 off (default): the vector-ALU kernel with adj_rings}
  *   mx_adj_cb                16-map mixed spin-2 adjoint in column-block form (each wave
  *                            contracts 4 maps over the work-group's rings; the same bits)
- *                            {1 on (default), 2 off: per-wave tiles and a cross-wave sum} */
+ *                            {1 on (default), 2 off: per-wave tiles and a cross-wave sum}
+ *   mx_single                one-map calls of a mixed plan run the FP64 vector-FMA
+ *                            recursion kernels on complex64 ring coefficients, with the
+ *                            FP32 ring FFT {1 on (default), 2 off: the 4-map matrix-core
+ *                            kernels}.  f64_adj_mfma does not apply to mixed plans. */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
   int mx_adj_group, mx_adj_occ, mx_adj_wide;
   int f64_adj_mfma;
   int mx_adj_cb;
+  int mx_single;
 } jhtx_tuning;
 
 /* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An

@@ -255,6 +255,19 @@ struct StageTimer {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// A one-map call of a mixed plan that runs the FP64 recursion kernels (mx_single).
+bool single_route(const jhtx_plan* p, int batch) {
+  return p->precision == JHTX_PREC_MIXED && p->mx_single && batch == 1;
+}
+
+// The synthesis recursion kernel (stage 0) into F of the precision's type.
+hipError_t launch_synth_stage(const jhtx_plan* p, const void* alm, void* F, int batch, hipStream_t s) {
+  const double2* a = reinterpret_cast<const double2*>(alm);
+  if (p->precision != JHTX_PREC_MIXED) return launch_synth_rec(p, a, reinterpret_cast<double2*>(F), batch, s);
+  if (single_route(p, batch)) return launch_synth_rec(p, a, reinterpret_cast<float2*>(F), batch, s);
+  return launch_synth_mixed(p, a, reinterpret_cast<float2*>(F), batch, s);
+}
+
 // The adjoint recursion kernel (stage 3), then the sum of its ring-chunk partials into
 // alm (stage 4).  F: ring coefficients of the precision's type; part: the workspace's
 // partials.
@@ -262,7 +275,7 @@ hipError_t launch_adjoint_rec_finish(const jhtx_plan* p, const void* F, void* pa
                                      hipStream_t s) {
   hipError_t e;
   double2* out = reinterpret_cast<double2*>(alm);
-  if (p->precision == JHTX_PREC_MIXED) {
+  if (p->precision == JHTX_PREC_MIXED && !single_route(p, batch)) {
     float2* part32 = reinterpret_cast<float2*>(part);
     StageTimer t3(p, s, 3);
     e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, batch, s);
@@ -274,7 +287,8 @@ hipError_t launch_adjoint_rec_finish(const jhtx_plan* p, const void* F, void* pa
   } else {
     double2* part64 = reinterpret_cast<double2*>(part);
     StageTimer t3(p, s, 3);
-    e = launch_adj_rec(p, reinterpret_cast<const double2*>(F), part64, batch, s);
+    e = (p->precision == JHTX_PREC_MIXED) ? launch_adj_rec(p, reinterpret_cast<const float2*>(F), part64, batch, s)
+                                           : launch_adj_rec(p, reinterpret_cast<const double2*>(F), part64, batch, s);
     t3.done();
     if (e != hipSuccess) return e;
     StageTimer t4(p, s, 4);
@@ -358,7 +372,12 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
   // the 16-map adjoint in column-block form (1) or with per-wave tiles summed across waves (0)
   p->mx_adj_cb = pick2("JHTX_MX_ADJ_CB", 1, 0, 1);
-  p->f64_adj_mfma = (p->spin == 2) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
+  p->f64_adj_mfma = (p->spin == 2 && p->precision != JHTX_PREC_MIXED) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
+  // One-map calls of a mixed plan: a matrix-core group of 4 maps would leave 3/4 of its
+  // columns empty, and one 16x16x4 instruction per 64 (ring, l) pairs then costs 4x the
+  // two FP64 FMAs per pair of the vector contraction.  Such calls run the FP64 recursion
+  // kernels on complex64 F / V instead (the ring FFT stays FP32).
+  p->mx_single = pick2("JHTX_MX_SINGLE", 1, 0, 1);
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);
@@ -380,7 +399,8 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
-        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2})) && (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2}));
+        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2})) && (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) &&
+        (t.mx_single == 0 || in(t.mx_single, {1, 2}));
     if (!ok) {
       set_error("jhtx_tuning field out of range");
       return JHTX_EINVAL;
@@ -393,8 +413,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
     if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
     if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
     if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
-    if (t.f64_adj_mfma && p->spin == 2) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
+    if (t.f64_adj_mfma && p->spin == 2 && p->precision != JHTX_PREC_MIXED) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
     if (t.mx_adj_cb) p->mx_adj_cb = (t.mx_adj_cb == 1);
+    if (t.mx_single) p->mx_single = (t.mx_single == 1);
   }
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
@@ -409,11 +430,16 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   p->ws_F_bytes = align256(f_elem * (size_t)p->max_batch * p->ncomp * p->nrings * p->M);
   // the mixed kernels use one ring per lane and 256-lane work-groups
   // the FP64 matrix-core adjoint writes one partial per 256 rings
-  const int part_chunks = (p->precision == JHTX_PREC_MIXED) ? jhtx::mixed_adj_chunks(p)
-                          : p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256)
-                                            : p->adj_nchunk;
-  const size_t part_elem = (p->precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
-  p->ws_part_bytes = align256(part_elem * (size_t)part_chunks * p->max_batch * p->ncomp * p->K);
+  // (a mixed plan's one-map calls through the FP64 kernels keep FP64 partials)
+  size_t part_bytes;
+  if (p->precision == JHTX_PREC_MIXED) {
+    part_bytes = sizeof(float2) * (size_t)jhtx::mixed_adj_chunks(p) * p->max_batch * p->ncomp * p->K;
+    if (p->mx_single) part_bytes = std::max<size_t>(part_bytes, sizeof(double2) * (size_t)p->adj_nchunk * p->ncomp * p->K);
+  } else {
+    const int part_chunks = p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256) : p->adj_nchunk;
+    part_bytes = sizeof(double2) * (size_t)part_chunks * p->max_batch * p->ncomp * p->K;
+  }
+  p->ws_part_bytes = align256(part_bytes);
   p->ws_bytes = p->ws_F_bytes + p->ws_part_bytes;
   return JHTX_OK;
 }
@@ -446,6 +472,7 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->mx_adj_occ = p->mx_adj_occ;
   t->mx_adj_wide = p->mx_adj_wide ? 1 : 2;
   t->f64_adj_mfma = p->f64_adj_mfma ? 1 : 2;
+  t->mx_single = p->mx_single ? 1 : 2;
   t->mx_adj_cb = p->mx_adj_cb ? 1 : 2;
   return JHTX_OK;
 }
@@ -570,9 +597,7 @@ int jhtx_legendre_synth(jhtx_plan* p, const void* alm, void* F, int batch, void*
   if (rc) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   StageTimer t0(p, s, 0);
-  hipError_t e = (p->precision == JHTX_PREC_MIXED)
-                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<float2*>(F), batch, s)
-                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<double2*>(F), batch, s);
+  hipError_t e = launch_synth_stage(p, alm, F, batch, s);
   t0.done();
   if (e != hipSuccess) {
     set_error(std::string("Legendre synthesis launch failed: ") + hipGetErrorString(e));
@@ -606,9 +631,7 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   void* F = ws;
   StageTimer t0(p, s, 0);
-  hipError_t e = (p->precision == JHTX_PREC_MIXED)
-                     ? launch_synth_mixed(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<float2*>(F), batch, s)
-                     : launch_synth_rec(p, reinterpret_cast<const double2*>(alm), reinterpret_cast<double2*>(F), batch, s);
+  hipError_t e = launch_synth_stage(p, alm, F, batch, s);
   t0.done();
   if (e == hipSuccess) {
     StageTimer t1(p, s, 1);

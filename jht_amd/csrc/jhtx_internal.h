@@ -106,13 +106,15 @@ struct jhtx_plan {
   int mx_adj_occ;      // register budget of the 8-map adjoint: 1 or 2 work-groups per CU
   int mx_adj_wide;     // 8-map adjoint with 512-ring work-groups on large grids (0 or 1)
   int mx_adj_cb;       // 16-map adjoint in column-block form (0 or 1)
+  int mx_single;       // mixed plans: one-map calls run the FP64 vector-FMA recursion kernels
+                       // on complex64 F / V (0 or 1)
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)
   int profiling;
   void* prof;  // jhtx::Profile*
   // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2 (FP64) / float2 (mixed)]
-  // [chunk partials: nchunk*batch*ncomp*K double2 (FP64 path) or float2 (mixed path)]
+  // [chunk partials: nchunk*batch*ncomp*K double2 (FP64 kernels) or float2 (mixed kernels)]
   size_t ws_F_bytes, ws_part_bytes, ws_bytes;
 };
 
@@ -123,7 +125,10 @@ void set_error(const std::string& msg);
 // launchers (return hipError_t)
 hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
                             hipStream_t s);
+hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, float2* F, int batch,
+                            hipStream_t s);
 hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s);
+hipError_t launch_adj_rec(const jhtx_plan* p, const float2* V, double2* part, int batch, hipStream_t s);
 int adj_rec_chunks(const jhtx_plan* p);
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);

@@ -22,6 +22,14 @@ namespace jhtx {
 
 namespace {
 
+// Ring coefficients F / V are complex128 in FP64 plans and complex64 in mixed plans (whose
+// ring FFT runs in FP32).  The recursion and its sums stay FP64 in both; only the value
+// stored to or loaded from F / V changes type.
+__device__ __forceinline__ double2 ld_c(const double2& v) { return v; }
+__device__ __forceinline__ double2 ld_c(const float2& v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ void st_c(double2* q, double2 v) { *q = v; }
+__device__ __forceinline__ void st_c(float2* q, double2 v) { *q = make_float2((float)v.x, (float)v.y); }
+
 constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 constexpr double kLog2e = 1.44269504088896338700e+00;
@@ -108,9 +116,9 @@ __device__ __forceinline__ void rescale(double (&mu)[R], double (&mup)[R], int (
   for (int r = 0; r < R; ++r) rescale_pair(mu[r], mup[r], ex[r]);
 }
 
-template <int R>
+template <int R, typename C>
 __global__ __launch_bounds__(256, 2) void k_synth_s0(RecArgs A, const double2* __restrict__ alm,
-                                                  double2* __restrict__ F) {
+                                                  C* __restrict__ F) {
   __shared__ double s_al[CH];
   __shared__ double2 s_a[CH];
   int m, chunk;
@@ -180,13 +188,13 @@ __global__ __launch_bounds__(256, 2) void k_synth_s0(RecArgs A, const double2* _
       }
     }
   }
-  double2* Fb = F + (long long)b * A.nrings * A.M;
+  C* Fb = F + (long long)b * A.nrings * A.M;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int t = tb + r;
     if (t < A.T) {
-      Fb[(long long)t * A.M + m] = c0[r] + c1[r];
-      if (t < A.T - 1) Fb[(long long)(A.nrings - 1 - t) * A.M + m] = c0[r] - c1[r];
+      st_c(&Fb[(long long)t * A.M + m], c0[r] + c1[r]);
+      if (t < A.T - 1) st_c(&Fb[(long long)(A.nrings - 1 - t) * A.M + m], c0[r] - c1[r]);
     }
   }
 }
@@ -245,9 +253,9 @@ __device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, doub
   return L;
 }
 
-template <int R>
+template <int R, typename C>
 __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __restrict__ alm,
-                                                  double2* __restrict__ F) {
+                                                  C* __restrict__ F) {
   __shared__ double2 s_ag[CH];
   __shared__ double2 s_e[CH];
   __shared__ double2 s_b[CH];
@@ -329,8 +337,8 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
   }
   // bp = -(aE + i aB), bm = -(-1)^s (aE - i aB); south values carry (-1)^(l-m)
   const long long ringM = (long long)A.M;
-  double2* Fp = F + (long long)b * 2 * A.nrings * ringM;
-  double2* Fm = Fp + (long long)A.nrings * ringM;
+  C* Fp = F + (long long)b * 2 * A.nrings * ringM;
+  C* Fm = Fp + (long long)A.nrings * ringM;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int t = tb + r;
@@ -344,14 +352,14 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
     const double2 ME = c0[r].me + c1[r].me, MB = c0[r].mb + c1[r].mb;
     // -PE - i PB  and  -ME + i MB
     const double sg = A.ssign;
-    Fp[(long long)t * ringM + m] = make_double2(-PE.x + PB.y, -PE.y - PB.x);
-    Fm[(long long)t * ringM + m] = make_double2(sg * (-ME.x - MB.y), sg * (-ME.y + MB.x));
+    st_c(&Fp[(long long)t * ringM + m], make_double2(-PE.x + PB.y, -PE.y - PB.x));
+    st_c(&Fm[(long long)t * ringM + m], make_double2(sg * (-ME.x - MB.y), sg * (-ME.y + MB.x)));
     if (t < A.T - 1) {
       const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
       const double2 dE = c0[r].me - c1[r].me, dB = c0[r].mb - c1[r].mb;  // from mu-
       const double2 qE = c0[r].pe - c1[r].pe, qB = c0[r].pb - c1[r].pb;  // from mu+
-      Fp[rs] = make_double2(-dE.x + dB.y, -dE.y - dB.x);
-      Fm[rs] = make_double2(sg * (-qE.x - qB.y), sg * (-qE.y + qB.x));
+      st_c(&Fp[rs], make_double2(-dE.x + dB.y, -dE.y - dB.x));
+      st_c(&Fm[rs], make_double2(sg * (-qE.x - qB.y), sg * (-qE.y + qB.x)));
     }
   }
 }
@@ -452,8 +460,8 @@ __device__ __forceinline__ void s0_adj_block(const double* s_al, int j0, const d
   }
 }
 
-template <int R>
-__global__ __launch_bounds__(256, 2) void k_adj_s0(RecArgs A, const double2* __restrict__ V,
+template <int R, typename C>
+__global__ __launch_bounds__(256, 2) void k_adj_s0(RecArgs A, const C* __restrict__ V,
                                                 double2* __restrict__ part) {
   __shared__ double s_al[CH];
   __shared__ double2 red[kRedSize];
@@ -466,7 +474,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s0(RecArgs A, const double2* __r
   const int lmax = A.lmax;
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const long long ringM = A.M;
-  const double2* Vb = V + (long long)b * A.nrings * ringM;
+  const C* Vb = V + (long long)b * A.nrings * ringM;
   double2* pb = part + ((long long)chunk * batch + b) * A.K + cb;
   double x[R], mu[R], mup[R], f[R];
   int ex[R];
@@ -485,9 +493,9 @@ __global__ __launch_bounds__(256, 2) void k_adj_s0(RecArgs A, const double2* __r
       double s;
       seed_split(L, sgn, s, ex[r]);
       mup[r] = -s;
-      const double2 vn = Vb[(long long)t * ringM + m];
+      const double2 vn = ld_c(Vb[(long long)t * ringM + m]);
       double2 vs = make_double2(0.0, 0.0);
-      if (t < A.T - 1) vs = Vb[(long long)(A.nrings - 1 - t) * ringM + m];
+      if (t < A.T - 1) vs = ld_c(Vb[(long long)(A.nrings - 1 - t) * ringM + m]);
       U0[r] = vn + vs;  // (l - m) even
       U1[r] = vn - vs;  // (l - m) odd
     }
@@ -581,8 +589,8 @@ __device__ __forceinline__ void s2_adj_block(const double2* s_ag, int j0, const 
   }
 }
 
-template <int R>
-__global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __restrict__ V,
+template <int R, typename C>
+__global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const C* __restrict__ V,
                                                 double2* __restrict__ part) {
   __shared__ double2 s_ag[CH];
   __shared__ double2 red[kRedSize];
@@ -597,8 +605,8 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
   if (lmin > lmax) return;
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const long long ringM = A.M;
-  const double2* Vp = V + (long long)b * 2 * A.nrings * ringM;
-  const double2* Vm = Vp + (long long)A.nrings * ringM;
+  const C* Vp = V + (long long)b * 2 * A.nrings * ringM;
+  const C* Vm = Vp + (long long)A.nrings * ringM;
   double2* p_p2 = part + (((long long)chunk * batch + b) * 2 + 0) * A.K + cb;
   double2* p_m2 = part + (((long long)chunk * batch + b) * 2 + 1) * A.K + cb;
   double x[R], pp[R], ppp[R], pm[R], pmp[R], fp[R], fm[R];
@@ -620,14 +628,15 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const double2* __rest
       ppp[r] = -s;
       seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em[r]);
       pmp[r] = -s;
-      Vpn[r] = Vp[(long long)t * ringM + m];
-      Vmn[r] = Vm[(long long)t * ringM + m];
+      Vpn[r] = ld_c(Vp[(long long)t * ringM + m]);
+      Vmn[r] = ld_c(Vm[(long long)t * ringM + m]);
       if (t < A.T - 1) {
         // (-1)^(l-m) = (-1)^(l-lmin) (-1)^(lmin-m): fold the second factor into the data
         const double sg = ((lmin - m) & 1) ? -1.0 : 1.0;
         const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
-        Vps[r] = make_double2(sg * Vp[rs].x, sg * Vp[rs].y);
-        Vms[r] = make_double2(sg * Vm[rs].x, sg * Vm[rs].y);
+        const double2 up = ld_c(Vp[rs]), um = ld_c(Vm[rs]);
+        Vps[r] = make_double2(sg * up.x, sg * up.y);
+        Vms[r] = make_double2(sg * um.x, sg * um.y);
       }
     }
   }
@@ -947,53 +956,81 @@ RecArgs make_args(const jhtx_plan* p, bool adjoint) {
 
 }  // namespace
 
-hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
-                            hipStream_t s) {
+namespace {
+
+template <typename C>
+hipError_t synth_rec_t(const jhtx_plan* p, const double2* alm, C* F, int batch, hipStream_t s) {
   const RecArgs a = make_args(p, false);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   dim3 grid(Mpad * p->nchunk, batch);
   dim3 block(p->rec_block);
   if (p->spin == 0) {
     switch (p->rec_R) {
-      case 8: hipLaunchKernelGGL(k_synth_s0<8>, grid, block, 0, s, a, alm, F); break;
-      case 4: hipLaunchKernelGGL(k_synth_s0<4>, grid, block, 0, s, a, alm, F); break;
-      default: hipLaunchKernelGGL(k_synth_s0<1>, grid, block, 0, s, a, alm, F); break;
+      case 8: hipLaunchKernelGGL((k_synth_s0<8, C>), grid, block, 0, s, a, alm, F); break;
+      case 4: hipLaunchKernelGGL((k_synth_s0<4, C>), grid, block, 0, s, a, alm, F); break;
+      default: hipLaunchKernelGGL((k_synth_s0<1, C>), grid, block, 0, s, a, alm, F); break;
     }
   } else {
     switch (p->rec_R) {
-      case 4: hipLaunchKernelGGL(k_synth_s2<4>, grid, block, 0, s, a, alm, F); break;
-      case 2: hipLaunchKernelGGL(k_synth_s2<2>, grid, block, 0, s, a, alm, F); break;
-      default: hipLaunchKernelGGL(k_synth_s2<1>, grid, block, 0, s, a, alm, F); break;
+      case 4: hipLaunchKernelGGL((k_synth_s2<4, C>), grid, block, 0, s, a, alm, F); break;
+      case 2: hipLaunchKernelGGL((k_synth_s2<2, C>), grid, block, 0, s, a, alm, F); break;
+      default: hipLaunchKernelGGL((k_synth_s2<1, C>), grid, block, 0, s, a, alm, F); break;
     }
   }
   return hipGetLastError();
 }
 
-// the recursion + contraction kernel of the FP64 adjoint; its chunk partials are summed by
-// launch_adj_finish (adj_rec_chunks chunks)
-hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s) {
+// the vector-FMA adjoint kernels (the FP64 matrix-core form reads complex128 V only)
+template <typename C>
+hipError_t adj_rec_t(const jhtx_plan* p, const C* V, double2* part, int batch, hipStream_t s) {
   const RecArgs a = make_args(p, true);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   dim3 grid(Mpad * p->adj_nchunk, batch);
   dim3 block(p->adj_block);
   if (p->spin == 0) {
     switch (p->adj_R) {
-      case 8: hipLaunchKernelGGL(k_adj_s0<8>, grid, block, 0, s, a, V, part); break;
-      case 4: hipLaunchKernelGGL(k_adj_s0<4>, grid, block, 0, s, a, V, part); break;
-      default: hipLaunchKernelGGL(k_adj_s0<1>, grid, block, 0, s, a, V, part); break;
+      case 8: hipLaunchKernelGGL((k_adj_s0<8, C>), grid, block, 0, s, a, V, part); break;
+      case 4: hipLaunchKernelGGL((k_adj_s0<4, C>), grid, block, 0, s, a, V, part); break;
+      default: hipLaunchKernelGGL((k_adj_s0<1, C>), grid, block, 0, s, a, V, part); break;
     }
-  } else if (p->f64_adj_mfma) {
-    RecArgs am = a;
-    am.nchunk = (p->T + 255) / 256;
-    hipLaunchKernelGGL(k_adj_s2_mfma, dim3(Mpad * am.nchunk, batch), dim3(256), 0, s, am, V, part);
   } else {
     switch (p->adj_R) {
-      case 4: hipLaunchKernelGGL(k_adj_s2<4>, grid, block, 0, s, a, V, part); break;
-      case 2: hipLaunchKernelGGL(k_adj_s2<2>, grid, block, 0, s, a, V, part); break;
-      default: hipLaunchKernelGGL(k_adj_s2<1>, grid, block, 0, s, a, V, part); break;
+      case 4: hipLaunchKernelGGL((k_adj_s2<4, C>), grid, block, 0, s, a, V, part); break;
+      case 2: hipLaunchKernelGGL((k_adj_s2<2, C>), grid, block, 0, s, a, V, part); break;
+      default: hipLaunchKernelGGL((k_adj_s2<1, C>), grid, block, 0, s, a, V, part); break;
     }
   }
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, double2* F, int batch,
+                            hipStream_t s) {
+  return synth_rec_t(p, alm, F, batch, s);
+}
+
+hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, float2* F, int batch,
+                            hipStream_t s) {
+  return synth_rec_t(p, alm, F, batch, s);
+}
+
+// the recursion + contraction kernel of the FP64 adjoint; its chunk partials are summed by
+// launch_adj_finish (adj_rec_chunks chunks)
+hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s) {
+  if (p->spin == 2 && p->f64_adj_mfma) {
+    RecArgs am = make_args(p, true);
+    am.nchunk = (p->T + 255) / 256;
+    const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
+    hipLaunchKernelGGL(k_adj_s2_mfma, dim3(Mpad * am.nchunk, batch), dim3(256), 0, s, am, V, part);
+    return hipGetLastError();
+  }
+  return adj_rec_t(p, V, part, batch, s);
+}
+
+// complex64 V (a mixed plan's FP32 ring FFT output), FP64 recursion and partials
+hipError_t launch_adj_rec(const jhtx_plan* p, const float2* V, double2* part, int batch, hipStream_t s) {
+  return adj_rec_t(p, V, part, batch, s);
 }
 
 // ring chunks (partials per map and component) of the FP64 adjoint

@@ -75,6 +75,11 @@ def autotune(nside: int, lmax: int, spin: int = 0, batch: int = 1, precision: st
                 set_tuning(nside, lmax, spin, prec, res["tuning"])
             return res
     synth_c, adj_c = _CANDIDATES.get((precision, spin), ([{}], [{}]))
+    if precision == "mixed" and batch == 1:
+        # one-map calls of a mixed plan run the FP64 vector-FMA kernels (mx_single): their
+        # rings per lane are the shapes to time (the FP64 matrix-core adjoint does not apply)
+        synth_c, adj_c = _CANDIDATES[("f64", spin)]
+        adj_c = [c for c in adj_c if c.get("f64_adj_mfma") != 1]
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     a, maps, back = _inputs(nside, lmax, spin, batch, dev)
     t_syn, t_adj = {}, {}

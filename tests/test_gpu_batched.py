@@ -75,13 +75,14 @@ def test_graph_ragged_mixed_batch():
             assert worst_s <= 1e-5 and worst_a <= 1e-5
 
 
-@pytest.mark.parametrize("precision,spin", [("f64", 0), ("f64", 2), ("mixed", 0), ("mixed", 2)])
-def test_autotune_choice_is_exact(precision, spin, tmp_path):
+@pytest.mark.parametrize("precision,spin,B", [("f64", 0, 1), ("f64", 2, 1), ("mixed", 0, 4), ("mixed", 2, 4),
+                                              ("mixed", 2, 1)])
+def test_autotune_choice_is_exact(precision, spin, B, tmp_path):
     import jht_amd
     from jht_amd import healpix as H
     from jht_amd.tune import autotune
 
-    nside, lmax, B = 128, 256, 4 if precision == "mixed" else 1
+    nside, lmax = 128, 256
     cache = str(tmp_path / "tune.json")
     H.clear_plans()
     try:

@@ -193,6 +193,45 @@ def test_mixed_matches_f64(nside, lmax, spin, batch):
         assert _rel(s64[0], O.synthesis(a[0], nside, lmax, spin)) <= TOL
 
 
+@pytest.mark.parametrize("single", [1, 0])
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("nside,lmax", [(16, 32), (64, 128), (256, 400)])
+def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
+    """One-map calls of a mixed plan: the FP64 vector-FMA recursion kernels on complex64
+    ring coefficients with the FP32 ring FFT (single 1, the default) and the 4-map
+    matrix-core kernels (single 0), against the FP64 path and, on the small grids, the
+    oracle."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_SINGLE", str(single))
+    H.clear_plans()
+    try:
+        rng = np.random.default_rng(4100 + nside + spin)
+        a = (O.gaussian_alm(lmax, rng) if spin == 0 else
+             np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)]))
+        at = torch.tensor(a[None], device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, spin, precision="f64")
+            smx = j.synthesis_batch(at, nside, lmax, spin, precision="mixed")
+            b64 = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="f64")
+            bmx = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="mixed")
+            plan = H.get_plan(nside, lmax, spin, 1, H._prec("mixed"))
+            assert plan.tuning["mx_single"] == (1 if single else 2)
+    finally:
+        H.clear_plans()
+    s64, smx, b64, bmx = (t[0].cpu().numpy() for t in (s64, smx, b64, bmx))
+    es, ea = _rel(smx, s64), _rel(bmx, b64)
+    print(f"mixed one-map single={single} nside={nside} lmax={lmax} spin={spin}: synth {es:.2e} adj {ea:.2e}",
+          flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
+    if nside <= 64:
+        v = s64
+        assert _rel(smx, O.synthesis(a, nside, lmax, spin)) <= MIXED_TOL
+        assert _rel(bmx, O.adjoint_synthesis(v, nside, lmax, spin)) <= MIXED_TOL
+
+
 @pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ",
                          [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2),
                           (16, 2, 16, 2), (16, 2, 4, 1)])
