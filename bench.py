@@ -399,6 +399,8 @@ def main():
         "vs_baseline": None,
         "dtype": ("f64" if prec == _lib.PREC_F64 else
                   "f64 recursion and contraction / f32 ring fft" if fp64_kernels else
+                  "f64 recursion / f16 hi+lo split mfma synthesis, f32 mfma adjoint"
+                  if (spin == 2 and B >= 16 and plan.tuning["mx_f16"] == 1) else
                   "f64 recursion / f32 mfma contraction"),
         "data": "synthetic (seeded Gaussian E/B a_lm, C_l=(l+1)^-2, B=0.1 E, l<2 zero)",
         "config": {"workload": f"spin-{spin} synthesis+adjoint, nside={nside}, lmax={lmax}",

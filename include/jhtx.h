@@ -73,7 +73,10 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *   mx_single                one-map calls of a mixed plan run the FP64 vector-FMA
  *                            recursion kernels on complex64 ring coefficients, with the
  *                            FP32 ring FFT {1 on (default), 2 off: the 4-map matrix-core
- *                            kernels}.  f64_adj_mfma does not apply to mixed plans. */
+ *                            kernels}.  f64_adj_mfma does not apply to mixed plans.
+ *   mx_f16                   16-map mixed spin-2 synthesis on the FP16 matrix cores, each
+ *                            FP32 operand split into hi + lo halves after a power-of-two
+ *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles} */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
@@ -81,6 +84,7 @@ typedef struct jhtx_tuning {
   int f64_adj_mfma;
   int mx_adj_cb;
   int mx_single;
+  int mx_f16;
 } jhtx_tuning;
 
 /* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An

@@ -108,6 +108,10 @@ struct jhtx_plan {
   int mx_adj_cb;       // 16-map adjoint in column-block form (0 or 1)
   int mx_single;       // mixed plans: one-map calls run the FP64 vector-FMA recursion kernels
                        // on complex64 F / V (0 or 1)
+  int mx_f16;          // 16-map spin-2 synthesis on the FP16 matrix cores, operands split
+                       // into hi + lo halves (0 or 1)
+  double mx_sA;        // power-of-two scale of the recursion values for the FP16 split
+  double mx_gmax;      // max |G| over the triangle (bounds the staged a_lm G)
   jhtx::RecTables rec;
   jhtx::FftTables fft;
   // stage timing (jhtx_plan_set_profiling)
@@ -115,7 +119,8 @@ struct jhtx_plan {
   void* prof;  // jhtx::Profile*
   // workspace layout (bytes): [F or V: batch*ncomp*nrings*M double2 (FP64) / float2 (mixed)]
   // [chunk partials: nchunk*batch*ncomp*K double2 (FP64 kernels) or float2 (mixed kernels)]
-  size_t ws_F_bytes, ws_part_bytes, ws_bytes;
+  // [scales: kScaleBlocks*batch double partial maxima, then batch float scales (FP16 split)]
+  size_t ws_F_bytes, ws_part_bytes, ws_scale_bytes, ws_bytes;
 };
 
 namespace jhtx {
@@ -135,7 +140,8 @@ hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* a
 hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
-                              hipStream_t s);
+                              void* scale_ws, hipStream_t s);
+constexpr int kScaleBlocks = 256;  // partial maxima per map of the FP16 split's scale pass
 hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int batch,
                             hipStream_t s);
 // ring chunks (partials per map) of the mixed adjoint

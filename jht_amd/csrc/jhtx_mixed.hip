@@ -622,6 +622,280 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
 // steps (K) of 16 rings (M) against 8 maps x (re, im) (N): 2 parities x 4 row blocks x NCB
 // column blocks = 8 NCB instructions per 8 steps for 64 rings x 8 NCB maps, a quarter of
 // the instructions of the rank-1 4x4x1 form, with the recursion shared by 16 or 32 maps.
+// ---------------------------------------------------------------------------------- //
+// FP16 split of the 16-map spin-2 tile synthesis.  Every FP32 operand x is carried as two
+// FP16 halves, hi = fp16(x) and lo = fp16(x - hi) (|x - hi - lo| <= 2^-22 |x|), after a
+// power-of-two scale: sA for the recursion values (from the bound |mu_l| <= sqrt((2l+1)/4pi)
+// / |G_l|), sB[map] for G_l a_lm (from max |a_lm| of the map and max |G|), both putting
+// the operands at or below 2^12.  v_mfma_f32_16x16x32_f16 takes 8 halves per lane; each
+// l step fills 4 of them with the four products hi hi, lo hi, hi lo, lo lo, so one
+// instruction contracts 8 same-parity steps: half the matrix instructions of the FP32
+// 16x16x4 form, at about the same cycles per instruction.  Accumulation stays FP32; the
+// scales are undone in FP64 in the epilogue.
+// ---------------------------------------------------------------------------------- //
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+// Partial maxima of |Re|, |Im| over a map's coefficients: block (x, y) covers map y with
+// stride kScaleBlocks * 256.
+__global__ __launch_bounds__(256) void k_alm_absmax(const double2* __restrict__ alm, long long n_per_map,
+                                                    double* __restrict__ part) {
+  __shared__ double red[256];
+  const double2* a = alm + (long long)blockIdx.y * n_per_map;
+  double mx = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n_per_map; i += (long long)kScaleBlocks * 256) {
+    const double2 v = a[i];
+    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * kScaleBlocks + blockIdx.x] = red[0];
+}
+static_assert(kScaleBlocks == 256, "k_alm_scale reduces one partial per thread");
+
+// sB[map] = 2^e with 2^e max|a_lm| max|G| <= 2^12 (1 for an all-zero or non-finite map)
+__global__ __launch_bounds__(256) void k_alm_scale(const double* __restrict__ part, double gmax,
+                                                   float* __restrict__ sB) {
+  __shared__ double red[256];
+  red[threadIdx.x] = part[(long long)blockIdx.x * kScaleBlocks + threadIdx.x];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double bound = red[0] * gmax;
+    int e = 1;
+    if (bound > 0.0 && bound < 1e300) (void)frexp(4096.0 / bound, &e);
+    sB[blockIdx.x] = ldexpf(1.0f, min(max(e - 1, -120), 120));
+  }
+}
+
+__device__ __forceinline__ uint32_t split_f16(float v) {
+  const _Float16 hi = (_Float16)v;
+  const _Float16 lo = (_Float16)(v - (float)hi);
+  return __builtin_bit_cast(uint32_t, halfx2{hi, lo});
+}
+
+__device__ __forceinline__ uint32_t dup_f16(_Float16 h) { return __builtin_bit_cast(uint32_t, halfx2{h, h}); }
+
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const double2* __restrict__ alm,
+                                                              const float* __restrict__ sB, double sA,
+                                                              float2* __restrict__ F, int batch) {
+  constexpr int MG = 4 * NCB;
+  constexpr int NCOL = 4 * MG;
+  constexpr int ASH = 2 * NCOL + 8;  // s_b row stride (dwords): 64-bit B reads, 4 rows apart per lane group
+  constexpr int TSH = 68;            // tile row stride (dwords): A reads 4 rows apart per lane group
+  constexpr int GS = 16;             // l steps per group
+  constexpr int RINGS = 128;         // rings per work-group (4 waves x 32)
+  constexpr int LOOP_DW = CH * ASH + 4 * 2 * GS * TSH;
+  constexpr int EPI_FLOATS = 4 * 2 * 64 * 16;
+  static_assert(CH % GS == 0, "groups tile the chunk");
+  __shared__ double2 s_ag[CH][2];
+  __shared__ __align__(16) uint32_t s_buf[LOOP_DW > EPI_FLOATS ? LOOP_DW : EPI_FLOATS];
+  uint32_t* s_b = s_buf;              // [l][2 column + {hi hi, lo lo}]
+  uint32_t* tiles = s_buf + CH * ASH;  // [wave][buffer][step][lane]: (hi, lo)
+  int m, chunk;
+  if (!decode_block<false>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r32 = lane & 31, sg = lane >> 5;
+  const int ii = lane & 15, kk = lane >> 4;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb0 = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + wave * 32 + r32;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid && lmin <= lmax) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m + 4 * sg;
+    double sv;
+    seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
+    mup = -sv;
+  }
+  floatx4 D[2][4][NCB];  // [parity][row block][column block]
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint32_t* const tile0 = tiles + wave * 2 * GS * TSH;
+  uint32_t* const tile1 = tile0 + GS * TSH;
+  // the maps' scales (published by the first barrier of the loop)
+  __shared__ float s_sc[MG];
+  if (tid < MG) s_sc[tid] = (mg * MG + tid < batch) ? sB[mg * MG + tid] : 0.f;
+  constexpr int NP = 2 * MG * CH / 256;
+  double2 pf_a[NP], pf_ag = make_double2(0.0, 0.0);
+  double pf_g = 0.0;
+  auto prefetch = [&](int l0n) {
+    if (tid < 2 * CH) {
+      const int l = l0n + (tid >> 1);
+      pf_ag = make_double2(0.0, 0.0);
+      if (l <= lmax && l != lmin)
+        pf_ag = make_double2(A.alpha[cb0 + l], (tid & 1) ? -A.gamma[cb0 + l] : A.gamma[cb0 + l]);
+    }
+    load_alm_s2<MG>(A, alm, cb0, l0n, lmax, mg, batch, tid, pf_a, pf_g);
+  };
+  auto store_b = [&]() {
+    const int j = tid % CH, pr0 = tid / CH;
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+      const int pr = pr0 + u * (256 / CH);
+      uint32_t* d = s_b + j * ASH + 2 * (4 * (pr >> 1) + 2 * (pr & 1));
+      const double gs = pf_g * (double)s_sc[pr >> 1];
+      const float vr = (float)(pf_a[u].x * gs);
+      const float vi = (float)(pf_a[u].y * gs);
+      const _Float16 hr = (_Float16)vr, hi_ = (_Float16)vi;
+      const _Float16 lr = (_Float16)(vr - (float)hr), li = (_Float16)(vi - (float)hi_);
+      *reinterpret_cast<uint4*>(d) = make_uint4(dup_f16(hr), dup_f16(lr), dup_f16(hi_), dup_f16(li));
+    }
+  };
+  if (lmin <= lmax) prefetch(lmin);
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < 2 * CH) s_ag[tid >> 1][tid & 1] = pf_ag;
+    store_b();
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+    const int nst = min(CH, (lmax - l0 + 1 + GS - 1) & ~(GS - 1));
+    int j0 = 0;
+    // every lane below 2^-256: recursion only (a lane's exponent never decreases)
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double2 ag[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[j0 + 8 * h + jj][sg];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+          mup = mu;
+          mu = n;
+        }
+        rescale_pair(mu, mup, ex);
+      }
+      j0 += GS;
+    }
+    auto rec_group = [&](int jg, uint32_t* tl) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double f = (ex == 0) ? sA : 0.0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          double2 ag[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) ag[jj] = s_ag[jg + 8 * h + 4 * q + jj][sg];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+            mup = mu;
+            mu = n;
+            // the 1/0 mask in FP64 first: a lane below 2^-256 may hold a mantissa above range
+            tl[(8 * h + 4 * q + jj) * TSH + lane] = split_f16((float)(n * f));
+          }
+        }
+        rescale_pair(mu, mup, ex);
+      }
+    };
+    // lane group kk takes steps 4 kk + p and 4 kk + 2 + p: A halves (hi, lo, hi, lo) of each,
+    // B halves (hi, hi, lo, lo)
+    auto mfma_group = [&](int jg, const uint32_t* tl) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        halfx8 bv[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+          const uint2 b0 = *reinterpret_cast<const uint2*>(s_b + (jg + 4 * kk + p) * ASH + 2 * (16 * c + ii));
+          const uint2 b1 = *reinterpret_cast<const uint2*>(s_b + (jg + 4 * kk + 2 + p) * ASH + 2 * (16 * c + ii));
+          bv[c] = __builtin_bit_cast(halfx8, make_uint4(b0.x, b0.y, b1.x, b1.y));
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const uint32_t a0 = tl[(4 * kk + p) * TSH + 16 * rb + ii];
+          const uint32_t a1 = tl[(4 * kk + 2 + p) * TSH + 16 * rb + ii];
+          const halfx8 av = __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1));
+#pragma unroll
+          for (int c = 0; c < NCB; ++c)
+            D[p][rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv[c], D[p][rb][c], 0, 0, 0);
+        }
+      }
+    };
+    if (j0 < nst) {
+      rec_group(j0, tile0);
+      wave_sync();
+      bool odd = false;
+      for (; j0 + GS < nst; j0 += GS) {
+        mfma_group(j0, odd ? tile1 : tile0);
+        rec_group(j0 + GS, odd ? tile0 : tile1);
+        wave_sync();
+        odd = !odd;
+      }
+      mfma_group(j0, odd ? tile1 : tile0);
+      wave_sync();
+    }
+  }
+  // epilogue (as k_synth_tile_s2), with the scales undone in FP64
+  __syncthreads();
+  float* ep = reinterpret_cast<float*>(s_buf) + wave * 2 * 64 * 16;
+  const int swap = (lmin - m) & 1;
+  const long long ringM = A.M;
+  const int tt = chunk * RINGS + wave * 32 + r32;
+  const int half = lane >> 5;
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep[((p ^ swap) * 64 + 16 * rb + 4 * kk + r) * 16 + ii] = D[p][rb][c][r];
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int g = 2 * half + h;
+      const int b = mg * MG + 4 * c + g;
+      if (tt < A.T && b < batch) {
+        float2* Fp = F + (long long)b * 2 * A.nrings * ringM;
+        float2* Fm = Fp + (long long)A.nrings * ringM;
+        const long long rs = (long long)(A.nrings - 1 - tt) * ringM + m;
+        if (lmin <= lmax) {
+          const double inv = 1.0 / (sA * (double)sB[b]);  // a power of two: exact
+          double P0[4], P1[4], M0[4], M1[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            P0[k] = inv * ep[(0 * 64 + r32) * 16 + 4 * g + k];
+            P1[k] = inv * ep[(1 * 64 + r32) * 16 + 4 * g + k];
+            M0[k] = inv * ep[(0 * 64 + 32 + r32) * 16 + 4 * g + k];
+            M1[k] = inv * ep[(1 * 64 + 32 + r32) * 16 + 4 * g + k];
+          }
+          const double pe_r = P0[0] + P1[0], pe_i = P0[1] + P1[1], pb_r = P0[2] + P1[2], pb_i = P0[3] + P1[3];
+          const double me_r = M0[0] + M1[0], me_i = M0[1] + M1[1], mb_r = M0[2] + M1[2], mb_i = M0[3] + M1[3];
+          Fp[(long long)tt * ringM + m] = f2(-pe_r + pb_i, -pe_i - pb_r);
+          Fm[(long long)tt * ringM + m] = f2(-me_r - mb_i, -me_i + mb_r);
+          if (tt < A.T - 1) {
+            const double de_r = M0[0] - M1[0], de_i = M0[1] - M1[1], db_r = M0[2] - M1[2], db_i = M0[3] - M1[3];
+            const double qe_r = P0[0] - P1[0], qe_i = P0[1] - P1[1], qb_r = P0[2] - P1[2], qb_i = P0[3] - P1[3];
+            Fp[rs] = f2(-de_r + db_i, -de_i - db_r);
+            Fm[rs] = f2(-qe_r - qb_i, -qe_i + qb_r);
+          }
+        } else {
+          Fp[(long long)tt * ringM + m] = Fm[(long long)tt * ringM + m] = make_float2(0.f, 0.f);
+          if (tt < A.T - 1) Fp[rs] = Fm[rs] = make_float2(0.f, 0.f);
+        }
+      }
+    }
+    wave_sync();
+  }
+}
+
 // Same software pipeline as k_synth_tile_s2: the matrix instructions of group j and the
 // recursion of group j + 1 (into the other tile buffer) form one straight-line block.
 
@@ -1503,7 +1777,7 @@ int mixed_group(const jhtx_plan* p, bool adjoint) {
 }
 
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
-                              hipStream_t s) {
+                              void* scale_ws, hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, false);
@@ -1512,7 +1786,15 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     RecArgs at = a;
     at.nchunk = (p->T + 127) / 128;
     dim3 gt(Mpad * at.nchunk, (batch + 15) / 16);
-    hipLaunchKernelGGL((k_synth_tile_s2<4, 2>), gt, dim3(256), 0, s, at, alm, F, batch);
+    if (p->mx_f16 && scale_ws) {
+      double* part = reinterpret_cast<double*>(scale_ws);
+      float* sB = reinterpret_cast<float*>(part + (size_t)kScaleBlocks * batch);
+      hipLaunchKernelGGL(k_alm_absmax, dim3(kScaleBlocks, batch), dim3(256), 0, s, alm, 2 * p->K, part);
+      hipLaunchKernelGGL(k_alm_scale, dim3(batch), dim3(256), 0, s, part, p->mx_gmax, sB);
+      hipLaunchKernelGGL((k_synth_tile_s2h<4, 2>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
+    } else {
+      hipLaunchKernelGGL((k_synth_tile_s2<4, 2>), gt, dim3(256), 0, s, at, alm, F, batch);
+    }
   } else if (p->spin == 0 && MG == 32)
     hipLaunchKernelGGL((k_synth_tile_s0<4, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (p->spin == 0 && MG == 16)

@@ -25,7 +25,8 @@ _CANDIDATES = {
     ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)],
                  [{"f64_adj_mfma": 1}] + [{"f64_adj_mfma": 2, "adj_rings": r} for r in (2, 4, 1)]),
     ("mixed", 0): ([{"mx_synth_group": g} for g in (32, 16, 8)], [{"mx_adj_group": g} for g in (16, 8)]),
-    ("mixed", 2): ([{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
+    ("mixed", 2): ([{"mx_synth_group": 16, "mx_f16": f} for f in (1, 2)] +
+                   [{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
                    [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_wide": w, "mx_adj_cb": c}
                     for g, o, w, c in ((16, 2, 2, 1), (16, 2, 2, 2), (8, 2, 2, 2), (8, 2, 1, 2), (4, 1, 2, 2),
                                        (8, 1, 2, 2))]),

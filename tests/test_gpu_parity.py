@@ -232,6 +232,39 @@ def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
         assert _rel(bmx, O.adjoint_synthesis(v, nside, lmax, spin)) <= MIXED_TOL
 
 
+@pytest.mark.parametrize("f16", [1, 0])
+@pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
+                                                  (128, 256, 16, 1e25), (256, 400, 32, 1.0)])
+def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
+    """The 16-map spin-2 synthesis on the FP16 matrix cores (f16 1: hi + lo halves of every
+    operand after power-of-two scales) and on the FP32 ones (f16 0) against the FP64 path:
+    ragged map groups, inputs far from unit scale (the per-map scale pass), and an all-zero
+    map, which must synthesise to exact zeros."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_F16", str(f16))
+    H.clear_plans()
+    try:
+        rng = np.random.default_rng(4400 + nside + batch)
+        a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                      for _ in range(batch)]) * amp
+        a[1] = 0.0
+        at = torch.tensor(a, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, 2, precision="f64").cpu().numpy()
+            smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed").cpu().numpy()
+            plan = H.get_plan(nside, lmax, 2, batch, H._prec("mixed"))
+            assert plan.tuning["mx_f16"] == (1 if f16 else 2)
+    finally:
+        H.clear_plans()
+    assert np.all(smx[1] == 0.0)
+    es = max(_rel(smx[i], s64[i]) for i in range(batch) if i != 1)
+    print(f"mixed f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: synth {es:.2e}", flush=True)
+    assert es <= MIXED_TOL
+
+
 @pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ",
                          [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2),
                           (16, 2, 16, 2), (16, 2, 4, 1)])
