@@ -1,6 +1,8 @@
 // Mixed-precision batched recursion + contraction: the l-recursion stays in FP64 on the
 // vector ALUs (one ring per lane, exponent-tracked exactly as in the FP64 kernels); the
-// contraction over l runs on the FP32 matrix cores for a group of MG maps at once.
+// contraction over l runs on the FP32 matrix cores for a group of MG maps at once (the
+// 16-map spin-2 synthesis on the FP16 ones, with every operand split into hi + lo halves:
+// k_synth_tile_s2h).
 //
 // Synthesis, F[t][col] = sum_l lambda_l(t) a~_l[col]: v_mfma_f32_4x4x1f32 (16 blocks of 4x4)
 // takes A from lane 4b+i and B from lane 4b+j, so the recursion value of the lane's ring is
