@@ -17,9 +17,10 @@ chunk reduction is the separate stage adj_finish) is timed live with HIP events 
 the library on the call's stream around that kernel alone.  Work unit: one (l, m,
 north-half ring) triple of the active triangle l >= max(m, |s|).  FP64 path: 26 flops per
 triple per map (spin 2; 8 for spin 0) against the 78.6 TFLOP/s FP64 peak.  Mixed path: the
-contraction that runs on the FP32 matrix cores, 16 flops per triple per map (spin 2; 4 for
-spin 0), against the 157.3 TFLOP/s dense FP32 MFMA peak; a one-map mixed call runs the FP64
-kernels and is priced as the FP64 path.  CPU baseline: the reference's
+contraction, 16 flops per triple per map (spin 2; 4 for spin 0), against the 157.3 TFLOP/s
+dense FP32 MFMA peak; the 16-map spin-2 kernels run it on the FP16 matrix cores as four FP16
+products per FP32-equivalent product, against a quarter of the 2500 TFLOP/s dense FP16 peak; a
+one-map mixed call runs the FP64 kernels and is priced as the FP64 path.  CPU baseline: the reference's
 fused masked scan restated in numpy (oracle/jht_oracle.py) on a sample of orders m over
 every l and ring, scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs.
 """
@@ -42,9 +43,14 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # this script (each counter in its own run, scaled by the counters of a 1 GiB device copy:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax, spin, batch, stage)
 MEASURED_TRAFFIC = {
-    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 57.1e9,
+    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 77.7e9,
+    ("mixed", 2048, 4096, 2, 16, "synth_rec"): 60.5e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X dense FP16 matrix
+# the FP16-split kernels spend 4 FP16 products (hi hi, lo hi, hi lo, lo lo) per FP32-equivalent
+# product: their ceiling for the FP32-equivalent contraction flops is a quarter of the FP16 peak
+F16_SPLIT_PRODUCTS = 4
 HBM_PEAK_GBS = 8000.0
 
 
@@ -381,8 +387,17 @@ def main():
     tri = triples(nside, lmax, spin)
     # one-map calls of a mixed plan run the FP64 vector-FMA recursion kernels (mx_single)
     fp64_kernels = prec == _lib.PREC_F64 or (B == 1 and plan.tuning["mx_single"] == 1)
+    tun = plan.tuning
+    # the 16-map spin-2 kernels on the FP16 matrix cores (hi + lo split)
+    tile16 = prec != _lib.PREC_F64 and spin == 2 and B >= 16
+    f16 = {"synth_rec": tile16 and tun["mx_f16"] == 1,
+           "adj_rec": tile16 and tun["mx_adj_f16"] == 1 and tun["mx_adj_cb"] == 1 and tun["mx_adj_wide"] != 1}
+    peak_basis = None
     if fp64_kernels:
         flops, peak, bound = tri * (26 if spin else 8) * B, PEAK_FP64_TFLOPS, "mfma"
+    elif f16[dom]:
+        flops, peak, bound = tri * 16 * B, PEAK_FP16_MFMA_TFLOPS / F16_SPLIT_PRODUCTS, "mfma"
+        peak_basis = "fp16 dense 2500 TFLOP/s / 4 fp16 products per fp32-equivalent product"
     else:
         flops, peak, bound = tri * (16 if spin else 4) * B, PEAK_FP32_MFMA_TFLOPS, "mfma"
     achieved = flops / (per[dom][0] * 1e-3) / 1e12
@@ -399,16 +414,15 @@ def main():
         "vs_baseline": None,
         "dtype": ("f64" if prec == _lib.PREC_F64 else
                   "f64 recursion and contraction / f32 ring fft" if fp64_kernels else
-                  "f64 recursion / f16 hi+lo split mfma synthesis, f32 mfma adjoint"
-                  if (spin == 2 and B >= 16 and plan.tuning["mx_f16"] == 1) else
-                  "f64 recursion / f32 mfma contraction"),
+                  "f64 recursion / fp32 contraction as fp16 hi+lo split mfma" if f16["synth_rec"] and f16["adj_rec"]
+                  else "f64 recursion / f32 mfma contraction"),
         "data": "synthetic (seeded Gaussian E/B a_lm, C_l=(l+1)^-2, B=0.1 E, l<2 zero)",
         "config": {"workload": f"spin-{spin} synthesis+adjoint, nside={nside}, lmax={lmax}",
                    "nside": nside, "lmax": lmax, "spin": spin, "batch": B,
                    "precision": args.precision, "parallelism": "single-gpu"},
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4),
+                     "frac": round(achieved / peak, 4), **({"peak_basis": peak_basis} if peak_basis else {}),
                      "traffic": MEASURED_TRAFFIC.get((args.precision, nside, lmax, spin, B, dom))},
         "stages_ms": {k: round(v[0], 3) for k, v in per.items()},
         "adjoint_identity": {"lhs": lhs, "rhs": rhs, "rel": ident},

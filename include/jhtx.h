@@ -76,7 +76,9 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            kernels}.  f64_adj_mfma does not apply to mixed plans.
  *   mx_f16                   16-map mixed spin-2 synthesis on the FP16 matrix cores, each
  *                            FP32 operand split into hi + lo halves after a power-of-two
- *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles} */
+ *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles}
+ *   mx_adj_f16               the same split for the 16-map mixed spin-2 adjoint in
+ *                            column-block form {1 on (default), 2 off} */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
@@ -85,6 +87,7 @@ typedef struct jhtx_tuning {
   int mx_adj_cb;
   int mx_single;
   int mx_f16;
+  int mx_adj_f16;
 } jhtx_tuning;
 
 /* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An

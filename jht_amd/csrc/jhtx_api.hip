@@ -400,6 +400,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // four products: half the matrix instructions of FP32): 77.5 -> 62.1 ms at nside 2048 /
   // lmax 4096, batch 16, error against FP64 4.7e-7 -> 1.0e-6
   p->mx_f16 = pick2("JHTX_MX_F16", 1, 0, 1);
+  // the same split in the 16-map column-block adjoint: 81.0 -> 67.2 ms at nside 2048 /
+  // lmax 4096, batch 16, adjoint error against FP64 1.5e-7 -> 1.6e-7
+  p->mx_adj_f16 = pick2("JHTX_MX_ADJ_F16", 1, 0, 1);
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);
@@ -422,7 +425,8 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
         (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
         (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2})) && (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) &&
-        (t.mx_single == 0 || in(t.mx_single, {1, 2})) && (t.mx_f16 == 0 || in(t.mx_f16, {1, 2}));
+        (t.mx_single == 0 || in(t.mx_single, {1, 2})) && (t.mx_f16 == 0 || in(t.mx_f16, {1, 2})) &&
+        (t.mx_adj_f16 == 0 || in(t.mx_adj_f16, {1, 2}));
     if (!ok) {
       set_error("jhtx_tuning field out of range");
       return JHTX_EINVAL;
@@ -439,6 +443,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
     if (t.mx_adj_cb) p->mx_adj_cb = (t.mx_adj_cb == 1);
     if (t.mx_single) p->mx_single = (t.mx_single == 1);
     if (t.mx_f16) p->mx_f16 = (t.mx_f16 == 1);
+    if (t.mx_adj_f16) p->mx_adj_f16 = (t.mx_adj_f16 == 1);
   }
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
@@ -500,6 +505,7 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->f64_adj_mfma = p->f64_adj_mfma ? 1 : 2;
   t->mx_single = p->mx_single ? 1 : 2;
   t->mx_f16 = p->mx_f16 ? 1 : 2;
+  t->mx_adj_f16 = p->mx_adj_f16 ? 1 : 2;
   t->mx_adj_cb = p->mx_adj_cb ? 1 : 2;
   return JHTX_OK;
 }

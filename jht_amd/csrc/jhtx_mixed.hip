@@ -1521,6 +1521,211 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
   }
 }
 
+// k_adj_mx_s2_cb<1> on the FP16 matrix cores.  The tiles hold (hi, lo) FP16 pairs of the
+// recursion values (scale sA) instead of floats, the B operands (hi, lo) pairs of the lane's
+// V values; v_mfma_f32_16x16x32_f16 contracts 8 rings per instruction (A halves hi, lo, hi, lo
+// of a ring, B halves hi, hi, lo, lo: the four products), half the instructions of the FP32
+// form.  B's scale is local: each work-group scales every map's V values of its rings by a
+// power of two that puts their maximum at or below 2^12, and undoes it (with sA) before the
+// FP64 sums are stored, so the chunk partials are those of the FP32 kernel up to rounding.
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float2* __restrict__ V, double sA,
+                                                         float2* __restrict__ part, int batch) {
+  constexpr int NW = 4, MG = 16, RINGS = 256;
+  __shared__ double2 s_ag[CH];
+  __shared__ __align__(16) uint32_t tiles[NW][2][TL][TS2];  // [ring block][sign][row][col]: (hi, lo)
+  __shared__ int s_act[NW];
+  int m, chunk;
+  if (!decode_block<true>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  if (lmin > lmax) return;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
+  int ep = 0, em = 0;
+  if (valid) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m;
+    const double lc = A.lcos[t], ls = A.lsin[t];
+    double s;
+    seed_split(spin2_seed_L(sd, lc, ls), sd[3], s, ep);
+    ppp = -s;
+    seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
+    pmp = -s;
+  }
+  // B operands of column block cw: chunk ring 4 q + lane/16 (q = 0..63), column lane%16 ->
+  // map 4 cw + (lane%16)/4, component lane%4 (as k_adj_mx_s2_cb)
+  const int cw = wave;
+  const int g = (lane & 15) >> 2, cc = lane & 3;
+  const int bcol = mg * MG + 4 * cw + g;
+  uint32_t Bv[2][64];  // [sign][q]: (hi, lo) of the scaled value
+  double inv;          // 1 / (sA sB) of this lane's map
+  {
+    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const bool lo = cc < 2;
+    const int ri = cc & 1;
+    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
+    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
+    // the lane's two B values of ring 4 q + lane/16 (signs +, -)
+    auto load_v = [&](int q, float& bp, float& bm) {
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
+      double vp = 0.0, vm = 0.0;
+      if (tr < A.T && bcol < batch) {
+        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
+        const bool south = tr < A.T - 1;  // the equator ring has no mirror
+        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
+        if (lo) {
+          vp = (double)Vp[rn];
+          vm = south ? par0 * (double)Vp[rs] : 0.0;
+        } else {
+          vp = south ? par0 * (double)Vm[rs] : 0.0;
+          vm = (double)Vm[rn];
+        }
+      }
+      bp = (float)vp;
+      bm = (float)vm;
+    };
+    // two passes over the values (the second from cache): the maximum, then the split
+    float mx = 0.f;
+#pragma unroll 8
+    for (int q = 0; q < 64; ++q) {
+      float bp, bm;
+      load_v(q, bp, bm);
+      mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
+    }
+    // the map's maximum over its 16 lanes (4 components x 4 ring residues): lanes that
+    // differ in bits 0, 1, 4 and 5
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 2));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    int e = 1;
+    if (mx > 0.f && mx < 3.0e38f) (void)frexpf(4096.f / mx, &e);
+    e = min(max(e - 1, -120), 120);
+    const float sB = ldexpf(1.f, e);
+    inv = 1.0 / (sA * (double)sB);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      float bp, bm;
+      load_v(q, bp, bm);
+      Bv[0][q] = split_f16(bp * sB);
+      Bv[1][q] = split_f16(bm * sB);
+    }
+  }
+  float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
+  const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
+  uint32_t* mytile = &tiles[wave][0][0][0];
+  const int tcol = s2_tile_col(lane);
+  const uint32_t* abase = &tiles[0][0][0][0] + (lane & 15) * TS2 + 16 * (lane >> 4);
+  double2 pf_ag = make_double2(0.0, 0.0);
+  auto prefetch = [&](int l0n) {
+    const int l = l0n + tid;
+    pf_ag = make_double2(0.0, 0.0);
+    if (tid < CH && l <= lmax && l != lmin) pf_ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
+  };
+  prefetch(lmin);
+  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) s_ag[tid] = pf_ag;
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+#pragma unroll 1
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+      int act = 0;
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const bool deep = valid && ((ep < 0) || (em < 0));
+        const int anydeep = __ballot(deep) != 0ull;
+        act |= (__ballot(valid && ep == 0) != 0ull ? 1 : 0) | (__ballot(valid && em == 0) != 0ull ? 2 : 0);
+        const double fp = (ep == 0 && valid) ? sA : 0.0, fm = (em == 0 && valid) ? sA : 0.0;
+        double2 ag[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[jb + j0 + jj];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double cp = fma(ag[jj].x, x, ag[jj].y);
+          const double cm = fma(ag[jj].x, x, -ag[jj].y);
+          const double np = fma(cp, pp, -ppp);
+          const double nm = fma(cm, pm, -pmp);
+          ppp = pp;
+          pp = np;
+          pmp = pm;
+          pm = nm;
+          mytile[(0 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = split_f16((float)(np * fp));
+          // odd rows (odd l - lmin) of the sign - tile negated: one B serves both parities
+          mytile[(1 * TL + s2_tile_row(j0 + jj)) * TS2 + tcol] = split_f16((float)(nm * ((jj & 1) ? -fm : fm)));
+        }
+        if (anydeep) {
+          rescale_pair(pp, ppp, ep);
+          rescale_pair(pm, pmp, em);
+        }
+      }
+      if (lane == 0) s_act[wave] = act;
+      __syncthreads();
+      int actv[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) actv[w] = s_act[w];
+      // byte selectors of (hi, hi) and (lo, lo) from a (hi, lo) word.  pz is always 0, but
+      // the compiler cannot prove it: the B operands built from the loop-invariant Bv then
+      // stay in the loop instead of being hoisted as 256 more registers
+      const uint32_t pz = (uint32_t)(l0 + jb > (1 << 30));
+      const uint32_t sel_hh = 0x01000100u | pz, sel_ll = 0x03020302u | pz;
+      // D[p][rb]: parity p, ring block rb, FP32 over the block's 64 rings, FP64 over blocks
+      floatx4 D[2][4];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) D[p][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          if (!((actv[rb] >> sg) & 1)) continue;
+          uint4 a[2][4];  // this lane's row: the 16 rings 4 n + lane/16 of the block, (hi, lo)
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const uint4* src = reinterpret_cast<const uint4*>(abase + ((rb * 2 + sg) * TL + 16 * p) * TS2);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) a[p][n] = src[n];
+          }
+          // instruction n: rings 4 (2 n) + k and 4 (2 n + 1) + k of lane group k
+#pragma unroll
+          for (int n = 0; n < 8; ++n) {
+            const uint32_t w0 = Bv[sg][16 * rb + 2 * n], w1 = Bv[sg][16 * rb + 2 * n + 1];
+            const halfx8 bq = __builtin_bit_cast(
+                halfx8, make_uint4(__builtin_amdgcn_perm(w0, w0, sel_hh), __builtin_amdgcn_perm(w0, w0, sel_ll),
+                                   __builtin_amdgcn_perm(w1, w1, sel_hh), __builtin_amdgcn_perm(w1, w1, sel_ll)));
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+              const uint4 av4 = a[p][n >> 1];
+              const uint32_t a0 = (n & 1) ? av4.z : av4.x, a1 = (n & 1) ? av4.w : av4.y;
+              D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1)),
+                                                                bq, D[p][rb], 0, 0, 0);
+            }
+          }
+        }
+      if (bcol < batch) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + p;
+            double v = 0.0;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) v += (double)D[p][rb][r];
+            v *= inv;
+            if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * (float)v : (float)v;
+          }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
 __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __restrict__ V,
                                                    float2* __restrict__ part, int batch) {
@@ -1832,6 +2037,8 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
     hipLaunchKernelGGL(k_adj_mx_s2_cb<2>, grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 16 && mixed_adj_waves(p) == 8)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
+  else if (MG == 16 && p->mx_adj_cb && p->mx_adj_f16)
+    hipLaunchKernelGGL(k_adj_mx_s2_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
   else if (MG == 16 && p->mx_adj_cb)
     hipLaunchKernelGGL(k_adj_mx_s2_cb<1>, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16)
