@@ -74,7 +74,7 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            recursion kernels on complex64 ring coefficients, with the
  *                            FP32 ring FFT {1 on (default), 2 off: the 4-map matrix-core
  *                            kernels}.  f64_adj_mfma does not apply to mixed plans.
- *   mx_f16                   16-map mixed spin-2 synthesis on the FP16 matrix cores, each
+ *   mx_f16                   16-map mixed synthesis (spin 2 and spin 0) on the FP16 matrix cores, each
  *                            FP32 operand split into hi + lo halves after a power-of-two
  *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles}
  *   mx_adj_f16               the same split for the 16-map mixed spin-2 adjoint in

@@ -108,7 +108,7 @@ struct jhtx_plan {
   int mx_adj_cb;       // 16-map adjoint in column-block form (0 or 1)
   int mx_single;       // mixed plans: one-map calls run the FP64 vector-FMA recursion kernels
                        // on complex64 F / V (0 or 1)
-  int mx_f16;          // 16-map spin-2 synthesis on the FP16 matrix cores, operands split
+  int mx_f16;          // 16-map synthesis on the FP16 matrix cores, operands split
                        // into hi + lo halves (0 or 1)
   int mx_adj_f16;      // the same for the 16-map column-block adjoint (0 or 1)
   double mx_sA;        // power-of-two scale of the recursion values for the FP16 split

@@ -1079,6 +1079,183 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
   }
 }
 
+// k_synth_tile_s0<NCB, MINB> on the FP16 matrix cores, with the operand split, scales and
+// instruction layout of k_synth_tile_s2h: 16-step groups, A halves (hi, lo, hi, lo) from the
+// wave tile, B halves (hi, hi, lo, lo) staged in LDS, four products per l step.
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s0h(RecArgs A, const double2* __restrict__ alm,
+                                                              const float* __restrict__ sB, double sA,
+                                                              float2* __restrict__ F, int batch) {
+  constexpr int MG = 8 * NCB, RINGS = 256;
+  constexpr int NCOL = 2 * MG;
+  constexpr int ASH = 2 * NCOL + 8;  // s_b row stride (dwords): 64-bit B reads, 4 rows apart per lane group
+  constexpr int TSH = 68;            // tile row stride (dwords): A reads 4 rows apart per lane group
+  constexpr int GS = 16;
+  constexpr int EPS = 17;
+  constexpr int LOOP_DW = CH * ASH + 4 * 2 * GS * TSH;
+  constexpr int EPI_FLOATS = 4 * 2 * 64 * EPS;
+  static_assert(CH % GS == 0, "groups tile the chunk");
+  __shared__ double s_al[CH];
+  __shared__ float s_sc[MG];
+  __shared__ __align__(16) uint32_t s_buf[LOOP_DW > EPI_FLOATS ? LOOP_DW : EPI_FLOATS];
+  uint32_t* s_b = s_buf;               // [l][2 column + {hi hi, lo lo}]
+  uint32_t* tiles = s_buf + CH * ASH;  // [wave][buffer][step][lane]: (hi, lo)
+  int m, chunk;
+  if (!decode_block<false>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ii = lane & 15, kk = lane >> 4;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double sv;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, sv, ex);
+    mup = -sv;
+  }
+  if (tid < MG) s_sc[tid] = (mg * MG + tid < batch) ? sB[mg * MG + tid] : 0.f;
+  floatx4 D[2][4][NCB];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) D[p][rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint32_t* const tile0 = tiles + wave * 2 * GS * TSH;
+  uint32_t* const tile1 = tile0 + GS * TSH;
+  double2 pf_a[MG / 4];
+  double pf_g = 0.0, pf_al = 0.0;
+  auto prefetch = [&](int l0n) {
+    if (tid < CH) {
+      const int l = l0n + tid;
+      pf_al = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    load_alm_s0<MG>(A, alm, cb, l0n, lmax, mg, batch, tid, pf_a, pf_g);
+  };
+  // s_b[j][2 (2 map + re/im) + {hi hi, lo lo}]
+  auto store_b = [&]() {
+    const int j = tid % CH, b0 = tid / CH;
+#pragma unroll
+    for (int u = 0; u < MG / 4; ++u) {
+      const int map = b0 + 4 * u;
+      const double gs = pf_g * (double)s_sc[map];
+      const uint32_t sr = split_f16((float)(pf_a[u].x * gs)), si = split_f16((float)(pf_a[u].y * gs));
+      *reinterpret_cast<uint4*>(s_b + j * ASH + 4 * map) = make_uint4(dup_lo(sr), dup_hi(sr), dup_lo(si), dup_hi(si));
+    }
+  };
+  prefetch(m);
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) s_al[tid] = pf_al;
+    store_b();
+    __syncthreads();
+    if (l0 + CH <= lmax) prefetch(l0 + CH);
+    const int nst = min(CH, (lmax - l0 + 1 + GS - 1) & ~(GS - 1));
+    int j0 = 0;
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double al[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) al[jj] = s_al[j0 + 8 * h + jj];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double n = fma(al[jj] * x, mu, -mup);
+          mup = mu;
+          mu = n;
+        }
+        rescale_pair(mu, mup, ex);
+      }
+      j0 += GS;
+    }
+    auto rec_group = [&](int jg, uint32_t* tl) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double f = (ex == 0) ? sA : 0.0;
+        double al[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) al[jj] = s_al[jg + 8 * h + jj];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double n = fma(al[jj] * x, mu, -mup);
+          mup = mu;
+          mu = n;
+          // the 1/0 mask in FP64 first: a lane below 2^-256 may hold a mantissa above range
+          tl[(8 * h + jj) * TSH + lane] = split_f16((float)(n * f));
+        }
+        rescale_pair(mu, mup, ex);
+      }
+    };
+    auto mfma_group = [&](int jg, const uint32_t* tl) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        halfx8 bv[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+          const uint2 b0 = *reinterpret_cast<const uint2*>(s_b + (jg + 4 * kk + p) * ASH + 2 * (16 * c + ii));
+          const uint2 b1 = *reinterpret_cast<const uint2*>(s_b + (jg + 4 * kk + 2 + p) * ASH + 2 * (16 * c + ii));
+          bv[c] = __builtin_bit_cast(halfx8, make_uint4(b0.x, b0.y, b1.x, b1.y));
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const uint32_t a0 = tl[(4 * kk + p) * TSH + 16 * rb + ii];
+          const uint32_t a1 = tl[(4 * kk + 2 + p) * TSH + 16 * rb + ii];
+          const halfx8 av = __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1));
+#pragma unroll
+          for (int c = 0; c < NCB; ++c)
+            D[p][rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv[c], D[p][rb][c], 0, 0, 0);
+        }
+      }
+    };
+    if (j0 < nst) {
+      rec_group(j0, tile0);
+      wave_sync();
+      bool odd = false;
+      for (; j0 + GS < nst; j0 += GS) {
+        mfma_group(j0, odd ? tile1 : tile0);
+        rec_group(j0 + GS, odd ? tile0 : tile1);
+        wave_sync();
+        odd = !odd;
+      }
+      mfma_group(j0, odd ? tile1 : tile0);
+      wave_sync();
+    }
+  }
+  // epilogue (as k_synth_tile_s0), with the scales undone in FP64
+  __syncthreads();
+  float* ep = reinterpret_cast<float*>(s_buf) + wave * 2 * 64 * EPS;
+  const long long ringM = A.M;
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep[(p * 64 + 16 * rb + 4 * kk + r) * EPS + ii] = D[p][rb][c][r];
+    wave_sync();
+    if (valid) {
+#pragma unroll
+      for (int h = 0; h < 8; ++h) {
+        const int b = mg * MG + 8 * c + h;
+        if (b >= batch) continue;
+        const double inv = 1.0 / (sA * (double)s_sc[8 * c + h]);  // a power of two: exact
+        const double c0r = inv * ep[lane * EPS + 2 * h], c0i = inv * ep[lane * EPS + 2 * h + 1];
+        const double c1r = inv * ep[(64 + lane) * EPS + 2 * h], c1i = inv * ep[(64 + lane) * EPS + 2 * h + 1];
+        float2* Fb = F + (long long)b * A.nrings * ringM;
+        Fb[(long long)t * ringM + m] = f2(c0r + c1r, c0i + c1i);
+        if (t < A.T - 1) Fb[(long long)(A.nrings - 1 - t) * ringM + m] = f2(c0r - c1r, c0i - c1i);
+      }
+    }
+    wave_sync();
+  }
+}
+
 // ---------------------------------------------------------------------------------- //
 // adjoint: per-wave FP32 tile of 32 l x 64 rings (per recursion sign), 16x16x4 MFMA
 // ---------------------------------------------------------------------------------- //
@@ -2007,7 +2184,13 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     }
   } else if (p->spin == 0 && MG == 32)
     hipLaunchKernelGGL((k_synth_tile_s0<4, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
-  else if (p->spin == 0 && MG == 16)
+  else if (p->spin == 0 && MG == 16 && p->mx_f16 && scale_ws) {
+    double* part = reinterpret_cast<double*>(scale_ws);
+    float* sB = reinterpret_cast<float*>(part + (size_t)kScaleBlocks * batch);
+    hipLaunchKernelGGL(k_alm_absmax, dim3(kScaleBlocks, batch), dim3(256), 0, s, alm, p->K, part);
+    hipLaunchKernelGGL(k_alm_scale, dim3(batch), dim3(256), 0, s, part, p->mx_gmax, sB);
+    hipLaunchKernelGGL((k_synth_tile_s0h<2, 2>), grid, dim3(256), 0, s, a, alm, sB, p->mx_sA, F, batch);
+  } else if (p->spin == 0 && MG == 16)
     hipLaunchKernelGGL((k_synth_tile_s0<2, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (p->spin == 0)
     hipLaunchKernelGGL((k_synth_mx_s0<8>), grid, dim3(256), 0, s, a, alm, F, batch);

@@ -267,6 +267,37 @@ def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
 
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
+                                                  (128, 256, 16, 1e25), (256, 400, 32, 1.0)])
+def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16, monkeypatch):
+    """The 16-map spin-0 tile synthesis on the FP16 matrix cores (f16 1) and on the FP32 ones
+    (f16 0) against the FP64 path: ragged groups, inputs far from unit scale, an all-zero
+    map (exact zeros)."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_F16", str(f16))
+    H.clear_plans()
+    try:
+        rng = np.random.default_rng(4600 + nside + batch)
+        a = np.stack([O.gaussian_alm(lmax, rng) for _ in range(batch)]) * amp
+        a[1] = 0.0
+        at = torch.tensor(a, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, 0, precision="f64").cpu().numpy()
+            smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed").cpu().numpy()
+            plan = H.get_plan(nside, lmax, 0, batch, H._prec("mixed"))
+            assert plan.tuning["mx_f16"] == (1 if f16 else 2)
+    finally:
+        H.clear_plans()
+    assert np.all(smx[1] == 0.0)
+    es = max(_rel(smx[i], s64[i]) for i in range(batch) if i != 1)
+    print(f"mixed spin-0 f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: synth {es:.2e}", flush=True)
+    assert es <= MIXED_TOL
+
+
+@pytest.mark.parametrize("f16", [1, 0])
+@pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (512, 640, 32, 1.0)])
 def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
     """The 16-map spin-2 column-block adjoint on the FP16 matrix cores (f16 1: hi + lo halves;
