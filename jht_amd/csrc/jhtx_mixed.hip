@@ -1766,13 +1766,16 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
       bp = (float)vp;
       bm = (float)vm;
     };
-    // two passes over the values (the second from cache): the maximum, then the split
+    // one pass: the values go into Bv as float bits, then are split in place once the
+    // maximum is known
     float mx = 0.f;
-#pragma unroll 8
+#pragma unroll
     for (int q = 0; q < 64; ++q) {
       float bp, bm;
       load_v(q, bp, bm);
       mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
+      Bv[0][q] = __float_as_uint(bp);
+      Bv[1][q] = __float_as_uint(bm);
     }
     // the map's maximum over its 16 lanes (4 components x 4 ring residues): lanes that
     // differ in bits 0, 1, 4 and 5
@@ -1787,10 +1790,8 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
     inv = 1.0 / (sA * (double)sB);
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      float bp, bm;
-      load_v(q, bp, bm);
-      Bv[0][q] = split_f16(bp * sB);
-      Bv[1][q] = split_f16(bm * sB);
+      Bv[0][q] = split_f16(__uint_as_float(Bv[0][q]) * sB);
+      Bv[1][q] = split_f16(__uint_as_float(Bv[1][q]) * sB);
     }
   }
   float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
