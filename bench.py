@@ -44,7 +44,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax, spin, batch, stage)
 MEASURED_TRAFFIC = {
     ("mixed", 2048, 4096, 2, 16, "adj_rec"): 54.6e9,
-    ("mixed", 2048, 4096, 2, 16, "synth_rec"): 60.5e9,
+    ("mixed", 2048, 4096, 2, 16, "synth_rec"): 60.4e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X dense FP16 matrix
