@@ -400,9 +400,10 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // four products: half the matrix instructions of FP32): 77.5 -> 62.1 ms at nside 2048 /
   // lmax 4096, batch 16, error against FP64 4.7e-7 -> 1.0e-6
   p->mx_f16 = pick2("JHTX_MX_F16", 1, 0, 1);
-  // the same split in the 16-map column-block adjoint: 81.0 -> 59.6 ms at nside 2048 /
-  // lmax 4096, batch 16, adjoint error against FP64 1.5e-7 -> 1.6e-7
-  p->mx_adj_f16 = pick2("JHTX_MX_ADJ_F16", 1, 0, 1);
+  // the same split in the 16-map column-block adjoint: spin 2 81.0 -> 59.6 ms at nside 2048
+  // / lmax 4096, batch 16, error against FP64 1.5e-7 -> 1.6e-7, default on.  Spin 0: 26.4 ->
+  // 23.9 ms, but its l >= 2048 band error rises from 2.5e-6 to 6.1e-6: off by default
+  p->mx_adj_f16 = pick2("JHTX_MX_ADJ_F16", p->spin == 2 ? 1 : 0, 0, 1);
   int block_cap = 256;
   if (const char* v = getenv("JHTX_REC_BLOCK")) {
     const int b = atoi(v);

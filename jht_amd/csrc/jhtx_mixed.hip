@@ -2122,6 +2122,144 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cb(RecArgs A, const float2
   }
 }
 
+// k_adj_mx_s0_cb on the FP16 matrix cores, with the split, the local V scale and the
+// instruction layout of k_adj_mx_s2_cbh: 8 rings per instruction, the two chains over the
+// block halves (rings 0-31, 32-63) kept.
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cbh(RecArgs A, const float2* __restrict__ V, double sA,
+                                                         float2* __restrict__ part, int batch) {
+  constexpr int NW = 4, MG = 16, RINGS = 256;
+  __shared__ double s_al[CH];
+  __shared__ __align__(16) uint32_t tiles[NW][TL][TS2];  // [ring block][row][col]: (hi, lo)
+  __shared__ int s_act[NW];
+  int m, chunk;
+  if (!decode_block<true>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lmax = A.lmax;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * RINGS + tid;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double L = __dadd_rn(A.seed[m], __dmul_rn((double)m, A.lsin[t]));
+    double sv;
+    seed_split(L, (m & 1) ? -1.0 : 1.0, sv, ex);
+    mup = -sv;
+  }
+  // this wave's output: parity pw, column block cw; lane column j = lane % 16 -> map
+  // 8 cw + j / 2, re / im = j % 2; B rows: chunk ring 4 q + lane / 16, q = 0..63
+  const int pw = wave >> 1, cw = wave & 1;
+  const int j = lane & 15;
+  const int bcol = mg * MG + 8 * cw + (j >> 1);
+  uint32_t Bv[64];  // (hi, lo) of the scaled value
+  double inv;
+  {
+    const float* Vb = reinterpret_cast<const float*>(V + (long long)bcol * A.nrings * ringM);
+    const int ri = j & 1;
+    float mx = 0.f;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
+      double u = 0.0;
+      if (tr < A.T && bcol < batch) {
+        const double vn = (double)Vb[2 * ((long long)tr * ringM + m) + ri];
+        // the equator ring has no mirror
+        const double vs = (tr < A.T - 1) ? (double)Vb[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri] : 0.0;
+        u = pw ? vn - vs : vn + vs;
+      }
+      const float uf = (float)u;
+      mx = fmaxf(mx, fabsf(uf));
+      Bv[q] = __float_as_uint(uf);
+    }
+    // the map's maximum over its 8 lanes (re / im x 4 ring residues): bits 0, 4 and 5
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    int e = 1;
+    if (mx > 0.f && mx < 3.0e38f) (void)frexpf(4096.f / mx, &e);
+    e = min(max(e - 1, -120), 120);
+    const float sB = ldexpf(1.f, e);
+    inv = 1.0 / (sA * (double)sB);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) Bv[q] = split_f16(__uint_as_float(Bv[q]) * sB);
+  }
+  float* pcol = reinterpret_cast<float*>(part + ((long long)chunk * batch + bcol) * A.K) + (j & 1);
+  uint32_t* mytile = &tiles[wave][0][0];
+  const int tcol = s2_tile_col(lane);
+  const uint32_t* abase = &tiles[0][0][0] + (16 * pw + (lane & 15)) * TS2 + 16 * (lane >> 4);
+  for (int l0 = m; l0 <= lmax; l0 += CH) {
+    __syncthreads();
+    if (tid < CH) {
+      const int l = l0 + tid;
+      s_al[tid] = (l <= lmax && l != m) ? A.alpha[cb + l] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += TL) {
+      int act = 0;
+#pragma unroll 1
+      for (int j0 = 0; j0 < TL; j0 += 8) {
+        const int anydeep = __ballot(valid && ex < 0) != 0ull;
+        act |= __ballot(valid && ex == 0) != 0ull;
+        const double f = (ex == 0 && valid) ? sA : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const double c = s_al[jb + j0 + jj] * x;
+          const double mn = fma(c, mu, -mup);
+          mup = mu;
+          mu = mn;
+          mytile[s2_tile_row(j0 + jj) * TS2 + tcol] = split_f16((float)(mn * f));
+        }
+        if (anydeep) rescale_pair(mu, mup, ex);
+      }
+      if (lane == 0) s_act[wave] = act;
+      __syncthreads();
+      // byte selectors of (hi, hi) and (lo, lo); pz is always 0 but not provably so, which
+      // keeps the operands built from the loop-invariant Bv inside the loop
+      const uint32_t pz = (uint32_t)(l0 + jb > (1 << 30));
+      const uint32_t sel_hh = 0x01000100u | pz, sel_ll = 0x03020302u | pz;
+      floatx4 D[NW][2];
+#pragma unroll
+      for (int rb = 0; rb < NW; ++rb) D[rb][0] = D[rb][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int rb = 0; rb < NW; ++rb) {
+        if (!s_act[rb]) continue;
+        uint4 a[4];  // this lane's row: rings 4 n + lane/16, n = 0..15, (hi, lo)
+        const uint4* src = reinterpret_cast<const uint4*>(abase + rb * TL * TS2);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) a[n] = src[n];
+        // instruction n: rings 4 (2 n) + k and 4 (2 n + 1) + k; chain n / 4 (block halves)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+          const uint32_t w0 = Bv[16 * rb + 2 * n], w1 = Bv[16 * rb + 2 * n + 1];
+          const halfx8 bq = __builtin_bit_cast(
+              halfx8, make_uint4(__builtin_amdgcn_perm(w0, w0, sel_hh), __builtin_amdgcn_perm(w0, w0, sel_ll),
+                                 __builtin_amdgcn_perm(w1, w1, sel_hh), __builtin_amdgcn_perm(w1, w1, sel_ll)));
+          const uint4 av4 = a[n >> 1];
+          const uint32_t a0 = (n & 1) ? av4.z : av4.x, a1 = (n & 1) ? av4.w : av4.y;
+          D[rb][n >> 2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1)), bq, D[rb][n >> 2], 0, 0, 0);
+        }
+      }
+      // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + jb + 2 i + pw
+      if (bcol < batch) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + pw;
+          double v = 0.0;
+#pragma unroll
+          for (int rb = 0; rb < NW; ++rb) v += (double)D[rb][0][r] + (double)D[rb][1][r];
+          if (l <= lmax) pcol[2 * (cb + l)] = (float)(v * inv);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace
 
 RecArgs make_args_mixed(const jhtx_plan* p) {
@@ -2213,7 +2351,9 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   const int MG = mixed_group(p, true);
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 0 && MG == 16)
+  if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
+    hipLaunchKernelGGL(k_adj_mx_s0_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
+  else if (p->spin == 0 && MG == 16)
     hipLaunchKernelGGL(k_adj_mx_s0_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);

@@ -328,6 +328,37 @@ def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
     assert ea <= MIXED_TOL
 
 
+@pytest.mark.parametrize("f16", [1, 0])
+@pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
+                                                  (128, 256, 16, 1e25), (512, 640, 32, 1.0)])
+def test_mixed_f16_split_adjoint_spin0(nside, lmax, batch, amp, f16, monkeypatch):
+    """The 16-map spin-0 column-block adjoint on the FP16 matrix cores (f16 1) and on the
+    FP32 ones (f16 0) against the FP64 adjoint: ragged groups, inputs far from unit scale,
+    an all-zero map (exact zeros), and a grid with 4 ring chunks."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    monkeypatch.setenv("JHTX_MX_ADJ_F16", str(f16))
+    H.clear_plans()
+    try:
+        rng = np.random.default_rng(4700 + nside + batch)
+        v = rng.standard_normal((batch, 12 * nside * nside)) * amp
+        v[1] = 0.0
+        vt = torch.tensor(v, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            b64 = j.adjoint_synthesis_batch(vt, nside, lmax, 0, precision="f64").cpu().numpy()
+            bmx = j.adjoint_synthesis_batch(vt, nside, lmax, 0, precision="mixed").cpu().numpy()
+            plan = H.get_plan(nside, lmax, 0, batch, H._prec("mixed"))
+            assert plan.tuning["mx_adj_f16"] == (1 if f16 else 2)
+    finally:
+        H.clear_plans()
+    assert np.all(bmx[1] == 0.0)
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch) if i != 1)
+    print(f"mixed spin-0 adjoint f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: {ea:.2e}", flush=True)
+    assert ea <= MIXED_TOL
+
+
 @pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ",
                          [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2),
                           (16, 2, 16, 2), (16, 2, 4, 1)])
