@@ -5,7 +5,9 @@ shapes (``jhtx_tuning`` fields: rings per lane of the FP64 recursion kernels, ma
 recursion, register budget and ring-chunk width of the mixed-precision kernels) with the library's
 own HIP-event stage timers, picks the fastest synthesis and adjoint shapes independently
 (they are separate kernels), and registers the winner so later plans of that geometry use
-it.  Results are exact for every candidate: only speed differs.
+it.  FP64 candidates give identical results (the same arithmetic in another shape); mixed
+candidates, which include the FP16-split matrix-core kernels, all stay within the mixed
+path's 1e-5 contract but differ in their last bits.
 """
 
 from __future__ import annotations
