@@ -230,6 +230,9 @@ def test_tuning_validation_without_gpu():
     bad = L.Tuning(mx_f16=3)
     rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
+    bad = L.Tuning(mx_adj_f16=3)
+    rc = lib.jhtx_plan_create_tuned(16, 24, 0, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
+    assert rc == L.JHTX_EINVAL and not h.value
 
 
 def test_offgrid_entry_validation_without_gpu():
