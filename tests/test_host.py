@@ -312,3 +312,18 @@ def test_enable_compilation_cache(tmp_path, monkeypatch):
     assert (tmp_path / "cache").is_dir()
     with pytest.raises(ValueError):
         jht_amd.enable_compilation_cache(str(tmp_path), min_compile_time_secs=-1)
+
+
+def test_tuning_struct_matches_header():
+    """The ctypes jhtx_tuning mirror has the header's fields in the header's order (the
+    struct crosses the C ABI by layout)."""
+    import re
+
+    from jht_amd import _lib as L
+
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "jhtx.h")).read()
+    body = re.search(r"typedef struct jhtx_tuning \{(.*?)\} jhtx_tuning;", hdr, re.S).group(1)
+    fields = [f.strip() for decl in re.findall(r"int ([^;]+);", body) for f in decl.split(",")]
+    assert tuple(fields) == L.TUNING_FIELDS
+    assert [f[0] for f in L.Tuning._fields_] == fields
+    assert ctypes.sizeof(L.Tuning) == 4 * len(fields)
