@@ -506,8 +506,9 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
     const long long ss = ring_start(A.nside, rs);
     for (int j = tid; j < N; j += BLOCK) {
       const cx<T> z = out[j];
-      mb[sn + j] = (double)z.x;
-      if (pair) mb[ss + j] = (double)z.y;
+      // streamed: the map is not read again by this launch
+      __builtin_nontemporal_store((double)z.x, &mb[sn + j]);
+      if (pair) __builtin_nontemporal_store((double)z.y, &mb[ss + j]);
     }
   } else {
     const cx<T>* Fpb = F + (long long)b * 2 * A.nrings * ringM;
@@ -521,8 +522,8 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
     const long long st = ring_start(A.nside, r);
     for (int j = tid; j < N; j += BLOCK) {
       const cx<T> z = out[j];
-      Q[st + j] = (double)z.x;
-      U[st + j] = (double)z.y;
+      __builtin_nontemporal_store((double)z.x, &Q[st + j]);
+      __builtin_nontemporal_store((double)z.y, &U[st + j]);
     }
   }
 }
