@@ -552,7 +552,10 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   const long long ss = ring_start(A.nside, rs);
   if constexpr (SPIN == 0) {
     const double* mb = map + (long long)b * A.npix;
-    for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)mb[sn + j], (T)(pair ? mb[ss + j] : 0.0));
+    // the map is read once: streamed past the caches
+    for (int j = tid; j < N; j += BLOCK)
+      l2[j] = mk2((T)__builtin_nontemporal_load(&mb[sn + j]),
+                  (T)(pair ? __builtin_nontemporal_load(&mb[ss + j]) : 0.0));
     __syncthreads();
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
@@ -577,7 +580,8 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
     const bool h = blockIdx.x & 1;
     const int r = h ? rs : rn;
     const long long st = h ? ss : sn;
-    for (int j = tid; j < N; j += BLOCK) l2[j] = mk2((T)Q[st + j], (T)U[st + j]);
+    for (int j = tid; j < N; j += BLOCK)
+      l2[j] = mk2((T)__builtin_nontemporal_load(&Q[st + j]), (T)__builtin_nontemporal_load(&U[st + j]));
     __syncthreads();
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
