@@ -391,7 +391,7 @@ def main():
     # the 16-map spin-2 kernels on the FP16 matrix cores (hi + lo split)
     tile16 = prec != _lib.PREC_F64 and spin == 2 and B >= 16
     f16 = {"synth_rec": tile16 and tun["mx_f16"] == 1,
-           "adj_rec": tile16 and tun["mx_adj_f16"] == 1 and tun["mx_adj_cb"] == 1 and tun["mx_adj_wide"] != 1}
+           "adj_rec": tile16 and tun["mx_adj_f16"] == 1 and tun["mx_adj_cb"] == 1}
     peak_basis = None
     if fp64_kernels:
         flops, peak, bound = tri * (26 if spin else 8) * B, PEAK_FP64_TFLOPS, "mfma"

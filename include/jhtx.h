@@ -55,7 +55,8 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
 
 /* Kernel shape choices of a plan (no counterpart in the reference: jht leaves tiling to
  * XLA).  A zero field keeps the library's default; jht_amd's autotuner times candidates
- * and passes the winner here.  Allowed values:
+ * and passes the winner here.  This struct is the only way to change a plan's kernels: the
+ * library reads no environment variables.  Allowed values:
  *   synth_rings, adj_rings   rings per lane of the FP64 recursion kernels: spin 0 {1,4,8},
  *                            spin 2 {1,2,4}
  *   block                    FP64 recursion work-group size cap {64,128,192,256}
@@ -64,16 +65,13 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            and 32, else 8
  *   mx_synth_occ             its register budget in work-groups per CU {1,2,3} (spin 2)
  *   mx_adj_group, mx_adj_occ the same for the mixed adjoint {4,8,16}, {1,2}
- *   mx_adj_wide              mixed spin-2 adjoint with 512-ring work-groups {1 on, 2 off}
- *   f64_adj_mfma             FP64 spin-2 adjoint contracted on the FP64 matrix cores
- *                            {1 on, 2 off (default): the vector-ALU kernel with adj_rings}
  *   mx_adj_cb                16-map mixed spin-2 adjoint in column-block form (each wave
  *                            contracts 4 maps over the work-group's rings; the same bits)
  *                            {1 on (default), 2 off: per-wave tiles and a cross-wave sum}
  *   mx_single                one-map calls of a mixed plan run the FP64 vector-FMA
  *                            recursion kernels on complex64 ring coefficients, with the
  *                            FP32 ring FFT {1 on (default), 2 off: the 4-map matrix-core
- *                            kernels}.  f64_adj_mfma does not apply to mixed plans.
+ *                            kernels}.
  *   mx_f16                   16-map mixed synthesis (spin 2 and spin 0) on the FP16 matrix cores, each
  *                            FP32 operand split into hi + lo halves after a power-of-two
  *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles}
@@ -83,8 +81,7 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
-  int mx_adj_group, mx_adj_occ, mx_adj_wide;
-  int f64_adj_mfma;
+  int mx_adj_group, mx_adj_occ;
   int mx_adj_cb;
   int mx_single;
   int mx_f16;

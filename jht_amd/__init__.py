@@ -29,7 +29,6 @@ from .offgrid import (  # noqa: F401
     synthesis_general,
     synthesis_general_real,
 )
-from .cache import enable_compilation_cache  # noqa: F401
 from .tune import autotune  # noqa: F401
 
 from .callers import (  # noqa: F401

@@ -1,22 +1,27 @@
-"""Build ``jht_amd/libjhtx.so`` (gfx950) in-tree with hipcc.  ``python -m jht_amd.build``."""
+"""Build ``jht_amd/libjhtx.so`` (gfx950) in-tree with hipcc.  ``python -m jht_amd.build``.
+
+Each source compiles to its own object in parallel (no device code crosses files), then one
+link step makes the shared library.
+"""
 
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["jhtx_api.hip", "jhtx_rec.hip", "jhtx_fft.hip", "jhtx_mixed.hip", "jhtx_offgrid.hip", "jhtx_peaks.hip"]
 OUT = os.path.join(HERE, "libjhtx.so")
+OBJDIR = os.path.join(HERE, "_build")
 
 FLAGS = [
     "-O3",
     "-std=c++17",
     "--offload-arch=gfx950",
     "-fPIC",
-    "-shared",
     # explicit fma() only: keeps the host tables and the seed arithmetic bit-reproducible
     "-ffp-contract=off",
     # never emit packed FP32 arithmetic
@@ -26,16 +31,26 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc, *FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {cmd[-1]}")
+    return res
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [[hipcc, *FLAGS, "-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(SOURCES, objs)]
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = OUT + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -332,85 +332,48 @@ int check_call(const jhtx_plan* p, const void* in, void* out, int batch, void* w
   return JHTX_OK;
 }
 
-// Kernel shapes (defaults, environment overrides, explicit tuning) and workspace sizes of
-// a plan whose geometry fields (M, T, nrings, ncomp, K) are set.
+// Kernel shapes (defaults, then an explicit jhtx_tuning) and workspace sizes of a plan whose
+// geometry fields (M, T, nrings, ncomp, K) are set.  Nothing but the tuning argument changes
+// which kernels a plan runs.
 int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
-  // rings per lane and work-group size of the recursion kernels; small grids use one
-  // ring per lane.  JHTX_SYNTH_R / JHTX_ADJ_R override the choice (tuning sweeps).
-  auto pick = [&](const char* env, int big, int small, int thresh, const int* allowed) {
-    int r = (p->T >= thresh) ? big : small;
-    if (const char* v = getenv(env)) {
-      const int want = atoi(v);
-      for (int i = 0; allowed[i]; ++i)
-        if (allowed[i] == want) r = want;
-    }
-    return r;
-  };
-  static const int s0_R[] = {1, 4, 8, 0};
-  static const int s2_R[] = {1, 2, 4, 0};
-  // defaults from sweeps at nside 2048 / lmax 4096 (synthesis: occupancy wins; adjoint:
+  // rings per lane of the FP64 recursion kernels; small grids use one ring per lane.
+  // Defaults from sweeps at nside 2048 / lmax 4096 (synthesis: occupancy wins; adjoint:
   // rings per lane amortise the per-lane ring reduction)
   if (p->spin == 0) {
-    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 512, s0_R);
+    p->rec_R = 1;
     // 8 rings per lane amortise the adjoint's cross-lane reduction over twice the work:
     // 9.31 -> 8.15 ms at nside 2048 / lmax 4096 (2 waves per SIMD, 30 spilled registers)
-    p->adj_R = pick("JHTX_ADJ_R", p->T >= 1024 ? 8 : 4, 1, 1024, s0_R);
+    p->adj_R = p->T >= 1024 ? 8 : 1;
   } else {
-    p->rec_R = pick("JHTX_SYNTH_R", 1, 1, 256, s2_R);
-    p->adj_R = pick("JHTX_ADJ_R", p->T >= 1024 ? 4 : 2, 1, 512, s2_R);
+    p->rec_R = 1;
+    p->adj_R = p->T >= 1024 ? 4 : (p->T >= 512 ? 2 : 1);
   }
-  // mixed spin-2 kernels: maps per recursion (4 or 8) and the register budget of the
-  // 8-map variants (1 or 2 work-groups per CU); defaults from sweeps at nside 2048 / lmax 4096
-  auto pick2 = [](const char* env, int dflt, int a, int b) {
-    int r = dflt;
-    if (const char* v = getenv(env)) {
-      const int want = atoi(v);
-      if (want == a || want == b) r = want;
-    }
-    return r;
-  };
-  // maps per recursion follow the batch the plan is sized for (a partly empty group costs
-  // matrix-core time, a larger one amortises the FP64 recursion over more maps)
-  // (spin 0 can also run 32 maps per recursion, by tuning: at 2 waves per SIMD instead of 3
-  // it measured +3 % at nside 2048 / batch 32 and -2 % on the ragged nside-512 batch)
-  p->mx_synth_group = pick2("JHTX_MX_SYNTH_GROUP", p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4), 4, 8);
-  if (const char* v = getenv("JHTX_MX_SYNTH_GROUP")) {
-    if (atoi(v) == 16) p->mx_synth_group = 16;  // the 16x16x4 tile kernel
-    if (atoi(v) == 32 && p->spin == 0) p->mx_synth_group = 32;
-  }
-  p->mx_synth_occ = pick2("JHTX_MX_SYNTH_OCC", 2, 1, 2);
-  if (const char* v = getenv("JHTX_MX_SYNTH_OCC")) {
-    if (atoi(v) == 3) p->mx_synth_occ = 3;
-  }
-  p->mx_adj_group = pick2("JHTX_MX_ADJ_GROUP", p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4), 4, 8);
-  if (const char* v = getenv("JHTX_MX_ADJ_GROUP")) {
-    if (atoi(v) == 16) p->mx_adj_group = 16;
-  }
-  p->mx_adj_occ = pick2("JHTX_MX_ADJ_OCC", 2, 1, 2);
-  p->mx_adj_wide = pick2("JHTX_MX_ADJ_WIDE", 0, 0, 1);
+  // maps per recursion of the mixed kernels follow the batch the plan is sized for (a partly
+  // empty group costs matrix-core time, a larger one amortises the FP64 recursion over more
+  // maps); spin 0 can also run 32 maps per recursion, by tuning: at 2 waves per SIMD instead
+  // of 3 it measured +3 % at nside 2048 / batch 32 and -2 % on the ragged nside-512 batch
+  const int group = p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4);
+  p->mx_synth_group = group;
+  p->mx_synth_occ = 2;  // register budget of the 8-map kernels, in work-groups per CU
+  p->mx_adj_group = group;
+  p->mx_adj_occ = 2;
   // the 16-map adjoint in column-block form (1) or with per-wave tiles summed across waves (0)
-  p->mx_adj_cb = pick2("JHTX_MX_ADJ_CB", 1, 0, 1);
-  p->f64_adj_mfma = (p->spin == 2 && p->precision != JHTX_PREC_MIXED) ? pick2("JHTX_F64_ADJ_MFMA", 0, 0, 1) : 0;
+  p->mx_adj_cb = 1;
   // One-map calls of a mixed plan: a matrix-core group of 4 maps would leave 3/4 of its
   // columns empty, and one 16x16x4 instruction per 64 (ring, l) pairs then costs 4x the
   // two FP64 FMAs per pair of the vector contraction.  Such calls run the FP64 recursion
   // kernels on complex64 F / V instead (the ring FFT stays FP32).
-  p->mx_single = pick2("JHTX_MX_SINGLE", 1, 0, 1);
-  // 16-map spin-2 synthesis on the FP16 matrix cores (hi + lo halves of every operand, all
-  // four products: half the matrix instructions of FP32): 77.5 -> 62.1 ms at nside 2048 /
+  p->mx_single = 1;
+  // 16-map synthesis on the FP16 matrix cores (hi + lo halves of every operand, all four
+  // products: half the matrix instructions of FP32): spin 2 77.5 -> 62.1 ms at nside 2048 /
   // lmax 4096, batch 16, error against FP64 4.7e-7 -> 1.0e-6
-  p->mx_f16 = pick2("JHTX_MX_F16", 1, 0, 1);
+  p->mx_f16 = 1;
   // the same split in the 16-map column-block adjoint: spin 2 81.0 -> 59.6 ms at nside 2048
   // / lmax 4096, batch 16, error against FP64 1.5e-7 -> 1.6e-7, default on.  Spin 0: 26.4 ->
   // 23.9 ms, but its l >= 2048 band error rises from 2.5e-6 to 6.1e-6: off by default
-  p->mx_adj_f16 = pick2("JHTX_MX_ADJ_F16", p->spin == 2 ? 1 : 0, 0, 1);
+  p->mx_adj_f16 = p->spin == 2 ? 1 : 0;
   int block_cap = 256;
-  if (const char* v = getenv("JHTX_REC_BLOCK")) {
-    const int b = atoi(v);
-    if (b == 64 || b == 128 || b == 192 || b == 256) block_cap = b;
-  }
   if (tuning) {
-    // explicit shapes override the defaults and the environment
     auto in = [](int v, std::initializer_list<int> ok) {
       for (int o : ok)
         if (v == o) return true;
@@ -423,11 +386,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
         (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16}) || (p->spin == 0 && t.mx_synth_group == 32)) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
-        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) &&
-        (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) && (t.mx_adj_wide == 0 || in(t.mx_adj_wide, {1, 2})) &&
-        (t.f64_adj_mfma == 0 || in(t.f64_adj_mfma, {1, 2})) && (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) &&
-        (t.mx_single == 0 || in(t.mx_single, {1, 2})) && (t.mx_f16 == 0 || in(t.mx_f16, {1, 2})) &&
-        (t.mx_adj_f16 == 0 || in(t.mx_adj_f16, {1, 2}));
+        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) && (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) &&
+        (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) && (t.mx_single == 0 || in(t.mx_single, {1, 2})) &&
+        (t.mx_f16 == 0 || in(t.mx_f16, {1, 2})) && (t.mx_adj_f16 == 0 || in(t.mx_adj_f16, {1, 2}));
     if (!ok) {
       set_error("jhtx_tuning field out of range");
       return JHTX_EINVAL;
@@ -439,8 +400,6 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
     if (t.mx_synth_occ) p->mx_synth_occ = t.mx_synth_occ;
     if (t.mx_adj_group) p->mx_adj_group = t.mx_adj_group;
     if (t.mx_adj_occ) p->mx_adj_occ = t.mx_adj_occ;
-    if (t.mx_adj_wide) p->mx_adj_wide = (t.mx_adj_wide == 1);
-    if (t.f64_adj_mfma && p->spin == 2 && p->precision != JHTX_PREC_MIXED) p->f64_adj_mfma = (t.f64_adj_mfma == 1);
     if (t.mx_adj_cb) p->mx_adj_cb = (t.mx_adj_cb == 1);
     if (t.mx_single) p->mx_single = (t.mx_single == 1);
     if (t.mx_f16) p->mx_f16 = (t.mx_f16 == 1);
@@ -457,16 +416,14 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // ring coefficients F / V: complex128, or complex64 for the mixed path (FP32 ring FFT)
   const size_t f_elem = (p->precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
   p->ws_F_bytes = align256(f_elem * (size_t)p->max_batch * p->ncomp * p->nrings * p->M);
-  // the mixed kernels use one ring per lane and 256-lane work-groups
-  // the FP64 matrix-core adjoint writes one partial per 256 rings
-  // (a mixed plan's one-map calls through the FP64 kernels keep FP64 partials)
+  // the mixed kernels use one ring per lane and 256-lane work-groups (a mixed plan's one-map
+  // calls through the FP64 kernels keep FP64 partials)
   size_t part_bytes;
   if (p->precision == JHTX_PREC_MIXED) {
     part_bytes = sizeof(float2) * (size_t)jhtx::mixed_adj_chunks(p) * p->max_batch * p->ncomp * p->K;
     if (p->mx_single) part_bytes = std::max<size_t>(part_bytes, sizeof(double2) * (size_t)p->adj_nchunk * p->ncomp * p->K);
   } else {
-    const int part_chunks = p->f64_adj_mfma ? std::max(p->adj_nchunk, (p->T + 255) / 256) : p->adj_nchunk;
-    part_bytes = sizeof(double2) * (size_t)part_chunks * p->max_batch * p->ncomp * p->K;
+    part_bytes = sizeof(double2) * (size_t)p->adj_nchunk * p->max_batch * p->ncomp * p->K;
   }
   p->ws_part_bytes = align256(part_bytes);
   p->ws_scale_bytes = (p->precision == JHTX_PREC_MIXED)
@@ -502,8 +459,6 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->mx_synth_occ = p->mx_synth_occ;
   t->mx_adj_group = p->mx_adj_group;
   t->mx_adj_occ = p->mx_adj_occ;
-  t->mx_adj_wide = p->mx_adj_wide ? 1 : 2;
-  t->f64_adj_mfma = p->f64_adj_mfma ? 1 : 2;
   t->mx_single = p->mx_single ? 1 : 2;
   t->mx_f16 = p->mx_f16 ? 1 : 2;
   t->mx_adj_f16 = p->mx_adj_f16 ? 1 : 2;

@@ -705,21 +705,13 @@ void launch_one(bool synth, const FftArgs& a, dim3 grid, size_t sh, hipStream_t 
 
 // threads of a launch class: the FP32 path runs the L <= 16384 chirp-z class as
 // 1024 threads x 16 elements (the FP64 path as 512 x 32: its data would not fit 1024 lanes)
-// (JHTX_FFT_WIDE=0 keeps 512 x 32 for both: a tuning switch)
-inline bool fft_wide_big() {
-  static const bool wide = [] {
-    const char* v = getenv("JHTX_FFT_WIDE");
-    return !(v && atoi(v) == 0);
-  }();
-  return wide;
-}
-inline int class_block(int lc, bool f32) { return (lc == 3 && f32 && fft_wide_big()) ? 1024 : kBlocks[lc]; }
+inline int class_block(int lc, bool f32) { return (lc == 3 && f32) ? 1024 : kBlocks[lc]; }
 
 template <int SPIN, typename T>
 void launch_class(bool synth, int lc, int kind, const FftArgs& a, dim3 grid, size_t sh,
                   hipStream_t s, const void* in, void* out) {
   if constexpr (sizeof(T) == 4) {  // FP64 at 1024 x 16 spills (128-register cap): 512 x 32
-    if (lc == 3 && fft_wide_big()) {
+    if (lc == 3) {
       launch_one<1024, 16, SPIN, 2, T>(synth, a, grid, sh, s, in, out);
       return;
     }

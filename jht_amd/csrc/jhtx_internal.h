@@ -99,12 +99,10 @@ struct jhtx_plan {
   int rec_R;      // rings per lane (synthesis)
   int adj_block, adj_R, adj_nchunk;  // the same for the adjoint recursion
   int block_cap;                     // work-group size cap of the FP64 recursion kernels
-  int f64_adj_mfma;                  // FP64 spin-2 adjoint on the FP64 matrix cores (0 or 1)
   int mx_synth_group;  // maps per work-group of the mixed spin-2 synthesis (4 or 8)
   int mx_synth_occ;    // register budget of the 8-map synthesis: 1 or 2 work-groups per CU
   int mx_adj_group;    // the same for the mixed spin-2 adjoint
   int mx_adj_occ;      // register budget of the 8-map adjoint: 1 or 2 work-groups per CU
-  int mx_adj_wide;     // 8-map adjoint with 512-ring work-groups on large grids (0 or 1)
   int mx_adj_cb;       // 16-map adjoint in column-block form (0 or 1)
   int mx_single;       // mixed plans: one-map calls run the FP64 vector-FMA recursion kernels
                        // on complex64 F / V (0 or 1)

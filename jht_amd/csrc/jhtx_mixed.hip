@@ -1508,22 +1508,18 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
   }
 }
 
-// Column-block form of the 16-map spin-2 adjoint: NH halves of 256 rings, 4 NH waves, one
-// chunk partial per 256 NH rings.  Wave w runs the recursion of rings 64 w .. 64 w + 63 of
-// the chunk into its tile, as in k_adj_mx_s2; after a barrier it contracts column block
-// w % 4 (maps 4 (w % 4) .. 4 (w % 4) + 3 of the group) over the 256 rings of half w / 4,
-// reading that half's four tiles.  Every wave then does the same matrix work and skips the
-// same all-zero (ring block, sign) tiles.  With NH = 1 its accumulators are the chunk
-// partials of its columns; with NH = 2 the two halves' FP64 sums meet in LDS (half 0 + half
-// 1), which halves the partials that k_adj_finish reads.
-template <int NH>
-__global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
-                                                                           float2* __restrict__ part, int batch) {
-  constexpr int NW = 4 * NH, MG = 16, RINGS = 256 * NH;
+// Column-block form of the 16-map spin-2 adjoint: 256 rings, 4 waves, one chunk partial per
+// 256 rings.  Wave w runs the recursion of rings 64 w .. 64 w + 63 of the chunk into its
+// tile, as in k_adj_mx_s2; after a barrier it contracts column block w (maps 4 w .. 4 w + 3
+// of the group) over the 256 rings, reading all four tiles.  Every wave then does the same
+// matrix work and skips the same all-zero (ring block, sign) tiles, and its accumulators are
+// the chunk partials of its columns.
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
+                                                        float2* __restrict__ part, int batch) {
+  constexpr int NW = 4, MG = 16, RINGS = 256;
   __shared__ double2 s_ag[CH];
   __shared__ __align__(16) float tiles[NW][2][TL][TS2];  // [ring block][sign][row][col]
   __shared__ int s_act[NW];
-  __shared__ double s_pair[NH == 2 ? 4 * 2 * 4 * 64 : 1];  // [column block][p][r][lane]: half 1's sums
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -1547,9 +1543,9 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
     seed_split(spin2_seed_L(sd + 4, lc, ls), sd[7], s, em);
     pmp = -s;
   }
-  // B operands of column block cw, half hw: chunk ring 256 hw + 4 q + lane/16 (q = 0..63),
+  // B operands of column block cw: chunk ring 4 q + lane/16 (q = 0..63),
   // column lane%16 -> map 4 cw + (lane%16)/4, component lane%4 (see k_adj_mx_s2)
-  const int cw = wave & 3, hw = wave >> 2;
+  const int cw = wave;
   const int g = (lane & 15) >> 2, cc = lane & 3;
   const int bcol = mg * MG + 4 * cw + g;  // the map of this lane's column
   float Bv[2][64];                          // [sign][q]
@@ -1561,7 +1557,7 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
     const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      const int tr = chunk * RINGS + 256 * hw + 4 * q + (lane >> 4);
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
       double bp = 0.0, bm = 0.0;
       if (tr < A.T && bcol < batch) {
         const long long rn = 2 * ((long long)tr * ringM + m) + ri;
@@ -1632,7 +1628,7 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
       __syncthreads();
       int actv[4];  // this half's ring blocks
 #pragma unroll
-      for (int w = 0; w < 4; ++w) actv[w] = s_act[4 * hw + w];
+      for (int w = 0; w < 4; ++w) actv[w] = s_act[w];
       // D[p][rb]: parity p, ring block rb.  Each ring block accumulates in FP32 over its 64
       // rings (both signs) and the four are summed in FP64, in ring-block order: the
       // arithmetic of the per-wave tiles and FP64 cross-wave sum of k_adj_mx_s2 (a single
@@ -1651,7 +1647,7 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
             const floatx4* src =
-                reinterpret_cast<const floatx4*>(abase + (((4 * hw + rb) * 2 + sg) * TL + 16 * p) * TS2);
+                reinterpret_cast<const floatx4*>(abase + ((rb * 2 + sg) * TL + 16 * p) * TS2);
 #pragma unroll
             for (int n = 0; n < 4; ++n) a[p][n] = src[n];
           }
@@ -1672,28 +1668,17 @@ __global__ __launch_bounds__(256 * NH, NH == 1 ? 2 : 1) void k_adj_mx_s2_cb(RecA
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb) v[p][r] += (double)D[p][rb][r];
         }
-      if constexpr (NH == 2) {
-        // half 1 publishes its sums; this barrier also retires every wave's tile reads
-        if (hw == 1) {
-#pragma unroll
-          for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) s_pair[((cw * 2 + p) * 4 + r) * 64 + lane] = v[p][r];
-        }
-        __syncthreads();
-      }
-      if (hw == 0 && bcol < batch) {
+      if (bcol < batch) {
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int l = l0 + jb + 2 * (4 * (lane >> 4) + r) + p;
-            double t = v[p][r];
-            if constexpr (NH == 2) t += s_pair[((cw * 2 + p) * 4 + r) * 64 + lane];
+            const double t = v[p][r];
             if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * (float)t : (float)t;
           }
       }
-      if constexpr (NH == 1) __syncthreads();
+      __syncthreads();
     }
   }
 }
@@ -2283,16 +2268,8 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
   return a;
 }
 
-// waves per work-group of the mixed adjoint: 8 (512 rings per partial) for the 8-map
-// spin-2 kernel on large grids, else 4
-int mixed_adj_waves(const jhtx_plan* p) {
-  return (p->spin == 2 && p->mx_adj_group >= 8 && p->mx_adj_wide && p->T >= 1024) ? 8 : 4;
-}
-
-int mixed_adj_chunks(const jhtx_plan* p) {
-  const int rings = 64 * mixed_adj_waves(p);
-  return (p->T + rings - 1) / rings;
-}
+// ring chunks of the mixed adjoint: every kernel's work-group covers 256 rings
+int mixed_adj_chunks(const jhtx_plan* p) { return (p->T + 255) / 256; }
 
 int mixed_group(const jhtx_plan* p, bool adjoint) {
   if (p->spin == 0) {
@@ -2357,18 +2334,12 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
     hipLaunchKernelGGL(k_adj_mx_s0_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
-  else if (MG == 16 && mixed_adj_waves(p) == 8 && p->mx_adj_cb)
-    hipLaunchKernelGGL(k_adj_mx_s2_cb<2>, grid, dim3(512), 0, s, a, V, part, batch);
-  else if (MG == 16 && mixed_adj_waves(p) == 8)
-    hipLaunchKernelGGL((k_adj_mx_s2<4, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 16 && p->mx_adj_cb && p->mx_adj_f16)
     hipLaunchKernelGGL(k_adj_mx_s2_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
   else if (MG == 16 && p->mx_adj_cb)
-    hipLaunchKernelGGL(k_adj_mx_s2_cb<1>, grid, dim3(256), 0, s, a, V, part, batch);
+    hipLaunchKernelGGL(k_adj_mx_s2_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16)
     hipLaunchKernelGGL((k_adj_mx_s2<4, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
-  else if (MG == 8 && mixed_adj_waves(p) == 8)
-    hipLaunchKernelGGL((k_adj_mx_s2<2, 1, 8>), grid, dim3(512), 0, s, a, V, part, batch);
   else if (MG == 8 && p->mx_adj_occ == 2)
     hipLaunchKernelGGL((k_adj_mx_s2<2, 2, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 8)

@@ -691,182 +691,12 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const C* __restrict__
   }
 }
 
-// ---------------------------------------------------------------------------------- //
-// spin 2 adjoint on the FP64 matrix cores: v_mfma_f64_4x4x4_4b_f64
-// ---------------------------------------------------------------------------------- //
-// One ring per lane, 256 rings per work-group.  Every 8 l-steps the wave writes its +-2
-// recursion values (FP64, masked) to an LDS tile [sign][step][ring].  The instruction holds
-// 4 blocks; lane = 16 r + 4 b + c, with A[i][k] at (r = k, c = i), B[k][j] at (r = k, c = j)
-// and D[i][j] at (r = i, c = j) of block b (layout measured on the device).  Rows i are the
-// 4 same-parity steps of the group (l = 2 i + p), K the 4 rings 16 q + 4 b + k, columns j
-// (p2.re, p2.im, m2.re, m2.im): per group 2 parities x 2 signs x 4 ring quads = 16
-// instructions for the wave's 64 rings, with the parity sign (-1)^(l-m) of the south terms
-// folded into B.  The 4 blocks are summed by two lane shuffles and the waves through LDS
-// once per 32 steps; the partials per 256-ring chunk are finished by k_adj_finish.
-constexpr int kF64TS = 68;  // tile row stride (doubles)
-
-__device__ __forceinline__ void wave_sync64() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__global__ __launch_bounds__(256) void k_adj_s2_mfma(RecArgs A, const double2* __restrict__ V,
-                                                     double2* __restrict__ part) {
-  constexpr int TLG = 4;  // 8-step groups per reduction (32 steps)
-  __shared__ double2 s_ag[CH];
-  __shared__ double tiles[4][2][8][kF64TS];  // [wave][sign][step][ring]
-  __shared__ double red[2][4][TLG][2][16];   // [buffer][wave][group][parity][4 i + j]
-  int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
-  const int b = blockIdx.y;
-  const int batch = gridDim.y;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane >> 4, lb = (lane >> 2) & 3, lc = lane & 3;
-  const int lmax = A.lmax;
-  const int lmin = max(m, 2);
-  if (lmin > lmax) return;
-  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
-  const long long ringM = A.M;
-  const int t = chunk * 256 + tid;
-  const bool valid = t < A.T;
-  double x = 0.0, pp = 0.0, ppp = 0.0, pm = 0.0, pmp = 0.0;
-  int ep = 0, em = 0;
-  if (valid) {
-    x = A.x[t];
-    const double* sd = A.seed + 8 * m;
-    const double lcs = A.lcos[t], lsn = A.lsin[t];
-    double sv;
-    seed_split(spin2_seed_L(sd, lcs, lsn), sd[3], sv, ep);
-    ppp = -sv;
-    seed_split(spin2_seed_L(sd + 4, lcs, lsn), sd[7], sv, em);
-    pmp = -sv;
-  }
-  // B operands [sign][parity][ring quad]: this lane's ring 16 q + 4 lb + lr, column lc.
-  // sign +: p2 <- Vpn, m2 <- s Vms ; sign -: p2 <- s Vps, m2 <- Vmn ; s = (-1)^(l-m)
-  double Bv[2][2][4];
-  {
-    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;  // (-1)^(l-m) at even (l - lmin)
-    const double* Vp = reinterpret_cast<const double*>(V + (long long)b * 2 * A.nrings * ringM);
-    const double* Vm = reinterpret_cast<const double*>(V + ((long long)b * 2 + 1) * A.nrings * ringM);
-    const bool lo = lc < 2;
-    const int ri = lc & 1;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int tr = chunk * 256 + wave * 64 + 16 * q + 4 * lb + lr;
-      double bn = 0.0, bs = 0.0, cn = 0.0, cs = 0.0;  // sign + (north, south) ; sign - (north, south)
-      if (tr < A.T) {
-        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
-        const bool south = tr < A.T - 1;  // the equator ring has no mirror
-        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
-        if (lo) {
-          bn = Vp[rn];
-          cs = south ? Vp[rs] : 0.0;
-        } else {
-          bs = south ? Vm[rs] : 0.0;
-          cn = Vm[rn];
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const double sp = p ? -par0 : par0;
-        Bv[0][p][q] = bn + sp * bs;
-        Bv[1][p][q] = cn + sp * cs;
-      }
-    }
-  }
-  double2* pb2 = part + (((long long)chunk * batch + b) * 2 + 0) * A.K + cb;
-  double2* pm2 = part + (((long long)chunk * batch + b) * 2 + 1) * A.K + cb;
-  double* mytile = &tiles[wave][0][0][0];
-  int buf = 0;
-  double2 pf_ag = make_double2(0.0, 0.0);
-  auto prefetch = [&](int l0n) {
-    const int l = l0n + tid;
-    pf_ag = make_double2(0.0, 0.0);
-    if (tid < CH && l <= lmax && l != lmin) pf_ag = make_double2(A.alpha[cb + l], A.gamma[cb + l]);
-  };
-  prefetch(lmin);
-  for (int l0 = lmin; l0 <= lmax; l0 += CH) {
-    __syncthreads();
-    if (tid < CH) s_ag[tid] = pf_ag;
-    __syncthreads();
-    if (l0 + CH <= lmax) prefetch(l0 + CH);
-#pragma unroll 1
-    for (int jb = 0; jb < CH && l0 + jb <= lmax; jb += 8 * TLG) {
-      double D[TLG][2];
-#pragma unroll
-      for (int g = 0; g < TLG; ++g) {
-        D[g][0] = D[g][1] = 0.0;
-        const int j0 = jb + 8 * g;
-        const int anydeep = __ballot(valid && ((ep < 0) || (em < 0))) != 0ull;
-        const double fp = (ep == 0 && valid) ? 1.0 : 0.0, fm = (em == 0 && valid) ? 1.0 : 0.0;
-        double2 ag[8];
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[j0 + jj];
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
-          const double np = fma(fma(ag[jj].x, x, ag[jj].y), pp, -ppp);
-          const double nm = fma(fma(ag[jj].x, x, -ag[jj].y), pm, -pmp);
-          ppp = pp;
-          pp = np;
-          pmp = pm;
-          pm = nm;
-          mytile[(0 * 8 + jj) * kF64TS + lane] = np * fp;
-          mytile[(1 * 8 + jj) * kF64TS + lane] = nm * fm;
-        }
-        if (anydeep) {
-          rescale_pair(pp, ppp, ep);
-          rescale_pair(pm, pmp, em);
-        }
-        wave_sync64();
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int sg = 0; sg < 2; ++sg)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const double av = mytile[(sg * 8 + 2 * lc + p) * kF64TS + 16 * q + 4 * lb + lr];
-              D[g][p] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, Bv[sg][p][q], D[g][p], 0, 0, 0);
-            }
-        wave_sync64();
-      }
-      // sum the 4 blocks (lanes differing in bits 2, 3); lanes with lb == 0 keep (i, j)
-#pragma unroll
-      for (int g = 0; g < TLG; ++g)
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          double v = D[g][p];
-          v += __shfl_xor(v, 4);
-          v += __shfl_xor(v, 8);
-          if (lb == 0) red[buf][wave][g][p][4 * lr + lc] = v;
-        }
-      __syncthreads();
-      // thread -> (group, parity, row i, p2 / m2): sum the 4 waves, store the chunk partial
-      if (tid < TLG * 2 * 4 * 2) {
-        const int half = tid & 1, i = (tid >> 1) & 3, p = (tid >> 3) & 1, g = tid >> 4;
-        const int l = l0 + jb + 8 * g + 2 * i + p;
-        if (l <= lmax) {
-          double re = 0.0, im = 0.0;
-#pragma unroll
-          for (int w = 0; w < 4; ++w) {
-            re += red[buf][w][g][p][4 * i + 2 * half];
-            im += red[buf][w][g][p][4 * i + 2 * half + 1];
-          }
-          (half ? pm2 : pb2)[l] = make_double2(re, im);
-        }
-      }
-      buf ^= 1;  // the next tile writes the other buffer: no barrier after the reads
-    }
-  }
-}
-
 // Sum the ring-chunk partials, apply G_l (and for spin s != 0 the E/B algebra
 // aE = (-p2 - (-1)^s m2)/2, aB = i (p2 - (-1)^s m2)/2, healpix.py:353-354 for s = 2),
 // write packed a_lm.
 // P: the chunk partials' type -- double2 (FP64 path) or float2 (mixed path, whose partials
-// are sums of FP32 matrix-core tiles); the chunk sum is taken in FP64 in a fixed order
-// Sum the ring-chunk partials in FP64 (chunk order, fixed), apply G_l and the E/B
-// combination.  One thread per (map, packed index): the grid runs over the flat triangle,
+// are sums of FP32 matrix-core tiles).  The chunk sum is taken in FP64 in chunk order
+// (fixed).  One thread per (map, packed index): the grid runs over the flat triangle,
 // so short high-m columns waste no threads, and the chunk loads are issued 8 at a time.
 // Entries with l < |s| are zero; they sit in the columns m < |s|.
 template <typename P>
@@ -1018,13 +848,6 @@ hipError_t launch_synth_rec(const jhtx_plan* p, const double2* alm, float2* F, i
 // the recursion + contraction kernel of the FP64 adjoint; its chunk partials are summed by
 // launch_adj_finish (adj_rec_chunks chunks)
 hipError_t launch_adj_rec(const jhtx_plan* p, const double2* V, double2* part, int batch, hipStream_t s) {
-  if (p->spin == 2 && p->f64_adj_mfma) {
-    RecArgs am = make_args(p, true);
-    am.nchunk = (p->T + 255) / 256;
-    const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
-    hipLaunchKernelGGL(k_adj_s2_mfma, dim3(Mpad * am.nchunk, batch), dim3(256), 0, s, am, V, part);
-    return hipGetLastError();
-  }
   return adj_rec_t(p, V, part, batch, s);
 }
 
@@ -1034,9 +857,7 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const float2* V, double2* part, in
 }
 
 // ring chunks (partials per map and component) of the FP64 adjoint
-int adj_rec_chunks(const jhtx_plan* p) {
-  return (p->spin == 2 && p->f64_adj_mfma) ? (p->T + 255) / 256 : p->adj_nchunk;
-}
+int adj_rec_chunks(const jhtx_plan* p) { return p->adj_nchunk; }
 
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s) {

@@ -15,7 +15,8 @@ from dataclasses import dataclass
 
 import torch
 
-from .healpix import _prec, alm_size, get_plan
+from . import _lib
+from .healpix import Plan, _prec, _tuning_for, alm_size, get_azimuth_fft_mode
 
 
 @dataclass(frozen=True)
@@ -33,6 +34,9 @@ class TransformGraph:
     Fill ``alm[i]`` (``(batch, K)`` or ``(batch, 2, K)`` complex128) in place, call
     :meth:`replay`, read ``maps[i] = S alm[i]`` and, with ``adjoint=True``,
     ``back[i] = S^T maps[i]``.
+
+    Each instance gets a private plan (tables and workspace) that no other call uses, so a
+    replay may run on one stream while eager transforms of the same geometry run on another.
     """
 
     def __init__(self, instances, adjoint: bool = True, device=None):
@@ -42,8 +46,11 @@ class TransformGraph:
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.adjoint = adjoint
         self.plans, self.alm, self.maps, self.back = [], [], [], []
+        flags = _lib.FLAG_BLUESTEIN_ALL if get_azimuth_fft_mode() == "looped" else 0
         for ins in self.instances:
-            plan = get_plan(ins.nside, ins.lmax, ins.spin, ins.batch, _prec(ins.precision), self.device)
+            prec = _prec(ins.precision)
+            plan = Plan(ins.nside, ins.lmax, ins.spin, ins.batch, prec, flags, self.device,
+                        _tuning_for(ins.nside, ins.lmax, ins.spin, prec, ins.batch))
             K, npix = alm_size(ins.lmax), 12 * ins.nside * ins.nside
             ashape = (ins.batch, K) if ins.spin == 0 else (ins.batch, 2, K)
             mshape = (ins.batch, npix) if ins.spin == 0 else (ins.batch, 2, npix)

@@ -131,7 +131,12 @@ def enable_looped_fft() -> None:
 # plans
 # --------------------------------------------------------------------------- #
 class Plan:
-    """A jhtx plan on one device plus its workspace (allocated once, reused per call)."""
+    """A jhtx plan on one device plus its workspace (allocated once, reused per call).
+
+    The workspace belongs to the plan, so calls on one Plan object must not overlap: keep a
+    plan (and the cached ones :func:`get_plan` returns) on one stream, or create a private
+    Plan per stream.
+    """
 
     def __init__(self, nside: int, lmax: int, spin: int, max_batch: int = 1,
                  precision: int = _lib.PREC_F64, flags: int = 0, device=None, tuning=None):
@@ -220,21 +225,29 @@ def _warn_once(nside: int, lmax: int, spin: int) -> None:
         )
 
 
-# kernel shapes chosen for a geometry (jht_amd.tune.autotune or set_tuning)
+# kernel shapes chosen for a geometry, precision and batch (jht_amd.tune.autotune or
+# set_tuning); batch None: any batch without an entry of its own
 _TUNED: dict = {}
 
 
-def set_tuning(nside: int, lmax: int, spin: int, precision: int, tuning) -> None:
-    """Use ``tuning`` (a dict of ``jhtx_tuning`` fields, or None for the defaults) for every
-    plan of this geometry and precision created from now on."""
-    key = (nside, lmax, spin, precision)
+def set_tuning(nside: int, lmax: int, spin: int, precision: int, tuning, batch: int | None = None) -> None:
+    """Use ``tuning`` (a dict of ``jhtx_tuning`` fields, or None for the defaults) for the
+    plans of this geometry and precision created from now on for calls of ``batch`` maps
+    (``None``: every batch that has no tuning of its own).  A winner timed at one batch is
+    not applied to another: the best maps-per-recursion group depends on it."""
+    key = (nside, lmax, spin, precision, batch)
     with _PLAN_LOCK:
         if tuning:
             _TUNED[key] = dict(tuning)
         else:
             _TUNED.pop(key, None)
-        for k in [k for k in _PLANS if k[:3] == key[:3] and k[3] == precision]:
+        for k in [k for k in _PLANS if k[:4] == key[:4]]:
             del _PLANS[k]
+
+
+def _tuning_for(nside: int, lmax: int, spin: int, precision: int, batch: int):
+    t = _TUNED.get((nside, lmax, spin, precision, batch))
+    return t if t is not None else _TUNED.get((nside, lmax, spin, precision, None))
 
 
 def get_plan(nside: int, lmax: int, spin: int, batch: int = 1, precision: int = _lib.PREC_F64,
@@ -247,7 +260,7 @@ def get_plan(nside: int, lmax: int, spin: int, batch: int = 1, precision: int = 
     dev = torch.cuda.current_device() if device is None else torch.device(device).index
     flags = _lib.FLAG_BLUESTEIN_ALL if _AZIMUTH_FFT_MODE == "looped" else 0
     if tuning is None:
-        tuning = _TUNED.get((nside, lmax, spin, precision))
+        tuning = _tuning_for(nside, lmax, spin, precision, batch)
     tkey = tuple(sorted(tuning.items())) if tuning else ()
     key = (nside, lmax, spin, precision, flags, dev, tkey)
     with _PLAN_LOCK:

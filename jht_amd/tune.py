@@ -24,15 +24,13 @@ from .healpix import Plan, _prec, alm_size, set_tuning
 # candidate shapes per (precision, spin): synthesis fields, adjoint fields
 _CANDIDATES = {
     ("f64", 0): ([{"synth_rings": r} for r in (1, 4, 8)], [{"adj_rings": r} for r in (4, 8, 1)]),
-    ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)],
-                 [{"f64_adj_mfma": 1}] + [{"f64_adj_mfma": 2, "adj_rings": r} for r in (2, 4, 1)]),
+    ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)], [{"adj_rings": r} for r in (4, 2, 1)]),
     ("mixed", 0): ([{"mx_synth_group": g} for g in (32, 16, 8)], [{"mx_adj_group": g} for g in (16, 8)]),
     ("mixed", 2): ([{"mx_synth_group": 16, "mx_f16": f} for f in (1, 2)] +
                    [{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
-                   [{"mx_adj_group": 16, "mx_adj_occ": 2, "mx_adj_wide": 2, "mx_adj_cb": 1, "mx_adj_f16": 1}] +
-                   [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_wide": w, "mx_adj_cb": c, "mx_adj_f16": 2}
-                    for g, o, w, c in ((16, 2, 2, 1), (16, 2, 2, 2), (8, 2, 2, 2), (8, 2, 1, 2), (4, 1, 2, 2),
-                                       (8, 1, 2, 2))]),
+                   [{"mx_adj_group": 16, "mx_adj_occ": 2, "mx_adj_cb": 1, "mx_adj_f16": 1}] +
+                   [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_cb": c, "mx_adj_f16": 2}
+                    for g, o, c in ((16, 2, 1), (16, 2, 2), (8, 2, 2), (4, 1, 2), (8, 1, 2))]),
 }
 
 
@@ -64,8 +62,8 @@ def autotune(nside: int, lmax: int, spin: int = 0, batch: int = 1, precision: st
              steps: int = 2, apply: bool = True, cache: str | None = None, device=None) -> dict:
     """Time the candidate shapes; return ``{"tuning": {...}, "synth_ms": {...}, "adj_ms": {...}}``.
 
-    With ``apply`` the winner is registered (``set_tuning``) for this geometry and
-    precision.  ``cache`` names a JSON file: a stored result for the same key is reused
+    With ``apply`` the winner is registered (``set_tuning``) for this geometry, precision and
+    batch.  ``cache`` names a JSON file: a stored result for the same key is reused
     instead of timing again, and new results are added to it.
     """
     prec = _prec(precision)
@@ -76,14 +74,13 @@ def autotune(nside: int, lmax: int, spin: int = 0, batch: int = 1, precision: st
         if key in stored:
             res = stored[key]
             if apply:
-                set_tuning(nside, lmax, spin, prec, res["tuning"])
+                set_tuning(nside, lmax, spin, prec, res["tuning"], batch)
             return res
     synth_c, adj_c = _CANDIDATES.get((precision, spin), ([{}], [{}]))
     if precision == "mixed" and batch == 1:
         # one-map calls of a mixed plan run the FP64 vector-FMA kernels (mx_single): their
         # rings per lane are the shapes to time (the FP64 matrix-core adjoint does not apply)
         synth_c, adj_c = _CANDIDATES[("f64", spin)]
-        adj_c = [c for c in adj_c if c.get("f64_adj_mfma") != 1]
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     a, maps, back = _inputs(nside, lmax, spin, batch, dev)
     t_syn, t_adj = {}, {}
@@ -101,7 +98,7 @@ def autotune(nside: int, lmax: int, spin: int = 0, batch: int = 1, precision: st
     tuning = {**best_s, **best_a}
     res = {"tuning": tuning, "synth_ms": t_syn, "adj_ms": t_adj}
     if apply:
-        set_tuning(nside, lmax, spin, prec, tuning)
+        set_tuning(nside, lmax, spin, prec, tuning, batch)
     if cache:
         stored = {}
         if os.path.exists(cache):

@@ -103,5 +103,5 @@ def test_autotune_choice_is_exact(precision, spin, B, tmp_path):
         again = autotune(nside, lmax, spin, B, precision, cache=cache)  # served from the cache
         assert again["tuning"] == res["tuning"]
     finally:
-        H.set_tuning(nside, lmax, spin, H._prec(precision), None)
+        H.set_tuning(nside, lmax, spin, H._prec(precision), None, B)
         H.clear_plans()

@@ -8,6 +8,7 @@ tests/test_vectorized.py:32, tests/test_gpu.py:37), the adjoint inner-product id
 
 from __future__ import annotations
 
+import contextlib
 import warnings
 
 import numpy as np
@@ -25,6 +26,22 @@ def _jht():
     import jht_amd
 
     return jht_amd
+
+
+@contextlib.contextmanager
+def _tuned(nside, lmax, spin, precision, batch=None, **fields):
+    """Plans of this geometry and precision use the given jhtx_tuning fields inside the
+    block (set_tuning: the library's only kernel switch)."""
+    from jht_amd import healpix as H
+
+    prec = H._prec(precision)
+    H.clear_plans()
+    H.set_tuning(nside, lmax, spin, prec, fields, batch)
+    try:
+        yield
+    finally:
+        H.set_tuning(nside, lmax, spin, prec, None, batch)
+        H.clear_plans()
 
 
 def _alm(lmax, rng, spin):
@@ -136,28 +153,19 @@ def test_input_validation_gpu():
 
 @pytest.mark.parametrize("spin,synth_r,adj_r,block", [
     (0, 4, 4, 64), (0, 8, 8, 64), (0, 1, 1, 64), (2, 2, 2, 64), (2, 4, 4, 64), (2, 4, 4, 256), (0, 8, 8, 256)])
-def test_lane_variants(spin, synth_r, adj_r, block, monkeypatch):
+def test_lane_variants(spin, synth_r, adj_r, block):
     """Every rings-per-lane instantiation and multi-chunk ring reductions (forced through
-    the plan's tuning overrides) against the oracle; spin 2 runs the vector-ALU adjoint
-    (the matrix-core variant is compared with it below)."""
+    the plan's tuning) against the oracle."""
     j = _jht()
-    from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_SYNTH_R", str(synth_r))
-    monkeypatch.setenv("JHTX_ADJ_R", str(adj_r))
-    monkeypatch.setenv("JHTX_REC_BLOCK", str(block))
-    monkeypatch.setenv("JHTX_F64_ADJ_MFMA", "0")  # the vector-ALU spin-2 adjoint with adj_r
-    H.clear_plans()
-    try:
-        nside, lmax = 128, 200
+    nside, lmax = 128, 200
+    with _tuned(nside, lmax, spin, "f64", synth_rings=synth_r, adj_rings=adj_r, block=block):
         rng = np.random.default_rng(17 + spin + synth_r)
         a = _alm(lmax, rng, spin)
         npix = 12 * nside * nside
         v = rng.standard_normal(npix if spin == 0 else (2, npix))
         out = j.synthesis(torch.tensor(a, device="cuda"), nside, lmax, spin).cpu().numpy()
         b = j.adjoint_synthesis(torch.tensor(v, device="cuda"), nside, lmax, spin).cpu().numpy()
-    finally:
-        H.clear_plans()
     e1 = _rel(out, O.synthesis(a, nside, lmax, spin))
     e2 = _rel(b, O.adjoint_synthesis(v, nside, lmax, spin))
     print(f"variant spin={spin} R={synth_r}/{adj_r} block={block}: synth {e1:.2e} adj {e2:.2e}", flush=True)
@@ -196,7 +204,7 @@ def test_mixed_matches_f64(nside, lmax, spin, batch):
 @pytest.mark.parametrize("single", [1, 0])
 @pytest.mark.parametrize("spin", [0, 2])
 @pytest.mark.parametrize("nside,lmax", [(16, 32), (64, 128), (256, 400)])
-def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
+def test_mixed_single_map(nside, lmax, spin, single):
     """One-map calls of a mixed plan: the FP64 vector-FMA recursion kernels on complex64
     ring coefficients with the FP32 ring FFT (single 1, the default) and the 4-map
     matrix-core kernels (single 0), against the FP64 path and, on the small grids, the
@@ -204,9 +212,7 @@ def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_SINGLE", str(single))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, spin, "mixed", mx_single=1 if single else 2):
         rng = np.random.default_rng(4100 + nside + spin)
         a = (O.gaussian_alm(lmax, rng) if spin == 0 else
              np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)]))
@@ -219,8 +225,6 @@ def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
             bmx = j.adjoint_synthesis_batch(s64, nside, lmax, spin, precision="mixed")
             plan = H.get_plan(nside, lmax, spin, 1, H._prec("mixed"))
             assert plan.tuning["mx_single"] == (1 if single else 2)
-    finally:
-        H.clear_plans()
     s64, smx, b64, bmx = (t[0].cpu().numpy() for t in (s64, smx, b64, bmx))
     es, ea = _rel(smx, s64), _rel(bmx, b64)
     print(f"mixed one-map single={single} nside={nside} lmax={lmax} spin={spin}: synth {es:.2e} adj {ea:.2e}",
@@ -235,7 +239,7 @@ def test_mixed_single_map(nside, lmax, spin, single, monkeypatch):
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (256, 400, 32, 1.0)])
-def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
+def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16):
     """The 16-map spin-2 synthesis on the FP16 matrix cores (f16 1: hi + lo halves of every
     operand after power-of-two scales) and on the FP32 ones (f16 0) against the FP64 path:
     ragged map groups, inputs far from unit scale (the per-map scale pass), and an all-zero
@@ -243,9 +247,7 @@ def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_F16", str(f16))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, 2, "mixed", mx_f16=1 if f16 else 2):
         rng = np.random.default_rng(4400 + nside + batch)
         a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
                       for _ in range(batch)]) * amp
@@ -257,8 +259,6 @@ def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
             smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed").cpu().numpy()
             plan = H.get_plan(nside, lmax, 2, batch, H._prec("mixed"))
             assert plan.tuning["mx_f16"] == (1 if f16 else 2)
-    finally:
-        H.clear_plans()
     assert np.all(smx[1] == 0.0)
     es = max(_rel(smx[i], s64[i]) for i in range(batch) if i != 1)
     print(f"mixed f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: synth {es:.2e}", flush=True)
@@ -268,16 +268,14 @@ def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16, monkeypatch):
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (256, 400, 32, 1.0)])
-def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16, monkeypatch):
+def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16):
     """The 16-map spin-0 tile synthesis on the FP16 matrix cores (f16 1) and on the FP32 ones
     (f16 0) against the FP64 path: ragged groups, inputs far from unit scale, an all-zero
     map (exact zeros)."""
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_F16", str(f16))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, 0, "mixed", mx_f16=1 if f16 else 2):
         rng = np.random.default_rng(4600 + nside + batch)
         a = np.stack([O.gaussian_alm(lmax, rng) for _ in range(batch)]) * amp
         a[1] = 0.0
@@ -288,8 +286,6 @@ def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16, monkeypat
             smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed").cpu().numpy()
             plan = H.get_plan(nside, lmax, 0, batch, H._prec("mixed"))
             assert plan.tuning["mx_f16"] == (1 if f16 else 2)
-    finally:
-        H.clear_plans()
     assert np.all(smx[1] == 0.0)
     es = max(_rel(smx[i], s64[i]) for i in range(batch) if i != 1)
     print(f"mixed spin-0 f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: synth {es:.2e}", flush=True)
@@ -299,7 +295,7 @@ def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16, monkeypat
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (512, 640, 32, 1.0)])
-def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
+def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16):
     """The 16-map spin-2 column-block adjoint on the FP16 matrix cores (f16 1: hi + lo halves;
     the recursion values scaled by the plan's power of two, the V values by one per map and
     work-group) and on the FP32 ones (f16 0) against the FP64 adjoint: ragged groups, inputs
@@ -307,9 +303,7 @@ def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_ADJ_F16", str(f16))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, 2, "mixed", mx_adj_f16=1 if f16 else 2):
         rng = np.random.default_rng(4500 + nside + batch)
         v = rng.standard_normal((batch, 2, 12 * nside * nside)) * amp
         v[1] = 0.0
@@ -320,8 +314,6 @@ def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
             bmx = j.adjoint_synthesis_batch(vt, nside, lmax, 2, precision="mixed").cpu().numpy()
             plan = H.get_plan(nside, lmax, 2, batch, H._prec("mixed"))
             assert plan.tuning["mx_adj_f16"] == (1 if f16 else 2)
-    finally:
-        H.clear_plans()
     assert np.all(bmx[1] == 0.0)
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch) if i != 1)
     print(f"mixed adjoint f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: {ea:.2e}", flush=True)
@@ -331,16 +323,14 @@ def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16, monkeypatch):
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (512, 640, 32, 1.0)])
-def test_mixed_f16_split_adjoint_spin0(nside, lmax, batch, amp, f16, monkeypatch):
+def test_mixed_f16_split_adjoint_spin0(nside, lmax, batch, amp, f16):
     """The 16-map spin-0 column-block adjoint on the FP16 matrix cores (f16 1) and on the
     FP32 ones (f16 0) against the FP64 adjoint: ragged groups, inputs far from unit scale,
     an all-zero map (exact zeros), and a grid with 4 ring chunks."""
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_ADJ_F16", str(f16))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, 0, "mixed", mx_adj_f16=1 if f16 else 2):
         rng = np.random.default_rng(4700 + nside + batch)
         v = rng.standard_normal((batch, 12 * nside * nside)) * amp
         v[1] = 0.0
@@ -351,8 +341,6 @@ def test_mixed_f16_split_adjoint_spin0(nside, lmax, batch, amp, f16, monkeypatch
             bmx = j.adjoint_synthesis_batch(vt, nside, lmax, 0, precision="mixed").cpu().numpy()
             plan = H.get_plan(nside, lmax, 0, batch, H._prec("mixed"))
             assert plan.tuning["mx_adj_f16"] == (1 if f16 else 2)
-    finally:
-        H.clear_plans()
     assert np.all(bmx[1] == 0.0)
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch) if i != 1)
     print(f"mixed spin-0 adjoint f16={f16} nside={nside} lmax={lmax} B={batch} amp={amp:g}: {ea:.2e}", flush=True)
@@ -362,19 +350,15 @@ def test_mixed_f16_split_adjoint_spin0(nside, lmax, batch, amp, f16, monkeypatch
 @pytest.mark.parametrize("synth_group,synth_occ,adj_group,adj_occ",
                          [(8, 1, 8, 1), (8, 2, 4, 1), (4, 1, 8, 2), (4, 1, 4, 1), (8, 3, 8, 2), (8, 2, 16, 2),
                           (16, 2, 16, 2), (16, 2, 4, 1)])
-def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkeypatch):
+def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ):
     """The 8-maps-per-recursion spin-2 kernels (both register budgets), forced through the
     plan's tuning overrides, against the FP64 path; batch 11 leaves a ragged last group."""
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_SYNTH_GROUP", str(synth_group))
-    monkeypatch.setenv("JHTX_MX_SYNTH_OCC", str(synth_occ))
-    monkeypatch.setenv("JHTX_MX_ADJ_GROUP", str(adj_group))
-    monkeypatch.setenv("JHTX_MX_ADJ_OCC", str(adj_occ))
-    H.clear_plans()
-    try:
-        nside, lmax, batch = 64, 128, 11
+    nside, lmax, batch = 64, 128, 11
+    with _tuned(nside, lmax, 2, "mixed", mx_synth_group=synth_group, mx_synth_occ=synth_occ,
+                mx_adj_group=adj_group, mx_adj_occ=adj_occ):
         rng = np.random.default_rng(77 + synth_group + synth_occ + adj_group + adj_occ)
         a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
                       for _ in range(batch)])
@@ -385,8 +369,6 @@ def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkey
             smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed")
             b64 = j.adjoint_synthesis_batch(s64, nside, lmax, 2, precision="f64")
             bmx = j.adjoint_synthesis_batch(s64, nside, lmax, 2, precision="mixed")
-    finally:
-        H.clear_plans()
     s64, smx, b64, bmx = (t.cpu().numpy() for t in (s64, smx, b64, bmx))
     es = max(_rel(smx[i], s64[i]) for i in range(batch))
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
@@ -395,34 +377,28 @@ def test_mixed_group_variants(synth_group, synth_occ, adj_group, adj_occ, monkey
     assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
-@pytest.mark.parametrize("wide,batch,cb", [(1, 9, 1), (0, 9, 1), (1, 18, 1), (0, 18, 1), (0, 18, 0), (0, 32, 1)])
-def test_mixed_adjoint_ring_chunks(wide, batch, cb, monkeypatch):
-    """Spin-2 mixed adjoint with 512-ring (wide) and 256-ring work-groups on a grid large
-    enough for the wide kernel (T = 1024 rings), 8-map (batch 9) and 16-map (batch 18, a
-    ragged last group; batch 32, two full groups) recursions, the 16-map one in column-block
-    form (cb 1) and with the cross-wave reduction (cb 0), against the FP64 adjoint."""
+@pytest.mark.parametrize("batch,cb", [(9, 1), (18, 1), (18, 0), (32, 1)])
+def test_mixed_adjoint_ring_chunks(batch, cb):
+    """Spin-2 mixed adjoint on a grid of 4 ring chunks (T = 1024 rings), 8-map (batch 9) and
+    16-map (batch 18, a ragged last group; batch 32, two full groups) recursions, the 16-map
+    one in column-block form (cb 1) and with the cross-wave reduction (cb 0), against the
+    FP64 adjoint."""
     j = _jht()
-    from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_ADJ_WIDE", str(wide))
-    monkeypatch.setenv("JHTX_MX_ADJ_CB", str(cb))
-    H.clear_plans()
-    try:
-        nside, lmax = 512, 640
-        rng = np.random.default_rng(500 + wide)
+    nside, lmax = 512, 640
+    with _tuned(nside, lmax, 2, "mixed", mx_adj_cb=1 if cb else 2):
+        rng = np.random.default_rng(500 + batch)
         v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
         b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
         bmx = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="mixed").cpu().numpy()
-    finally:
-        H.clear_plans()
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
-    print(f"mixed adjoint wide={wide} batch={batch} cb={cb}: {ea:.2e}", flush=True)
+    print(f"mixed adjoint batch={batch} cb={cb}: {ea:.2e}", flush=True)
     assert ea <= MIXED_TOL
 
 
 @pytest.mark.parametrize("group", [32, 16, 8])
 @pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
-def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
+def test_mixed_spin0_groups(nside, lmax, group):
     """Spin-0 mixed synthesis and adjoint with 32 maps per recursion (the tile synthesis
     with 4 column blocks; the adjoint takes 16), 16 (the 16x16x4 tile synthesis, the
     column-block adjoint) and 8 (the rank-1 synthesis, the per-wave-tile adjoint), batch 18
@@ -430,10 +406,7 @@ def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
     j = _jht()
     from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_SYNTH_GROUP", str(group))
-    monkeypatch.setenv("JHTX_MX_ADJ_GROUP", str(group))
-    H.clear_plans()
-    try:
+    with _tuned(nside, lmax, 0, "mixed", mx_synth_group=group, mx_adj_group=min(group, 16)):
         batch = 18
         rng = np.random.default_rng(90 + group + nside)
         a = np.stack([O.gaussian_alm(lmax, rng, 0) for _ in range(batch)])
@@ -442,8 +415,6 @@ def test_mixed_spin0_groups(nside, lmax, group, monkeypatch):
         smx = j.synthesis_batch(at, nside, lmax, 0, precision="mixed")
         b64 = j.adjoint_synthesis_batch(s64, nside, lmax, 0, precision="f64")
         bmx = j.adjoint_synthesis_batch(s64, nside, lmax, 0, precision="mixed")
-    finally:
-        H.clear_plans()
     s64, smx, b64, bmx = (x.cpu().numpy() for x in (s64, smx, b64, bmx))
     es = max(_rel(smx[i], s64[i]) for i in range(batch))
     ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
@@ -485,47 +456,39 @@ def test_mixed_16map_kernels_tiny_grids(nside, lmax, spin):
     assert es <= MIXED_TOL and ea <= MIXED_TOL
 
 
-def test_mixed_adjoint_column_block_bitwise(monkeypatch):
+def test_mixed_adjoint_column_block_bitwise():
     """The FP32 column-block 16-map adjoint keeps the arithmetic of the cross-wave form (FP32
     over each 64-ring block, FP64 over the blocks in order): identical bits, ragged batch."""
     j = _jht()
-    from jht_amd import healpix as H
 
-    monkeypatch.setenv("JHTX_MX_ADJ_F16", "0")
     nside, lmax, batch = 256, 400, 18
     rng = np.random.default_rng(808)
     v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
     out = {}
-    for cb in (1, 0):
-        monkeypatch.setenv("JHTX_MX_ADJ_CB", str(cb))
-        H.clear_plans()
-        try:
+    for cb in (1, 2):
+        with _tuned(nside, lmax, 2, "mixed", mx_adj_f16=2, mx_adj_cb=cb):
             out[cb] = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="mixed").cpu()
-        finally:
-            H.clear_plans()
-    assert torch.equal(out[0], out[1])
+    assert torch.equal(out[1], out[2])
 
 
 @pytest.mark.parametrize("nside,lmax", [(8, 12), (64, 128), (128, 300), (256, 200)])
-def test_f64_adjoint_kernels_agree(nside, lmax, monkeypatch):
-    """Spin-2 FP64 adjoint on the matrix cores against the vector-ALU kernel and the oracle
-    (multi-chunk grids, lmax above and below the ring count)."""
+def test_f64_adjoint_ring_shapes_agree(nside, lmax):
+    """Spin-2 FP64 adjoint with 1, 2 and 4 rings per lane (different ring-chunk counts and
+    in-lane sums) against each other and, on the small grids, the oracle (lmax above and
+    below the ring count)."""
     j = _jht()
-    from jht_amd import healpix as H
 
     rng = np.random.default_rng(900 + nside)
     v = rng.standard_normal((2, 12 * nside * nside))
     vt = torch.tensor(v, device="cuda")
     outs = {}
-    for mf in ("1", "0"):
-        monkeypatch.setenv("JHTX_F64_ADJ_MFMA", mf)
-        H.clear_plans()
-        with warnings.catch_warnings():
-            warnings.simplefilter("ignore")
-            outs[mf] = j.adjoint_synthesis(vt, nside, lmax, 2).cpu().numpy()
-    H.clear_plans()
-    e = _rel(outs["1"], outs["0"])
-    print(f"f64 adjoint mfma vs valu nside={nside} lmax={lmax}: {e:.2e}", flush=True)
+    for r in (1, 2, 4):
+        with _tuned(nside, lmax, 2, "f64", adj_rings=r):
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                outs[r] = j.adjoint_synthesis(vt, nside, lmax, 2).cpu().numpy()
+    e = max(_rel(outs[r], outs[1]) for r in (2, 4))
+    print(f"f64 adjoint rings per lane nside={nside} lmax={lmax}: {e:.2e}", flush=True)
     assert e <= TOL
     if nside <= 64:
-        assert _rel(outs["1"], O.adjoint_synthesis(v, nside, lmax, 2)) <= TOL
+        assert _rel(outs[4], O.adjoint_synthesis(v, nside, lmax, 2)) <= TOL

@@ -215,9 +215,6 @@ def test_tuning_validation_without_gpu():
     rc = lib.jhtx_plan_create_tuned(16, 24, 0, 0, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
     assert b"tuning" in lib.jhtx_last_error()
-    bad = L.Tuning(mx_adj_wide=3)
-    rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
-    assert rc == L.JHTX_EINVAL
     bad = L.Tuning(mx_adj_cb=3)
     rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
@@ -289,29 +286,20 @@ def test_offgrid_python_validation_without_gpu():
 
 
 def test_reference_public_names_exist():
-    """Every public name of the reference package (src/jht/__init__.py:66-106) has a
-    counterpart in jht_amd."""
+    """Every public name of the reference package (src/jht/__init__.py:66-106) on the
+    transform path and its callers has a counterpart in jht_amd.  The XLA compilation cache
+    (enable_compilation_cache, src/jht/_cache.py) has none: nothing here is compiled at run
+    time."""
     import jht_amd
 
     names = ["synthesis", "adjoint_synthesis", "synthesis_general", "adjoint_synthesis_general",
              "analysis", "map2alm", "bare_analysis", "ring_weights", "pixel_weights", "pseudo_alm",
              "deconvolve", "wiener", "constrained_realization", "synthesis_real", "analysis_real",
              "synthesis_general_real", "adjoint_synthesis_general_real", "bandpower", "alm_to_real",
-             "real_to_alm", "n_dof", "alm_size", "alm_metric_weight", "enable_compilation_cache",
+             "real_to_alm", "n_dof", "alm_size", "alm_metric_weight",
              "set_azimuth_fft_mode", "get_azimuth_fft_mode", "enable_looped_fft", "__version__"]
     missing = [n for n in names if not hasattr(jht_amd, n)]
     assert not missing, missing
-
-
-def test_enable_compilation_cache(tmp_path, monkeypatch):
-    import jht_amd
-
-    monkeypatch.delenv("ROCFFT_RTC_CACHE_PATH", raising=False)
-    jht_amd.enable_compilation_cache(str(tmp_path / "cache"), min_compile_time_secs=2.0)
-    assert os.environ["ROCFFT_RTC_CACHE_PATH"].startswith(str(tmp_path / "cache"))
-    assert (tmp_path / "cache").is_dir()
-    with pytest.raises(ValueError):
-        jht_amd.enable_compilation_cache(str(tmp_path), min_compile_time_secs=-1)
 
 
 def test_tuning_struct_matches_header():
