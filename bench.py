@@ -10,7 +10,7 @@ and the mixed outputs of the first map are compared with the FP64 GPU outputs
 
     python bench.py [--steps K] [--warmup W] [--precision f64|mixed] [--batch B]
                     [--nside N --lmax L --spin S] [--no-cpu] [--no-f64]
-                    [--offgrid NPTS | --c4 | --peaks]
+                    [--offgrid NPTS | --c4 | --peaks | --cpu-c2]
 
 Roofline: the dominant recursion + contraction kernel (synthesis or adjoint; the adjoint's
 chunk reduction is the separate stage adj_finish) is timed live with HIP events recorded by
@@ -21,8 +21,10 @@ contraction, 16 flops per triple per map (spin 2; 4 for spin 0), against the 157
 dense FP32 MFMA peak; the 16-map spin-2 kernels run it on the FP16 matrix cores as four FP16
 products per FP32-equivalent product, against a quarter of the 2500 TFLOP/s dense FP16 peak; a
 one-map mixed call runs the FP64 kernels and is priced as the FP64 path.  CPU baseline: the reference's
-fused masked scan restated in numpy (oracle/jht_oracle.py) on a sample of orders m over
-every l and ring, scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs.
+masked l-scans restated in numpy (oracle/jht_oracle.py), the synthesis scan and then the
+adjoint scan, on a sample of orders m over every l and ring spread over every host core
+(at most 16), scaled by (lmax+1)/sample, plus the full-size numpy ring FFTs; --cpu-c2 times
+the spin-0 nside 1024 / lmax 2048 pair exactly (every order).
 """
 
 from __future__ import annotations
@@ -73,44 +75,87 @@ def triples(nside, lmax, spin):
     return (K - inactive) * 2 * nside
 
 
-def cpu_baseline(nside, lmax, spin, rows=8, seed=1):
-    """Time the reference algorithm (numpy restatement) on a bounded sample: the fused masked
-    l-scan for `rows` orders m over all l and rings (synthesis + adjoint contractions),
-    scaled to all lmax+1 orders, plus the full ring-FFT assembly in both directions."""
+def cpu_cores():
+    """Host threads the CPU baseline uses: the process's CPU set, at most 16 (one GPU's share
+    of a node)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _cpu_rows(job):
+    """One worker's share of the sampled scans: the synthesis scan and then the adjoint scan
+    (the reference runs them as two jitted programs) over its orders m; returns seconds."""
+    nside, lmax, spin, ms, seed = job
     from oracle import jht_oracle as O
 
-    assert spin == 2, "the CPU baseline sample is written for the spin-2 workload"
+    rng = np.random.default_rng(seed)
+    x = O.RingInfo(nside).z[: 2 * nside]
+    R, L1, T = len(ms), lmax + 1, 2 * nside
+
+    def cplx(*shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    coef = cplx(R, L1) if spin == 0 else (cplx(R, L1), cplx(R, L1))
+    V = [cplx(R, T) for _ in range(2 if spin == 0 else 4)]
+    t0 = time.perf_counter()
+    O.sampled_synthesis_scan(x, lmax, ms, spin, coef)
+    O.sampled_adjoint_scan(x, lmax, ms, spin, V)
+    return time.perf_counter() - t0
+
+
+def _cpu_warm(_):
+    return os.getpid()
+
+
+def cpu_baseline(nside, lmax, spin, rows_per_core=2, all_orders=False, seed=1):
+    """Time the reference algorithm (the oracle's numpy restatement of jht's masked l-scans)
+    on every host core (a fork pool, started before this process touches the GPU).
+
+    Orders m are dealt round-robin to the workers; each runs the synthesis scan and then the
+    adjoint scan over its orders, every l and every north-half ring, masked as jht's lax.scan
+    runs them (so every order costs the same).  The slowest worker's time, scaled by
+    (lmax + 1) / orders timed, is the recursion time per map; the full-size numpy ring FFTs
+    of both directions are timed in this process and added.  all_orders: time every order
+    (no scaling)."""
+    import multiprocessing
+
+    from oracle import jht_oracle as O
+
+    P = cpu_cores()
+    M = lmax + 1
+    ms = np.arange(M) if all_orders else np.unique(np.linspace(0, lmax, P * rows_per_core).astype(np.int64))
+    chunks = [ms[k::P] for k in range(P) if len(ms[k::P])]
+    with multiprocessing.get_context("fork").Pool(len(chunks)) as pool:
+        pool.map(_cpu_warm, range(len(chunks)))
+        times = pool.map(_cpu_rows, [(nside, lmax, spin, c, seed + k) for k, c in enumerate(chunks)])
+    t_rec = max(times) * (M / ms.size)
     rng = np.random.default_rng(seed)
     geo = O.RingInfo(nside)
-    T = 2 * nside
-    x = geo.z[:T]
-    M = lmax + 1
-    ms = np.unique(np.linspace(0, lmax, rows).astype(np.int64))
-    R = ms.size
-    bp = rng.standard_normal((R, M)) + 1j * rng.standard_normal((R, M))
-    bm = rng.standard_normal((R, M)) + 1j * rng.standard_normal((R, M))
-    V = [rng.standard_normal((R, T)) + 1j * rng.standard_normal((R, T)) for _ in range(4)]
+    G = rng.standard_normal((M, geo.nrings)) + 1j * rng.standard_normal((M, geo.nrings))
     t0 = time.perf_counter()
-    O.sampled_spin2_pair(x, lmax, ms, bp, bm, *V)
-    t_rec = (time.perf_counter() - t0) * (M / R)
-    # ring FFT assembly, both directions, full size
-    G = (rng.standard_normal((M, geo.nrings)) + 1j * rng.standard_normal((M, geo.nrings)))
-    t0 = time.perf_counter()
-    qu = O.ring_synthesis(G, G, geo)
-    O.ring_analysis(qu, geo, lmax)
-    O.ring_analysis(qu, geo, lmax, minus=True)
+    out = O.ring_synthesis(G, G, geo)
+    if spin == 0:
+        O.ring_analysis(out.real.astype(complex), geo, lmax)
+    else:
+        O.ring_analysis(out, geo, lmax)
+        O.ring_analysis(out, geo, lmax, minus=True)
     t_fft = time.perf_counter() - t0
-    del G, qu
+    del G, out
     total = t_rec + t_fft
+    what = "every" if all_orders else f"{ms.size} of {M}"
     return {
         "value": 1.0 / total,
         "unit": "maps/s",
-        "cores": 1,
+        "cores": len(chunks),
         "kind": "port",
-        "sample": (f"numpy restatement of jht's fused masked l-scan (spin 2, both +-2 "
-                   f"recursions, synthesis + adjoint contractions) for {R} of {M} orders m over "
-                   f"all {M} l and {T} rings, scaled x{M / R:.0f} ({t_rec:.1f} s/map), plus "
-                   f"full-size numpy ring FFTs both ways ({t_fft:.1f} s/map); 1 thread"),
+        "sample": (f"numpy restatement of jht's masked l-scans (spin {spin}: synthesis scan then adjoint "
+                   f"scan{', both +-2 recursions in each' if spin else ''}) for {what} orders m over all "
+                   f"{M} l and {2 * nside} rings, {len(chunks)} processes, slowest {max(times):.1f} s"
+                   + ("" if all_orders else f", scaled x{M / ms.size:.1f}") +
+                   f" ({t_rec:.1f} s/map); full-size numpy ring FFTs both ways on one core ({t_fft:.1f} s/map)"),
     }
 
 
@@ -279,7 +324,11 @@ def main():
     ap.add_argument("--spin", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=12)
+    ap.add_argument("--cpu-rows", type=int, default=2, help="orders m per host core in the CPU baseline sample")
+    ap.add_argument("--cpu-c2", action="store_true",
+                    help="instead of the headline transform pair, time the CPU baseline exactly on the "
+                         "spin-0 nside 1024 / lmax 2048 pair (every order m, all host cores) and print "
+                         "its own JSON line; no GPU work")
     ap.add_argument("--offgrid", type=int, default=0, metavar="NPTS",
                     help="instead of the headline transform pair, time the off-grid pair "
                          "synthesis_general + adjoint_synthesis_general at --lmax / --spin on "
@@ -301,6 +350,15 @@ def main():
               "(maps are independent: several GPUs would each take a share of the batch)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_c2:
+        r = cpu_baseline(1024, 2048, 0, all_orders=True)
+        print(json.dumps({"metric": "CPU baseline maps/s synth+adjoint nside=1024 lmax=2048 (spin 0), every order m",
+                          **r}), flush=True)
+        return
+    headline = not (args.offgrid > 0 or args.c4 or args.peaks)
+    # the CPU baseline forks its workers before this process touches the GPU
+    cpu = cpu_baseline(args.nside, args.lmax, args.spin, args.cpu_rows) if headline and not args.no_cpu else None
 
     import torch
 
@@ -429,8 +487,8 @@ def main():
         "plan_create_s": round(t_plan, 3),
     }
     out.update(out_extra)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(nside, lmax, spin, args.cpu_rows)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -51,16 +51,27 @@ double ring_z(int nside, int t) {
   return 4.0 / 3.0 - (2.0 * (double)i) / (3.0 * (double)nside);
 }
 
-// Wigner-d single-term seed parameters of d^{lmin}_{M,M'} (_recursion.py:166-188).
+// ln 2 / 2: the Wigner seed tables are centred on cos(theta/2) = sin(theta/2) = 1/sqrt 2
+constexpr long double kHalfLn2 = 0.346573590279972654708616060729088284L;
+
+// Wigner-d single-term seed parameters of d^{lmin}_{M,M'} (_recursion.py:166-188):
+// log|d| = log_fac + pc log cos(theta/2) + ps log sin(theta/2).  Near the equator, where
+// high-m seeds live, the three terms are ~10^3 each and cancel to ~10^2, so the constant is
+// stored centred, log_fac - (pc + ps) ln2/2, and the ring tables hold log cos(theta/2) +
+// ln2/2 and log sin(theta/2) + ln2/2 (host_rec_tables_x): the kernels' L = c + pc lc + ps ls
+// then adds small terms.  log_fac comes from long-double lgamma (the float64 lgamma sum of
+// the reference's formula is off by up to ~1e-11 relative at lmin ~ 4096).
 void wigner_seed_params(int M, int Mp, int lmin, double out[4]) {
   const int k0 = std::max(0, M - Mp);
-  auto lg = [](double v) { return std::lgamma(v); };
-  const double log_fac =
-      0.5 * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) -
+  auto lg = [](long double v) { return lgammal(v); };
+  const long double log_fac =
+      0.5L * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) -
       (lg(k0 + 1) + lg(lmin + M - k0 + 1) + lg(lmin - Mp - k0 + 1) + lg(k0 - M + Mp + 1));
-  out[0] = log_fac;
-  out[1] = (double)(2 * lmin - 2 * k0 + M - Mp);  // power of cos(theta/2)
-  out[2] = (double)(2 * k0 - M + Mp);              // power of sin(theta/2)
+  const int pc = 2 * lmin - 2 * k0 + M - Mp;  // power of cos(theta/2)
+  const int ps = 2 * k0 - M + Mp;             // power of sin(theta/2)
+  out[0] = (double)(log_fac - (long double)(pc + ps) * kHalfLn2);
+  out[1] = (double)pc;
+  out[2] = (double)ps;
   out[3] = (k0 % 2) ? -1.0 : 1.0;
 }
 
@@ -69,11 +80,9 @@ struct HostRec {
   std::vector<double> x, lsin, lcos, alpha, gamma, G, seed;
 };
 
-// log of a non-negative factor; an exact zero (a ring on a pole) maps to -1e300, which
-// the seed split reads as an exact zero seed, while a zero power of it still gives 0.
-double pole_log(double v) { return v > 0.0 ? std::log(v) : -1.0e300; }
-
 // Tables over the north-half rings x[t] = cos(theta_t), t < T (the last one the equator).
+// The log of a factor that is exactly zero (a ring on a pole) is stored as -1e300, which the
+// seed split reads as an exact zero seed, while a zero power of it still gives 0.
 void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRec& h) {
   const int M = lmax + 1, T = (int)xs.size();
   const long long K = (long long)(lmax + 1) * (lmax + 2) / 2;
@@ -87,20 +96,25 @@ void host_rec_tables_x(const std::vector<double>& xs, int lmax, int spin, HostRe
   x = xs;
   lsin.assign(T, 0.0);
   for (int t = 0; t < T; ++t) {
-    const double z = x[t];
+    const long double z = x[t];
     if (spin == 0) {
-      const double s2 = std::min(std::max(1.0 - z * z, 0.0), 1.0);
-      lsin[t] = pole_log(std::sqrt(s2));
+      // sin^2 theta = (1 - z)(1 + z): 1 - z is exact, where 1 - z*z in double loses ~7e-10
+      // relative on the ring nearest a pole at nside 2048
+      const long double s2 = std::min(std::max((1.0L - z) * (1.0L + z), 0.0L), 1.0L);
+      lsin[t] = s2 > 0.0L ? (double)(0.5L * logl(s2)) : -1.0e300;
     }
   }
   if (spin != 0) {
+    // log cos(theta/2) + ln2/2 and log sin(theta/2) + ln2/2 in long double (the seed
+    // constants are centred to match: wigner_seed_params)
     lcos.resize(T);
+    auto centred_log = [](long double v) {
+      return v > 0.0L ? (double)(0.5L * logl(v) + kHalfLn2) : -1.0e300;
+    };
     for (int t = 0; t < T; ++t) {
-      const double z = x[t];
-      const double ch = std::sqrt(std::min(std::max((1.0 + z) / 2.0, 0.0), 1.0));
-      const double sh = std::sqrt(std::min(std::max((1.0 - z) / 2.0, 0.0), 1.0));
-      lcos[t] = pole_log(ch);
-      lsin[t] = pole_log(sh);
+      const long double z = x[t];
+      lcos[t] = centred_log(std::min(std::max((1.0L + z) / 2.0L, 0.0L), 1.0L));
+      lsin[t] = centred_log(std::min(std::max((1.0L - z) / 2.0L, 0.0L), 1.0L));
     }
   }
   alpha.assign(K, 0.0);

@@ -246,7 +246,8 @@ __device__ __forceinline__ void s2_synth_block(const double2* s_ag, const double
 }
 
 __device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, double ls) {
-  // sp = {log_fac, pow_cos, pow_sin, sign}; same operation order as the reference seed
+  // sp = {centred log_fac, pow_cos, pow_sin, sign}; lc, ls: centred log cos, log sin of
+  // theta/2 (the host tables), so L = log_fac + pc log cos(theta/2) + ps log sin(theta/2)
   double L = sp[0];
   if (sp[1] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[1], lc));
   if (sp[2] > 0.0) L = __dadd_rn(L, __dmul_rn(sp[2], ls));

@@ -149,25 +149,41 @@ def _renorm_step(raw, p, s):
     return raw / r, p / r, s + np.log(r)
 
 
-def normalized_legendre(x, m: int, lmax: int) -> np.ndarray:
-    """lambda_{l,m}(theta), l = m..lmax, shape (lmax-m+1, T) (``_recursion.py:59-117``)."""
-    x = np.asarray(x, dtype=np.float64)
-    sin_t = np.sqrt(np.clip(1.0 - x * x, 0.0, 1.0))
-    c = sectoral_seed_logabs(m)
+def normalized_legendre(x, m: int, lmax: int, precise: bool = False) -> np.ndarray:
+    """lambda_{l,m}(theta), l = m..lmax, shape (lmax-m+1, T) (``_recursion.py:59-117``).
+
+    ``precise=True`` runs the same recursion and seed in long double (64-bit mantissa) and
+    rounds the result to float64 once: the high-precision check of the float64 form."""
+    dt = np.longdouble if precise else np.float64
+    x = np.asarray(x, dtype=np.float64).astype(dt)
+    if precise:  # 1 - x is exact: no cancellation next to the poles
+        sin_t = np.sqrt(np.clip((1.0 - x) * (1.0 + x), 0.0, 1.0))
+    else:
+        sin_t = np.sqrt(np.clip(1.0 - x * x, 0.0, 1.0))
+    if precise:
+        k = np.arange(1, m + 1, dtype=dt)
+        c = -0.5 * np.log(4 * np.longdouble(np.pi)) + 0.5 * np.sum(np.log((2 * k + 1) / (2 * k)))
+    else:
+        c = sectoral_seed_logabs(m)
     if m == 0:
-        s = np.full(x.shape, c)
+        s = np.full(x.shape, c, dtype=dt)
     else:
         with np.errstate(divide="ignore"):
             s = c + m * np.log(sin_t)
-    p = np.full(x.shape, 1.0 if m % 2 == 0 else -1.0)
+    p = np.full(x.shape, 1.0 if m % 2 == 0 else -1.0, dtype=dt)
     q = np.zeros_like(x)
     out = [p * np.exp(s)]
-    a, b = legendre_recurrence_coeffs(m, lmax)
+    if precise:
+        l = np.arange(m + 1, lmax + 1, dtype=dt)
+        a = np.sqrt((2 * l - 1) * (2 * l + 1) / ((l - m) * (l + m)))
+        b = np.sqrt((2 * l + 1) * ((l - 1) ** 2 - m * m) / ((2 * l - 3) * (l - m) * (l + m)))
+    else:
+        a, b = legendre_recurrence_coeffs(m, lmax)
     for i in range(a.size):
         raw = a[i] * x * p - b[i] * q
         p, q, s = _renorm_step(raw, p, s)
         out.append(p * np.exp(s))
-    return np.array(out)
+    return np.array(out).astype(np.float64)
 
 
 def wigner_recurrence_coeffs(M: int, Mp: int, lmin: int, lmax: int):
@@ -180,26 +196,48 @@ def wigner_recurrence_coeffs(M: int, Mp: int, lmin: int, lmax: int):
     return A, C, B
 
 
-def wigner_seed_params(M: int, Mp: int, lmin: int):
-    """Single-term seed of d^{lmin}_{M,M'}: (sign, log_fac, pow_cos, pow_sin) (``_recursion.py:166-178``)."""
+@lru_cache(maxsize=None)
+def _exact_log_fac(M: int, Mp: int, lmin: int, k0: int) -> float:
+    """The seed's log-factorial ratio evaluated in mpmath (dps 40)."""
+    import mpmath as mp
+
+    with mp.workdps(40):
+        lg = mp.loggamma
+        v = mp.mpf(0.5) * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) - (
+            lg(k0 + 1) + lg(lmin + M - k0 + 1) + lg(lmin - Mp - k0 + 1) + lg(k0 - M + Mp + 1))
+        return mp.nstr(v, 30)
+
+
+def wigner_seed_params(M: int, Mp: int, lmin: int, precise: bool = False):
+    """Single-term seed of d^{lmin}_{M,M'}: (sign, log_fac, pow_cos, pow_sin) (``_recursion.py:166-178``).
+
+    ``precise=False`` is the reference's arithmetic: eight float64 ``lgamma`` values of
+    arguments up to 2 lmin, whose rounding leaves up to ~1e-11 relative error in the seed
+    near lmin = 4096 (``tests/test_oracle.py`` measures it against mpmath).  ``precise=True``
+    takes the same log-factorial ratio from mpmath, as a long double."""
     k0 = max(0, M - Mp)
-    lg = math.lgamma
-    log_fac = 0.5 * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) - (
-        lg(k0 + 1) + lg(lmin + M - k0 + 1) + lg(lmin - Mp - k0 + 1) + lg(k0 - M + Mp + 1)
-    )
+    if precise:
+        log_fac = np.longdouble(_exact_log_fac(M, Mp, lmin, k0))
+    else:
+        lg = math.lgamma
+        log_fac = 0.5 * (lg(lmin + M + 1) + lg(lmin - M + 1) + lg(lmin + Mp + 1) + lg(lmin - Mp + 1)) - (
+            lg(k0 + 1) + lg(lmin + M - k0 + 1) + lg(lmin - Mp - k0 + 1) + lg(k0 - M + Mp + 1)
+        )
     sign = -1.0 if (k0 % 2) else 1.0
     pow_cos = 2 * lmin - 2 * k0 + M - Mp
     pow_sin = 2 * k0 - M + Mp
     return sign, log_fac, pow_cos, pow_sin
 
 
-def wigner_seed(x, M: int, Mp: int, lmin: int):
-    """(sign, log|d^{lmin}_{M,M'}(theta)|) (``_recursion.py:166-188``)."""
-    x = np.asarray(x, dtype=np.float64)
-    sign, log_fac, pc, ps = wigner_seed_params(M, Mp, lmin)
+def wigner_seed(x, M: int, Mp: int, lmin: int, precise: bool = False):
+    """(sign, log|d^{lmin}_{M,M'}(theta)|) (``_recursion.py:166-188``); ``precise``: in long
+    double from the exact log-factorial ratio."""
+    dt = np.longdouble if precise else np.float64
+    x = np.asarray(x, dtype=np.float64).astype(dt)
+    sign, log_fac, pc, ps = wigner_seed_params(M, Mp, lmin, precise)
     cos_h = np.sqrt(np.clip((1.0 + x) / 2.0, 0.0, 1.0))
     sin_h = np.sqrt(np.clip((1.0 - x) / 2.0, 0.0, 1.0))
-    out = np.full(x.shape, np.float64(log_fac))
+    out = np.full(x.shape, log_fac, dtype=dt)
     with np.errstate(divide="ignore"):
         if pc > 0:
             out = out + pc * np.log(cos_h)
@@ -208,28 +246,46 @@ def wigner_seed(x, M: int, Mp: int, lmin: int):
     return sign, out
 
 
-def wigner_d_column(x, M: int, Mp: int, lmax: int) -> np.ndarray:
-    """d^l_{M,M'}(theta), l = max(|M|,|M'|)..lmax (``_recursion.py:191-221``)."""
-    x = np.asarray(x, dtype=np.float64)
+def wigner_d_column(x, M: int, Mp: int, lmax: int, precise: bool = False) -> np.ndarray:
+    """d^l_{M,M'}(theta), l = max(|M|,|M'|)..lmax (``_recursion.py:191-221``).
+
+    ``precise=True`` runs the same recursion with every quantity in long double (64-bit
+    mantissa) from the exact seed; the result is rounded to float64 once.  It is the
+    high-precision check of the float64 restatement, whose log-scale carry alone leaves
+    ~|log| 1e-16 relative error (5e-13 where the seed is e^-2800)."""
+    dt = np.longdouble if precise else np.float64
+    x = np.asarray(x, dtype=np.float64).astype(dt)
     lmin = max(abs(M), abs(Mp))
     if lmax < lmin:
         raise ValueError(f"lmax={lmax} < lmin={lmin}")
-    sign, s = wigner_seed(x, M, Mp, lmin)
-    p = np.full(x.shape, sign)
+    sign, s = wigner_seed(x, M, Mp, lmin, precise)
+    p = np.full(x.shape, sign, dtype=dt)
     q = np.zeros_like(x)
     out = [p * np.exp(s)]
-    A, C, B = wigner_recurrence_coeffs(M, Mp, lmin, lmax)
+    if precise:
+        l = np.arange(lmin, lmax, dtype=dt)
+        D = l * np.sqrt(((l + 1) ** 2 - M * M) * ((l + 1) ** 2 - Mp * Mp))
+        A = (2 * l + 1) * l * (l + 1) / D
+        C = -(2 * l + 1) * M * Mp / D
+        B = (l + 1) * np.sqrt((l * l - M * M) * (l * l - Mp * Mp)) / D
+    else:
+        A, C, B = wigner_recurrence_coeffs(M, Mp, lmin, lmax)
     for i in range(A.size):
         raw = (A[i] * x + C[i]) * p - B[i] * q
         p, q, s = _renorm_step(raw, p, s)
         out.append(p * np.exp(s))
-    return np.array(out)
+    return np.array(out).astype(np.float64)
 
 
-def spin_weighted_lambda(x, m: int, spin: int, lmax: int) -> np.ndarray:
+def spin_weighted_lambda(x, m: int, spin: int, lmax: int, precise: bool = False) -> np.ndarray:
     """_s lambda_{l,m}(theta) = (-1)^m sqrt((2l+1)/4pi) d^l_{-m,s}, l = max(m,|s|)..lmax
-    (``_recursion.py:224-235``)."""
+    (``_recursion.py:224-235``).  ``precise``: see :func:`wigner_d_column`."""
     lmin = max(m, abs(spin))
+    if precise:
+        d = wigner_d_column(x, -m, spin, lmax, True).astype(np.longdouble)
+        l = np.arange(lmin, lmax + 1, dtype=np.longdouble)
+        pref = ((-1.0) ** m) * np.sqrt((2 * l + 1) / (4 * np.longdouble(np.pi)))
+        return (pref[:, None] * d).astype(np.float64)
     d = wigner_d_column(x, -m, spin, lmax)
     l = np.arange(lmin, lmax + 1, dtype=np.float64)
     pref = ((-1.0) ** m) * np.sqrt((2.0 * l + 1.0) / FOUR_PI)
@@ -767,6 +823,10 @@ def random_alm(lmax: int, rng, lmin: int = 0) -> np.ndarray:
 
 
 class _RowTables:
+    """Per-row recursion tables of the sampled scans: the Legendre recursion and sectoral
+    seed for spin 0 (``_recursion.py:37-56``), the Wigner-d recursion and single-term seed
+    with the (-1)^m sqrt((2l+1)/4pi) prefactor for spin +-2 (``:136-188``)."""
+
     def __init__(self, x, spin, lmax, ms):
         ms = np.asarray(ms, dtype=np.int64)
         L1, R = lmax + 1, ms.size
@@ -776,14 +836,25 @@ class _RowTables:
         self.seed_sign = np.zeros(R)
         self.seed_log = np.zeros((R, x.size))
         self.lmin = np.maximum(ms, abs(spin))
+        sin_t = np.sqrt(np.clip(1.0 - x * x, 0.0, 1.0))
         for i, m in enumerate(ms):
             lm = int(self.lmin[i])
-            if lm < lmax:
-                Ac, Cc, Bc = wigner_recurrence_coeffs(-int(m), spin, lm, lmax)
-                self.A[lm + 1:, i], self.C[lm + 1:, i], self.B[lm + 1:, i] = Ac, Cc, Bc
-            self.seed_sign[i], self.seed_log[i] = wigner_seed(x, -int(m), spin, lm)
+            if spin == 0:
+                if lm < lmax:
+                    self.A[lm + 1:, i], self.B[lm + 1:, i] = legendre_recurrence_coeffs(int(m), lmax)
+                self.seed_sign[i] = -1.0 if m % 2 else 1.0
+                with np.errstate(divide="ignore"):
+                    self.seed_log[i] = sectoral_seed_logabs(int(m)) + (m * np.log(sin_t) if m else 0.0)
+            else:
+                if lm < lmax:
+                    Ac, Cc, Bc = wigner_recurrence_coeffs(-int(m), spin, lm, lmax)
+                    self.A[lm + 1:, i], self.C[lm + 1:, i], self.B[lm + 1:, i] = Ac, Cc, Bc
+                self.seed_sign[i], self.seed_log[i] = wigner_seed(x, -int(m), spin, lm)
         l = np.arange(L1, dtype=np.float64)
-        self.pref = ((-1.0) ** ms)[None, :] * np.sqrt((2.0 * l + 1.0) / FOUR_PI)[:, None]
+        if spin == 0:
+            self.pref = np.ones((L1, R))
+        else:
+            self.pref = ((-1.0) ** ms)[None, :] * np.sqrt((2.0 * l + 1.0) / FOUR_PI)[:, None]
 
 
 def _masked_step(tb, l, x, p, q, s):
@@ -802,29 +873,59 @@ def _masked_step(tb, l, x, p, q, s):
     return p2, q2, s2, lam
 
 
-def sampled_spin2_pair(x, lmax, ms, bp, bm, Vpn, Sp, Vmn, Sm):
-    """Synthesis and adjoint contractions of the spin-2 fused scans restricted to the
-    orders ``ms`` (bp, bm: (R, L1); V*: (R, T)).  Returns (FpN, FpS, FmN, FmS, p2, m2)."""
+def sampled_synthesis_scan(x, lmax, ms, spin, coef):
+    """The fused synthesis scan restricted to the orders ``ms``, every step masked as the
+    reference's lax.scan runs it (spin 0: ``_recursion.py:441-469``; spin 2, both +-2
+    recursions in one scan: ``:512-543``).  coef: spin 0 the (R, L1) a_lm rows; spin 2 the
+    pair (bp, bm).  Returns the accumulators (spin 0: Ftot, Fsig; spin 2: FpN, FpS, FmN, FmS)."""
     x = np.asarray(x, dtype=np.float64)
-    tp, tm = _RowTables(x, 2, lmax, ms), _RowTables(x, -2, lmax, ms)
     R, T = len(ms), x.size
-    z = lambda: np.zeros((R, T))  # noqa: E731
-    pp, qp, sp_, pm, qm, sm = z(), z(), z(), z(), z(), z()
-    F = [np.zeros((R, T), complex) for _ in range(4)]
-    p2 = np.zeros((R, lmax + 1), complex)
-    m2 = np.zeros((R, lmax + 1), complex)
+    tabs = [_RowTables(x, s, lmax, ms) for s in ((0,) if spin == 0 else (2, -2))]
+    st = [[np.zeros((R, T)) for _ in range(3)] for _ in tabs]
+    F = [np.zeros((R, T), complex) for _ in range(2 if spin == 0 else 4)]
     with np.errstate(under="ignore", divide="ignore", invalid="ignore"):
         for l in range(lmax + 1):
-            pp, qp, sp_, lp = _masked_step(tp, l, x, pp, qp, sp_)
-            pm, qm, sm, lmn = _masked_step(tm, l, x, pm, qm, sm)
+            lams = []
+            for k, tb in enumerate(tabs):
+                *st[k], lam = _masked_step(tb, l, x, *st[k])
+                lams.append(lam)
             par = -1.0 if l % 2 else 1.0
-            pr = tp.pref[l]
-            cp = (bp[:, l] * pr)[:, None]
-            cm = (bm[:, l] * pr)[:, None]
-            F[0] += cp * lp
-            F[1] += par * cp * lmn
-            F[2] += cm * lmn
-            F[3] += par * cm * lp
-            p2[:, l] = pr * (np.einsum("rt,rt->r", lp, Vpn) + par * np.einsum("rt,rt->r", lmn, Sp))
-            m2[:, l] = pr * (np.einsum("rt,rt->r", lmn, Vmn) + par * np.einsum("rt,rt->r", lp, Sm))
-    return (*F, p2, m2)
+            pr = tabs[0].pref[l]
+            if spin == 0:
+                c = (coef[:, l] * pr)[:, None] * lams[0]
+                F[0] += c
+                F[1] += par * c
+            else:
+                cp = (coef[0][:, l] * pr)[:, None]
+                cm = (coef[1][:, l] * pr)[:, None]
+                F[0] += cp * lams[0]
+                F[1] += par * cp * lams[1]
+                F[2] += cm * lams[1]
+                F[3] += par * cm * lams[0]
+    return F
+
+
+def sampled_adjoint_scan(x, lmax, ms, spin, V):
+    """The fused adjoint scan restricted to the orders ``ms``, masked as the reference runs
+    it (spin 0: ``_recursion.py:472-497``, V = (Vn, Vs); spin 2: ``:546-572``,
+    V = (Vpn, Sp, Vmn, Sm), each (R, T)).  Returns the dense (R, L1) rows (spin 2: p2, m2)."""
+    x = np.asarray(x, dtype=np.float64)
+    R, T = len(ms), x.size
+    tabs = [_RowTables(x, s, lmax, ms) for s in ((0,) if spin == 0 else (2, -2))]
+    st = [[np.zeros((R, T)) for _ in range(3)] for _ in tabs]
+    out = [np.zeros((R, lmax + 1), complex) for _ in range(1 if spin == 0 else 2)]
+    with np.errstate(under="ignore", divide="ignore", invalid="ignore"):
+        for l in range(lmax + 1):
+            lams = []
+            for k, tb in enumerate(tabs):
+                *st[k], lam = _masked_step(tb, l, x, *st[k])
+                lams.append(lam)
+            par = -1.0 if l % 2 else 1.0
+            pr = tabs[0].pref[l]
+            dot = lambda a, b: np.einsum("rt,rt->r", a, b)  # noqa: E731
+            if spin == 0:
+                out[0][:, l] = pr * (dot(lams[0], V[0]) + par * dot(lams[0], V[1]))
+            else:
+                out[0][:, l] = pr * (dot(lams[0], V[0]) + par * dot(lams[1], V[1]))
+                out[1][:, l] = pr * (dot(lams[1], V[2]) + par * dot(lams[0], V[3]))
+    return out

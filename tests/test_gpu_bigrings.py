@@ -1,57 +1,57 @@
-"""Single-mode maps at nside 2048 (GPU): the ring-FFT launch classes that only exist on
-large grids (cap rings with 4096 < N < 8192 through the L = 16384 chirp-z class, the
-8192-point belt rings) against the closed form of one (l, m) mode.
+"""Single-mode spin-0 maps at nside 2048 / lmax 4096 (GPU) against their closed form on
+every pixel: every ring-FFT launch class of the large grid (cap rings with 4096 < N < 8192
+through the L = 16384 chirp-z class, power-of-two cap rings, the 8192-point belt rings,
+shifted and unshifted) and the polar cap rings where the exponent-tracked seeds of high m
+act.
 
-For spin 0 and a_lm = a on one (l, m > 0) only, the map on ring r is
-    f(theta_r, phi_j) = 2 Re(a lambda_lm(theta_r) e^{i m phi_j}),
-with lambda from the oracle's recursion (pinned against scipy and mpmath in
-tests/test_oracle.py).  Checked on rings of every class, relative to the map's peak
-(max |x - x_ref| / max |x_ref|): FP64 <= 1e-12, mixed <= 1e-5.
+For a_lm = a on one (l, m > 0) only, f(theta_r, phi_j) = 2 Re(a lambda_lm(theta_r)
+e^{i m phi_j}) (tests/_closed_form.py, lambda from the oracle's recursion).  FP64: three
+modes in one call, <= 1e-12; mixed: 16 modes in one call, so the default 16-map kernels
+run, <= 1e-5 (max |x - x_ref| / max |x_ref| per map).
 """
 
 from __future__ import annotations
 
-import numpy as np
+import warnings
+
 import pytest
 import torch
 
 from oracle import jht_oracle as O
 
+from _closed_form import PixelGeometry
+
 pytestmark = pytest.mark.gpu
 
 NSIDE, LMAX = 2048, 4096
+MODES = [(3000, 2500), (4096, 700), (1800, 1799), (4096, 4096), (2, 0), (4096, 0), (1, 1), (100, 3),
+         (2500, 2048), (4095, 4000), (1024, 1), (3333, 3000), (700, 699), (4096, 1), (2047, 1024), (3900, 3899)]
 
 
-def _rings():
-    # north cap rings in the L = 16384 chirp-z class (i > 1024, N = 4i not a power of two),
-    # a power-of-two cap ring, belt rings (N = 8192), the equator, and their mirrors
-    caps = [1100, 1500, 1999, 2047, 1024]
-    belt = [2048, 3000, 4095]
-    north = [i - 1 for i in caps + belt]
-    return north + [4 * NSIDE - 2 - r for r in north if r < 4 * NSIDE - 2 - r]
-
-
-@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("mixed", 1e-5)])
-@pytest.mark.parametrize("l,m", [(3000, 2500), (4096, 700), (1800, 1799)])
-def test_single_mode_big_rings(precision, tol, l, m):
+@pytest.mark.parametrize("precision,tol,modes", [("f64", 1e-12, MODES[:3]), ("mixed", 1e-5, MODES)])
+def test_single_modes_big_rings(precision, tol, modes):
     import jht_amd
+    from jht_amd import healpix as H
 
-    geo = O.RingInfo(NSIDE)
-    a = 0.7 - 0.4j
-    alm = np.zeros(O.alm_size(LMAX), dtype=np.complex128)
-    alm[O.alm_column_base(m, LMAX) + (l - m)] = a
-    out = jht_amd.synthesis_batch(torch.tensor(alm[None], device="cuda"), NSIDE, LMAX, 0, precision=precision)[0]
-    rings = _rings()
-    lam_all = O.normalized_legendre(geo.z, m, l)[l - m]  # lambda_lm on every ring
-    peak = 2.0 * abs(a) * float(np.max(np.abs(lam_all)))  # the map's max |value| (the metric's scale)
-    lam = lam_all[rings]
+    B, K = len(modes), O.alm_size(LMAX)
+    alm = torch.zeros((B, K), dtype=torch.complex128, device="cuda")
+    vals = []
+    for i, (l, m) in enumerate(modes):
+        v = complex(0.7, -0.4) if m > 0 else complex(0.7, 0.0)
+        vals.append(v)
+        alm[i, O.alm_column_base(m, LMAX) + l - m] = v
+    print(f"single modes spin 0 {precision} batch {B}", flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        out = jht_amd.synthesis_batch(alm, NSIDE, LMAX, 0, precision=precision)
+    pg = PixelGeometry(NSIDE)
     worst = 0.0
-    for r, lv in zip(rings, lam):
-        N = int(geo.npix_ring[r])
-        phi = geo.phi0[r] + 2 * np.pi * np.arange(N) / N
-        want = 2.0 * np.real(a * lv * np.exp(1j * m * phi))
-        got = out[int(geo.startpix[r]) : int(geo.startpix[r]) + N].cpu().numpy()
-        worst = max(worst, float(np.max(np.abs(got - want))))
-    e = worst / peak  # max |x - x_ref| / max |x_ref| over the map, as the reference's tests measure
-    print(f"single mode l={l} m={m} {precision}: rel {e:.2e} over {len(rings)} rings", flush=True)
-    assert e <= tol
+    for i, (l, m) in enumerate(modes):
+        want = pg.spin0(l, m, vals[i], precise=precision == "f64")
+        e = float((out[i] - want).abs().max() / want.abs().max())
+        print(f"  mode l={l} m={m} {precision}: rel {e:.2e}", flush=True)
+        worst = max(worst, e)
+    del out, pg
+    H.clear_plans()
+    torch.cuda.empty_cache()
+    assert worst <= tol

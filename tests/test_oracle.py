@@ -116,6 +116,83 @@ def test_oracle_vs_mpmath_high_l(m, theta):
         assert abs(lam[l - m] - ref) / scale <= 1e-12, (l, m)
 
 
+def _mpmath_spin_seed(x, m, s):
+    """_s lambda_{lmin,m}(theta), lmin = max(m, |s|), from the single-term Wigner seed with
+    every factor (factorials, cos(theta/2) and sin(theta/2) powers) in mpmath (dps 60)."""
+    import mpmath as mp
+
+    with mp.workdps(60):
+        M, Mp, L = -m, s, max(m, abs(s))
+        k0 = max(0, M - Mp)
+        fac = mp.sqrt(mp.factorial(L + M) * mp.factorial(L - M) * mp.factorial(L + Mp) * mp.factorial(L - Mp)) / (
+            mp.factorial(k0) * mp.factorial(L + M - k0) * mp.factorial(L - Mp - k0) * mp.factorial(k0 - M + Mp))
+        xv = mp.mpf(x)
+        d = (-1) ** k0 * fac * mp.sqrt((1 + xv) / 2) ** (2 * L - 2 * k0 + M - Mp) * mp.sqrt((1 - xv) / 2) ** (
+            2 * k0 - M + Mp)
+        return float((-1) ** m * mp.sqrt((2 * L + 1) / (4 * mp.pi)) * d)
+
+
+@pytest.mark.parametrize("m", [700, 2048, 4095, 4096])
+def test_precise_seed_vs_mpmath(m):
+    """The oracle's precise mode (exact log-factorial ratio, long double) reproduces the
+    spin-2 seed at high m to 1e-15 relative.  The reference's arithmetic (the default: a
+    float64 lgamma sum and a float64 log-scale) is off by up to ~1e-11 there."""
+    nside = 2048
+    z = O.RingInfo(nside).z
+    rings = [3000, 4094, 4095, 5000, 8190]  # |x| < 0.4: seeds of m >= 700 above 1e-60
+    worst_p = worst_f64 = 0.0
+    for s in (2, -2):
+        for r in rings:
+            ref = _mpmath_spin_seed(z[r], m, s)
+            if abs(ref) < 1e-280:
+                continue
+            pr = O.spin_weighted_lambda(z[r:r + 1], m, s, max(m, 2), precise=True)[0, 0]
+            f64 = O.spin_weighted_lambda(z[r:r + 1], m, s, max(m, 2))[0, 0]
+            worst_p = max(worst_p, abs(pr - ref) / abs(ref))
+            worst_f64 = max(worst_f64, abs(f64 - ref) / abs(ref))
+    print(f"m={m}: precise seed {worst_p:.1e}, float64 (reference arithmetic) {worst_f64:.1e}")
+    assert worst_p <= 1e-15
+
+
+def _mpmath_spin_column(x, m, s, lmax):
+    """_s lambda_{l,m}(x), l = max(m,|s|)..lmax: the Wigner three-term recursion from the
+    exact seed, in mpmath (dps 30)."""
+    import mpmath as mp
+
+    with mp.workdps(30):
+        M, Mp = -m, s
+        lmin = max(m, abs(s))
+        xv = mp.mpf(x)
+        d0 = mp.mpf(_mpmath_spin_seed(x, m, s)) / ((-1) ** m * mp.sqrt((2 * lmin + 1) / (4 * mp.pi)))
+        out, prev, cur = [d0], mp.mpf(0), d0
+        for l in range(lmin, lmax):
+            D = l * mp.sqrt(((l + 1) ** 2 - M * M) * ((l + 1) ** 2 - Mp * Mp))
+            A = mp.mpf((2 * l + 1) * l * (l + 1)) / D
+            C = mp.mpf(-(2 * l + 1) * M * Mp) / D
+            B = (l + 1) * mp.sqrt(mp.mpf((l * l - M * M) * (l * l - Mp * Mp))) / D
+            prev, cur = cur, (A * xv + C) * cur - B * prev
+            out.append(cur)
+        return np.array([float((-1) ** m * mp.sqrt((2 * (lmin + i) + 1) / (4 * mp.pi)) * v)
+                         for i, v in enumerate(out)])
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("m,s,ring", [(0, 2, 1200), (1, -2, 3000), (2048, 2, 4095)])
+def test_precise_column_vs_mpmath(m, s, ring):
+    """The precise oracle column to l = 4096 against an mpmath recursion (rel. to the
+    column's sqrt((2l+1)/4pi) scale): <= 1e-15 at every l."""
+    lmax = 4096
+    x = O.RingInfo(2048).z[ring]
+    ref = _mpmath_spin_column(x, m, s, lmax)
+    got = O.spin_weighted_lambda(np.array([x]), m, s, lmax, precise=True)[:, 0]
+    f64 = O.spin_weighted_lambda(np.array([x]), m, s, lmax)[:, 0]
+    scale = np.sqrt((2 * np.arange(max(m, 2), lmax + 1) + 1) / (4 * np.pi))
+    e_p = float(np.max(np.abs(got - ref) / scale))
+    e_f = float(np.max(np.abs(f64 - ref) / scale))
+    print(f"m={m} s={s} ring={ring}: precise {e_p:.1e}, float64 {e_f:.1e}")
+    assert e_p <= 1e-15
+
+
 @pytest.mark.parametrize("m", [0, 1, 2, 5, 10])
 def test_oracle_wigner_vs_explicit(m):
     theta = np.linspace(0.1, np.pi - 0.1, 15)
