@@ -435,3 +435,40 @@ def test_golden_fixture_reproduced(path):
     if "v" in d:
         b = O.adjoint_synthesis(d["v"], nside, lmax, spin)
         assert np.max(np.abs(b - d["adj"])) / np.max(np.abs(d["adj"])) <= 1e-14
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_sampled_scans_match_per_order_columns(spin):
+    """The masked sampled scans that the benchmark's CPU baseline times (the reference's
+    synthesis scan and adjoint scan over a subset of orders) give the contractions of the
+    per-order lambda columns, including m = 0 and the sectoral order."""
+    nside, lmax = 8, 20
+    x = O.RingInfo(nside).z[: 2 * nside]
+    T = x.size
+    rng = np.random.default_rng(40 + spin)
+    ms = np.array([0, 1, 3, 7, 20])
+    R = ms.size
+
+    def cplx(*shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    a, b = cplx(R, lmax + 1), cplx(R, lmax + 1)
+    V = [cplx(R, T) for _ in range(4)]
+    F = O.sampled_synthesis_scan(x, lmax, ms, spin, a if spin == 0 else (a, b))
+    out = O.sampled_adjoint_scan(x, lmax, ms, spin, V[:2] if spin == 0 else V)
+    for i, m in enumerate(ms):
+        lmin = max(m, spin)
+        par = (-1.0) ** np.arange(lmin, lmax + 1)
+        if spin == 0:
+            lam = O.normalized_legendre(x, m, lmax)
+            want = [(F[0][i], lam.T @ a[i, m:]), (F[1][i], lam.T @ (par * a[i, m:])),
+                    (out[0][i, m:], lam @ V[0][i] + par * (lam @ V[1][i]))]
+        else:
+            lp, lm = O.spin_weighted_lambda(x, m, 2, lmax), O.spin_weighted_lambda(x, m, -2, lmax)
+            want = [(F[0][i], lp.T @ a[i, lmin:]), (F[1][i], lm.T @ (par * a[i, lmin:])),
+                    (F[2][i], lm.T @ b[i, lmin:]), (F[3][i], lp.T @ (par * b[i, lmin:])),
+                    (out[0][i, lmin:], lp @ V[0][i] + par * (lm @ V[1][i])),
+                    (out[1][i, lmin:], lm @ V[2][i] + par * (lp @ V[3][i]))]
+        for got, ref in want:
+            assert np.all(np.isfinite(got))
+            assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref))), (spin, m)
