@@ -111,6 +111,24 @@ int jhtx_synth(jhtx_plan* plan, const void* alm, void* map, int batch, void* wor
 int jhtx_adjoint(jhtx_plan* plan, const void* map, void* alm, int batch, void* workspace,
                  void* stream);
 
+/* jhtx_adjoint without a workspace argument.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  The call allocates nothing and calls no
+ * allocating routine: the adjoint's ring coefficients and chunk partials go to a
+ * library-owned static device array of JHTX_NOALLOC_SCRATCH_BYTES per GPU, whose address is
+ * resolved when a plan is created on that GPU and which the GPU holds until the process
+ * ends.  A call needs what jhtx_adjoint uses of its workspace for `batch` maps: the ring
+ * coefficients (batch ncomp nrings (lmax+1) complex128, complex64 on mixed plans) and the
+ * recursion's ring-chunk partials (nchunk batch ncomp K, FP64 or FP32 pairs, nchunk =
+ * ceil(2 nside / 256) on mixed plans); above the maximum it fails with JHTX_EINVAL and says
+ * how much it needed (e.g. nside 512, lmax 1024, spin 0, one FP64 map: 42 MB).  Every call
+ * on one GPU shares that array and it keeps the last call's contents (nothing in it needs to
+ * start at zero): calls that overlap on one GPU -- two streams, or two host threads -- race
+ * and return wrong results.  The plan's device must be current.  Results equal
+ * jhtx_adjoint's bit for bit. */
+#define JHTX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int jhtx_adjoint_noalloc(jhtx_plan* plan, const void* map, void* alm, int batch, void* stream);
+
 /* Releases the plan's device tables (the plan must not be in use on any stream). */
 void jhtx_plan_destroy(jhtx_plan* plan);
 

@@ -19,6 +19,7 @@ EXPORTS = (
     "jhtx_plan_workspace_bytes",
     "jhtx_synth",
     "jhtx_adjoint",
+    "jhtx_adjoint_noalloc",
     "jhtx_plan_destroy",
     "jhtx_last_error",
     "jhtx_plan_query",
@@ -46,6 +47,7 @@ JHTX_ENOMEM = -4
 
 PREC_F64 = 0
 PREC_MIXED = 1
+NOALLOC_SCRATCH_BYTES = 64 << 20  # JHTX_NOALLOC_SCRATCH_BYTES
 FLAG_BLUESTEIN_ALL = 1
 
 
@@ -119,6 +121,7 @@ def load() -> ctypes.CDLL:
         "jhtx_nufft_interp": [vp, i, i, i, d, vp, ll, vp, vp, vp, vp, ctypes.c_size_t, vp],
         "jhtx_nufft_spread": [vp, vp, vp, ll, vp, i, i, i, d, vp, ctypes.c_size_t, vp],
         "jhtx_probe_peaks": [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), vp],
+        "jhtx_adjoint_noalloc": [vp, vp, vp, i, vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

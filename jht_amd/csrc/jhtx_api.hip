@@ -14,6 +14,7 @@
 #include "jhtx_internal.h"
 
 #include <initializer_list>
+#include <mutex>
 
 namespace jhtx {
 
@@ -285,6 +286,36 @@ struct StageTimer {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// ---------------------------------------------------------------------------------- //
+// Scratch of jhtx_adjoint_noalloc: a file-scope device array, one copy per GPU, held until
+// the process ends.  Its name is a host-side stand-in: host code uses only the device
+// address hipGetSymbolAddress returns for each device, resolved once per device when a
+// plan is created there (plan set-up, with that device current).
+// ---------------------------------------------------------------------------------- //
+static __device__ __align__(256) unsigned char scratch[JHTX_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices];          // device address of scratch per device
+hipError_t g_scratch_err[kMaxDevices];     // result of the lookup (hipSuccess once resolved)
+bool g_scratch_done[kMaxDevices];
+
+// Called at plan creation with the plan's device current.
+void resolve_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_done[dev]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[dev] = e;
+  g_scratch_ptr[dev] = (e == hipSuccess) ? ptr : nullptr;
+  g_scratch_done[dev] = true;
+}
+
+// Bytes of scratch an adjoint call of `batch` maps uses: the ring coefficients V, then the
+// recursion's chunk partials (the workspace layout of jhtx_adjoint, sized for this batch).
+size_t adjoint_scratch_bytes(const jhtx_plan* p, int batch, size_t* part_offset);
+
 // A one-map call of a mixed plan that runs the FP64 recursion kernels (mx_single).
 bool single_route(const jhtx_plan* p, int batch) {
   return p->precision == JHTX_PREC_MIXED && p->mx_single && batch == 1;
@@ -328,6 +359,19 @@ hipError_t launch_adjoint_rec_finish(const jhtx_plan* p, const void* F, void* pa
     t4.done();
   }
   return e;
+}
+
+size_t adjoint_scratch_bytes(const jhtx_plan* p, int batch, size_t* part_offset) {
+  const bool mixed = p->precision == JHTX_PREC_MIXED;
+  const size_t f_elem = mixed ? sizeof(float2) : sizeof(double2);
+  const size_t v_bytes = align256(f_elem * (size_t)batch * p->ncomp * p->nrings * p->M);
+  size_t part;
+  if (mixed && !single_route(p, batch))
+    part = sizeof(float2) * (size_t)mixed_adj_chunks(p) * batch * p->ncomp * p->K;
+  else
+    part = sizeof(double2) * (size_t)adj_rec_chunks(p) * batch * p->ncomp * p->K;
+  if (part_offset) *part_offset = v_bytes;
+  return v_bytes + align256(part);
 }
 
 int check_call(const jhtx_plan* p, const void* in, void* out, int batch, void* ws) {
@@ -501,6 +545,8 @@ int jhtx_plan_create_tuned(int nside, int lmax, int spin, int precision, int max
     return JHTX_EINVAL;
   }
   jhtx_plan* p = new jhtx_plan();  // value-initialised: every field starts at zero / null
+  (void)hipGetDevice(&p->device);
+  resolve_scratch(p->device);
   p->nside = nside;
   p->lmax = lmax;
   p->spin = spin;
@@ -562,6 +608,7 @@ int jhtx_plan_create_rings(int nrings, const double* theta, int lmax, int spin, 
     return JHTX_EINVAL;
   }
   jhtx_plan* p = new jhtx_plan();
+  (void)hipGetDevice(&p->device);
   p->ring_set = 1;
   p->lmax = lmax;
   p->spin = spin;
@@ -665,6 +712,55 @@ int jhtx_adjoint(jhtx_plan* p, const void* map, void* alm, int batch, void* ws, 
   if (e == hipSuccess) {
     e = launch_adjoint_rec_finish(p, V, part, alm, batch, s);
   }
+  if (e != hipSuccess) {
+    set_error(std::string("adjoint launch failed: ") + hipGetErrorString(e));
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+int jhtx_adjoint_noalloc(jhtx_plan* p, const void* map, void* alm, int batch, void* stream) {
+  static char dummy;
+  int rc = check_call(p, map, alm, batch, &dummy);
+  if (rc) return rc;
+  if (p->ring_set) {
+    set_error("plan has no HEALPix geometry (a ring-set plan): use jhtx_legendre_adjoint");
+    return JHTX_EINVAL;
+  }
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device) {
+    set_error("jhtx_adjoint_noalloc: the plan's device (" + std::to_string(p->device) + ") is not current");
+    return JHTX_EINVAL;
+  }
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (dev < kMaxDevices && g_scratch_done[dev] && g_scratch_err[dev] == hipSuccess) base = g_scratch_ptr[dev];
+    else if (dev < kMaxDevices && g_scratch_done[dev]) {
+      set_error(std::string("jhtx_adjoint_noalloc: scratch address unavailable: ") +
+                hipGetErrorString(g_scratch_err[dev]));
+      return JHTX_EDEVICE;
+    }
+  }
+  if (!base) {
+    set_error("jhtx_adjoint_noalloc: no scratch resolved for device " + std::to_string(dev));
+    return JHTX_EDEVICE;
+  }
+  size_t part_off = 0;
+  const size_t need = adjoint_scratch_bytes(p, batch, &part_off);
+  if (need > (size_t)JHTX_NOALLOC_SCRATCH_BYTES) {
+    set_error("jhtx_adjoint_noalloc: this call needs " + std::to_string(need) + " bytes of scratch, above the " +
+              std::to_string((size_t)JHTX_NOALLOC_SCRATCH_BYTES) +
+              "-byte maximum (use jhtx_adjoint with a workspace, or fewer maps per call)");
+    return JHTX_EINVAL;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  void* V = base;
+  void* part = reinterpret_cast<char*>(base) + part_off;
+  StageTimer t2(p, s, 2);
+  hipError_t e = launch_ring_adjoint(p, reinterpret_cast<const double*>(map), V, batch, s);
+  t2.done();
+  if (e == hipSuccess) e = launch_adjoint_rec_finish(p, V, part, alm, batch, s);
   if (e != hipSuccess) {
     set_error(std::string("adjoint launch failed: ") + hipGetErrorString(e));
     return JHTX_EDEVICE;

@@ -92,6 +92,7 @@ struct FftTables {
 
 struct jhtx_plan {
   int nside, lmax, spin, precision, max_batch, flags;
+  int device;    // the HIP device current at creation (its tables live there)
   int ring_set;  // 1: an arbitrary symmetric ring set (jhtx_plan_create_rings), no ring FFT
   int M, T, nrings, ncomp, nchunk;
   long long npix, K;
