@@ -322,6 +322,9 @@ def main():
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=4096)
     ap.add_argument("--spin", type=int, default=2)
+    ap.add_argument("--tuning", default="",
+                    help="kernel shapes for the mixed plan, jhtx_tuning fields as key=value,... "
+                         "(e.g. mx_synth_fold=1,mx_synth_group=32)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=2, help="orders m per host core in the CPU baseline sample")
@@ -397,7 +400,8 @@ def main():
     back = torch.empty_like(alm)
     torch.cuda.synchronize()
     t_plan = time.perf_counter()
-    plan = get_plan(nside, lmax, spin, B, precision=prec)
+    tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(",") if kv)} or None
+    plan = get_plan(nside, lmax, spin, B, precision=prec, tuning=tuning)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t_plan
     dt, per = run(plan, alm, maps, back, B, args.steps, args.warmup)

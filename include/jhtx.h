@@ -60,9 +60,9 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *   synth_rings, adj_rings   rings per lane of the FP64 recursion kernels: spin 0 {1,4,8},
  *                            spin 2 {1,2,4}
  *   block                    FP64 recursion work-group size cap {64,128,192,256}
- *   mx_synth_group           maps per recursion, mixed synthesis {4,8,16} (spin 0 also
- *                            32): spin 2 uses the value; spin 0 runs tile kernels for 16
- *                            and 32, else 8
+ *   mx_synth_group           maps per recursion, mixed synthesis {4,8,16,32} (32: spin 0,
+ *                            or spin 2 with mx_synth_fold): spin 2 uses the value; spin 0
+ *                            runs tile kernels for 16 and 32, else 8
  *   mx_synth_occ             its register budget in work-groups per CU {1,2,3} (spin 2)
  *   mx_adj_group, mx_adj_occ the same for the mixed adjoint {4,8,16}, {1,2}
  *   mx_adj_cb                16-map mixed spin-2 adjoint in column-block form (each wave
@@ -77,7 +77,10 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            scale {1 on (default), 2 off: FP32 16x16x4 tiles}
  *   mx_adj_f16               the same split for the 16-map mixed adjoint in column-block
  *                            form {1 on, 2 off}; default on for spin 2, off for spin 0
- *                            (there its l >= 2048 band error rises from 2.5e-6 to 6.1e-6) */
+ *                            (there its l >= 2048 band error rises from 2.5e-6 to 6.1e-6)
+ *   mx_synth_fold            spin-2 FP16-split synthesis with the north/south parity folded
+ *                            into the B operand (one recursion sign per wave, no even/odd
+ *                            accumulators; 16 or 32 maps per recursion) {1 on (default), 2 off} */
 typedef struct jhtx_tuning {
   int synth_rings, adj_rings, block;
   int mx_synth_group, mx_synth_occ;
@@ -86,6 +89,7 @@ typedef struct jhtx_tuning {
   int mx_single;
   int mx_f16;
   int mx_adj_f16;
+  int mx_synth_fold;
 } jhtx_tuning;
 
 /* jhtx_plan_create with explicit kernel shapes (tuning may be NULL: all defaults).  An

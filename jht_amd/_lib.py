@@ -52,7 +52,8 @@ FLAG_BLUESTEIN_ALL = 1
 
 
 TUNING_FIELDS = ("synth_rings", "adj_rings", "block", "mx_synth_group", "mx_synth_occ",
-                 "mx_adj_group", "mx_adj_occ", "mx_adj_cb", "mx_single", "mx_f16", "mx_adj_f16")
+                 "mx_adj_group", "mx_adj_occ", "mx_adj_cb", "mx_single", "mx_f16", "mx_adj_f16",
+                 "mx_synth_fold")
 
 
 class Tuning(ctypes.Structure):

@@ -411,7 +411,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // maps); spin 0 can also run 32 maps per recursion, by tuning: at 2 waves per SIMD instead
   // of 3 it measured +3 % at nside 2048 / batch 32 and -2 % on the ragged nside-512 batch
   const int group = p->max_batch >= 16 ? 16 : (p->max_batch >= 5 ? 8 : 4);
-  p->mx_synth_group = group;
+  // spin 2: 32 maps per synthesis recursion (the parity-folded FP16 kernel over 256 rings)
+  // once the plan holds 32: 101.2 ms per 32 maps against 2 x 59.2 ms at nside 2048 / lmax 4096
+  p->mx_synth_group = group;  // raised to 32 below for spin 2 when the plan holds 32 maps
   p->mx_synth_occ = 2;  // register budget of the 8-map kernels, in work-groups per CU
   p->mx_adj_group = group;
   p->mx_adj_occ = 2;
@@ -430,6 +432,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // / lmax 4096, batch 16, error against FP64 1.5e-7 -> 1.6e-7, default on.  Spin 0: 26.4 ->
   // 23.9 ms, but its l >= 2048 band error rises from 2.5e-6 to 6.1e-6: off by default
   p->mx_adj_f16 = p->spin == 2 ? 1 : 0;
+  // spin-2 FP16-split synthesis with the N/S parity folded into B: 60.9 -> 59.2 ms at
+  // nside 2048 / lmax 4096, batch 16 (3 waves per SIMD instead of 2), and the 32-map form
+  p->mx_synth_fold = 1;
   int block_cap = 256;
   if (tuning) {
     auto in = [](int v, std::initializer_list<int> ok) {
@@ -442,7 +447,8 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.synth_rings == 0 || (p->spin == 0 ? in(t.synth_rings, {1, 4, 8}) : in(t.synth_rings, {1, 2, 4}))) &&
         (t.adj_rings == 0 || (p->spin == 0 ? in(t.adj_rings, {1, 4, 8}) : in(t.adj_rings, {1, 2, 4}))) &&
         (t.block == 0 || in(t.block, {64, 128, 192, 256})) &&
-        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16}) || (p->spin == 0 && t.mx_synth_group == 32)) &&
+        (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16, 32})) &&
+        (t.mx_synth_fold == 0 || in(t.mx_synth_fold, {1, 2})) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
         (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) && (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) &&
         (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) && (t.mx_single == 0 || in(t.mx_single, {1, 2})) &&
@@ -462,7 +468,15 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
     if (t.mx_single) p->mx_single = (t.mx_single == 1);
     if (t.mx_f16) p->mx_f16 = (t.mx_f16 == 1);
     if (t.mx_adj_f16) p->mx_adj_f16 = (t.mx_adj_f16 == 1);
+    if (t.mx_synth_fold) p->mx_synth_fold = (t.mx_synth_fold == 1);
+    // 32 maps per recursion: spin 0's tile kernel, or spin 2's parity-folded FP16 kernel
+    if (p->mx_synth_group == 32 && p->spin == 2 && !(p->mx_synth_fold && p->mx_f16)) {
+      set_error("jhtx_tuning: 32 maps per spin-2 synthesis recursion needs mx_synth_fold = 1 and mx_f16 = 1");
+      return JHTX_EINVAL;
+    }
   }
+  if (p->spin == 2 && p->max_batch >= 32 && p->mx_synth_fold && p->mx_f16 && !(tuning && tuning->mx_synth_group))
+    p->mx_synth_group = 32;
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
     const int lanes = (p->T + R - 1) / R;
@@ -521,6 +535,7 @@ int jhtx_plan_get_tuning(const jhtx_plan* p, jhtx_tuning* t) {
   t->mx_f16 = p->mx_f16 ? 1 : 2;
   t->mx_adj_f16 = p->mx_adj_f16 ? 1 : 2;
   t->mx_adj_cb = p->mx_adj_cb ? 1 : 2;
+  t->mx_synth_fold = p->mx_synth_fold ? 1 : 2;
   return JHTX_OK;
 }
 

@@ -110,6 +110,7 @@ struct jhtx_plan {
   int mx_f16;          // 16-map synthesis on the FP16 matrix cores, operands split
                        // into hi + lo halves (0 or 1)
   int mx_adj_f16;      // the same for the 16-map column-block adjoints (0 or 1)
+  int mx_synth_fold;   // spin-2 FP16-split synthesis with the N/S parity folded into B (0 or 1)
   double mx_sA;        // power-of-two scale of the recursion values for the FP16 split
   double mx_gmax;      // max |G| over the triangle (bounds the staged a_lm G)
   jhtx::RecTables rec;

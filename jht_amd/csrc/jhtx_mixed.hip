@@ -901,6 +901,225 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const d
   }
 }
 
+// ---------------------------------------------------------------------------------- //
+// Spin-2 synthesis on the FP16 matrix cores with the N/S parity folded into the B operand.
+// Wave w runs one recursion sign (sg = w & 1) for the 64 rings of ring block w >> 1, so a
+// work-group of NW waves covers 32 NW rings.  The lanes of sign sg hold lambda^sg of their
+// north ring, which enters two sums per map: X = sum_l bX_l lambda^sg_l, the north ring of
+// the field X (bp for +2, bm for -2), and Y = sum_l (-1)^(l+m) bY_l lambda^sg_l, the south
+// ring of the other field (lambda^s(pi - theta) = (-1)^(l+m) lambda^-s(theta)).  B_sg
+// therefore carries [X.re, X.im, s Y.re, s Y.im] per map with s = (-1)^(l+m) folded in:
+// no even / odd accumulators (4 registers per 16 x 16 block instead of 8), and one
+// v_mfma_f32_16x16x32_f16 contracts 8 consecutive steps.  Operands are split into FP16
+// hi + lo halves after power-of-two scales, as in k_synth_tile_s2h: A halves (hi, lo, hi,
+// lo) of two steps from the wave's tile, B halves (hi, hi, lo, lo) of the same steps,
+// staged in LDS so that one 16-byte read is the B operand.
+// ---------------------------------------------------------------------------------- //
+constexpr int kP_GS = 8;   // l steps per pipeline group: one matrix K block
+constexpr int kP_CH = 32;  // l steps per staged chunk of B
+constexpr int kP_TS = 72;  // tile row stride (dwords): 2 kP_TS = 16 mod 64, conflict-free A reads
+
+template <int NCB, int NW, int MINB>
+__global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const double2* __restrict__ alm,
+                                                            const float* __restrict__ sB, double sA,
+                                                            float2* __restrict__ F, int batch) {
+  constexpr int MG = 4 * NCB;            // maps per work-group
+  constexpr int NCOL = 4 * MG;           // B columns per sign
+  constexpr int NT = 64 * NW;            // threads
+  constexpr int RINGS = 32 * NW;         // rings per work-group
+  constexpr int BROW = 4 * NCOL + 4;     // dwords per (sign, step pair) row of s_b; +4 rotates banks
+  constexpr int NLP = kP_CH / 2;         // step pairs per chunk
+  constexpr int SB_DW = 2 * NLP * BROW;  // s_b: [sign][step pair][column][(hh, ll) of 2 steps]
+  constexpr int TILE_DW = NW * 2 * kP_GS * kP_TS;  // [wave][buffer][step][lane]: (hi, lo)
+  constexpr int EP_DW = NW * 64 * 16;    // epilogue: [wave][ring][16 columns] floats
+  constexpr int BUF_DW = (SB_DW + TILE_DW > EP_DW) ? SB_DW + TILE_DW : EP_DW;
+  constexpr int NE = kP_CH * MG / NT;    // staged (l, map) entries per thread
+  static_assert(NE * NT == kP_CH * MG, "the threads cover a chunk's entries exactly");
+  static_assert(kP_CH % kP_GS == 0, "groups tile the chunk");
+  __shared__ __align__(16) uint32_t s_buf[BUF_DW];
+  __shared__ double2 s_ag[kP_CH][2];  // (alpha, +gamma), (alpha, -gamma)
+  __shared__ float s_sc[MG];
+  uint32_t* const s_b = s_buf;
+  int m, chunk;
+  if (!decode_block<false>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int sg = wave & 1;
+  const int ii = lane & 15, kk = lane >> 4;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb0 = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + 64 * (wave >> 1) + lane;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid && lmin <= lmax) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m + 4 * sg;
+    double sv;
+    seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
+    mup = -sv;
+  }
+  floatx4 D[4][NCB];  // [row block][column block]
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) D[rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint32_t* const tile0 = s_buf + SB_DW + wave * 2 * kP_GS * kP_TS;
+  uint32_t* const tile1 = tile0 + kP_GS * kP_TS;
+  if (tid < MG) s_sc[tid] = (mg * MG + tid < batch) ? sB[mg * MG + tid] : 0.f;
+  // staging: entry e = g kP_CH + j (map g, step j) -- consecutive threads read consecutive l
+  double2 pf_e[NE], pf_b[NE], pf_ag = make_double2(0.0, 0.0);
+  double pf_g[NE];
+  auto prefetch = [&](int l0n) {
+    if (tid < kP_CH) {
+      const int l = l0n + tid;
+      pf_ag = (l <= lmax && l != lmin) ? make_double2(A.alpha[cb0 + l], A.gamma[cb0 + l]) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + u * NT, j = e % kP_CH, g = e / kP_CH;
+      const int l = l0n + j, b = mg * MG + g;
+      const bool ok = l <= lmax && b < batch;
+      pf_g[u] = ok ? A.G[cb0 + l] : 0.0;
+      pf_e[u] = ok ? alm[((long long)b * 2 + 0) * A.K + cb0 + l] : make_double2(0.0, 0.0);
+      pf_b[u] = ok ? alm[((long long)b * 2 + 1) * A.K + cb0 + l] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store_b = [&](int l0c) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + u * NT, j = e % kP_CH, g = e / kP_CH;
+      const double gs = pf_g[u] * (double)s_sc[g];
+      const double er = pf_e[u].x * gs, ei = pf_e[u].y * gs, br = pf_b[u].x * gs, bi = pf_b[u].y * gs;
+      // bp = -(E + iB), bm = -(E - iB)
+      const uint32_t w[4] = {split_f16((float)(bi - er)), split_f16((float)(-ei - br)),
+                             split_f16((float)(-er - bi)), split_f16((float)(br - ei))};
+      // s = (-1)^(l+m): flip the sign bits of both halves of the folded copies
+      const uint32_t neg = (((l0c + j + m) & 1) != 0) ? 0x80008000u : 0u;
+      uint32_t* row = s_b + (j >> 1) * BROW + 4 * (4 * g) + 2 * (j & 1);
+      // sign +: [bp.re, bp.im, s bm.re, s bm.im]; sign -: [bm.re, bm.im, s bp.re, s bp.im]
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t wp = (c < 2) ? w[c] : (w[c] ^ neg);
+        const uint32_t wm = (c < 2) ? w[c + 2] : (w[c - 2] ^ neg);
+        *reinterpret_cast<uint2*>(row + 4 * c) = make_uint2(dup_lo(wp), dup_hi(wp));
+        *reinterpret_cast<uint2*>(row + NLP * BROW + 4 * c) = make_uint2(dup_lo(wm), dup_hi(wm));
+      }
+    }
+  };
+  // 8 steps of the wave's recursion into tile tl (the 1/0 mask in FP64 first: a lane below
+  // 2^-256 may hold a mantissa above FP32's range)
+  auto rec_group = [&](int jg, uint32_t* tl) {
+    const double f = (ex == 0) ? sA : 0.0;
+    double2 ag[kP_GS];
+#pragma unroll
+    for (int jj = 0; jj < kP_GS; ++jj) ag[jj] = s_ag[jg + jj][sg];
+#pragma unroll
+    for (int jj = 0; jj < kP_GS; ++jj) {
+      const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+      mup = mu;
+      mu = n;
+      tl[jj * kP_TS + lane] = split_f16((float)(n * f));
+    }
+    rescale_pair(mu, mup, ex);
+  };
+  // lane group kk takes steps 2 kk, 2 kk + 1 of the group: A halves (hi, lo, hi, lo) of each
+  // from the tile, B halves (hi, hi, lo, lo) of each from one 16-byte read of s_b
+  auto mfma_group = [&](int jg, const uint32_t* tl) {
+    halfx8 av[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const uint32_t a0 = tl[(2 * kk) * kP_TS + 16 * rb + ii];
+      const uint32_t a1 = tl[(2 * kk + 1) * kP_TS + 16 * rb + ii];
+      av[rb] = __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1));
+    }
+    const uint32_t* brow = s_b + sg * NLP * BROW + ((jg >> 1) + kk) * BROW + 4 * ii;
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      const halfx8 bv = __builtin_bit_cast(halfx8, *reinterpret_cast<const uint4*>(brow + 64 * c));
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[rb], bv, D[rb][c], 0, 0, 0);
+    }
+  };
+  if (lmin <= lmax) prefetch(lmin);
+  for (int l0 = lmin; l0 <= lmax; l0 += kP_CH) {
+    __syncthreads();
+    if (tid < kP_CH) {
+      s_ag[tid][0] = pf_ag;
+      s_ag[tid][1] = make_double2(pf_ag.x, -pf_ag.y);
+    }
+    store_b(l0);
+    __syncthreads();
+    if (l0 + kP_CH <= lmax) prefetch(l0 + kP_CH);
+    const int nst = min(kP_CH, (lmax - l0 + 1 + kP_GS - 1) & ~(kP_GS - 1));
+    int j0 = 0;
+    // every lane below 2^-256: recursion only (a lane's exponent never decreases)
+    while (j0 < nst && __ballot(valid && ex == 0) == 0ull) {
+      double2 ag[kP_GS];
+#pragma unroll
+      for (int jj = 0; jj < kP_GS; ++jj) ag[jj] = s_ag[j0 + jj][sg];
+#pragma unroll
+      for (int jj = 0; jj < kP_GS; ++jj) {
+        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+        mup = mu;
+        mu = n;
+      }
+      rescale_pair(mu, mup, ex);
+      j0 += kP_GS;
+    }
+    if (j0 < nst) {
+      rec_group(j0, tile0);
+      wave_sync();
+      bool odd = false;
+      for (; j0 + kP_GS < nst; j0 += kP_GS) {
+        mfma_group(j0, odd ? tile1 : tile0);
+        rec_group(j0 + kP_GS, odd ? tile0 : tile1);
+        wave_sync();
+        odd = !odd;
+      }
+      mfma_group(j0, odd ? tile1 : tile0);
+    }
+  }
+  // epilogue: per column block, the wave's D through LDS to (ring, map) rows; the scales
+  // are undone in FP64
+  __syncthreads();
+  float* ep = reinterpret_cast<float*>(s_buf) + wave * 64 * 16;
+  const long long ringM = A.M;
+  const int g4 = ii >> 2;  // map within the column block, for the lane's column
+  (void)g4;
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ep[(16 * rb + 4 * kk + r) * 16 + ii] = D[rb][c][r];
+    wave_sync();
+    if (valid) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int b = mg * MG + 4 * c + g;
+        if (b < batch) {
+          float2* Fx = F + ((long long)b * 2 + sg) * A.nrings * ringM;       // the field of this sign
+          float2* Fy = F + ((long long)b * 2 + (sg ^ 1)) * A.nrings * ringM; // the other field
+          const long long rn = (long long)t * ringM + m;
+          const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
+          if (lmin <= lmax) {
+            const double inv = 1.0 / (sA * (double)s_sc[4 * c + g]);  // a power of two: exact
+            const float* e = ep + lane * 16 + 4 * g;
+            Fx[rn] = f2(inv * e[0], inv * e[1]);
+            if (t < A.T - 1) Fy[rs] = f2(inv * e[2], inv * e[3]);
+          } else {
+            Fx[rn] = make_float2(0.f, 0.f);
+            if (t < A.T - 1) Fy[rs] = make_float2(0.f, 0.f);
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+}
+
 // Same software pipeline as k_synth_tile_s2: the matrix instructions of group j and the
 // recursion of group j + 1 (into the other tile buffer) form one straight-line block.
 
@@ -1855,19 +2074,26 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
 #pragma unroll
             for (int n = 0; n < 4; ++n) a[p][n] = src[n];
           }
-          // instruction n: rings 4 (2 n) + k and 4 (2 n + 1) + k of lane group k
+          // instruction pair n: rings 4 (4 n + i) + k, i = 0..3, of lane group k.  A lane's K
+          // slice is the 4 rings' (hi, lo) words as the tile holds them (no copies); the pair
+          // multiplies it by the (hi, hi) and by the (lo, lo) words of the rings' V values,
+          // which is the split's four products at two instructions per 16 rings
 #pragma unroll
-          for (int n = 0; n < 8; ++n) {
-            const uint32_t w0 = Bv[sg][16 * rb + 2 * n], w1 = Bv[sg][16 * rb + 2 * n + 1];
-            const halfx8 bq = __builtin_bit_cast(
-                halfx8, make_uint4(__builtin_amdgcn_perm(w0, w0, sel_hh), __builtin_amdgcn_perm(w0, w0, sel_ll),
-                                   __builtin_amdgcn_perm(w1, w1, sel_hh), __builtin_amdgcn_perm(w1, w1, sel_ll)));
+          for (int n = 0; n < 4; ++n) {
+            uint32_t hh[4], ll[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const uint32_t w = Bv[sg][16 * rb + 4 * n + i];
+              hh[i] = __builtin_amdgcn_perm(w, w, sel_hh);
+              ll[i] = __builtin_amdgcn_perm(w, w, sel_ll);
+            }
+            const halfx8 bh = __builtin_bit_cast(halfx8, make_uint4(hh[0], hh[1], hh[2], hh[3]));
+            const halfx8 bl = __builtin_bit_cast(halfx8, make_uint4(ll[0], ll[1], ll[2], ll[3]));
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-              const uint4 av4 = a[p][n >> 1];
-              const uint32_t a0 = (n & 1) ? av4.z : av4.x, a1 = (n & 1) ? av4.w : av4.y;
-              D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1)),
-                                                                bq, D[p][rb], 0, 0, 0);
+              const halfx8 av = __builtin_bit_cast(halfx8, a[p][n]);
+              D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bh, D[p][rb], 0, 0, 0);
+              D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bl, D[p][rb], 0, 0, 0);
             }
           }
         }
@@ -2283,9 +2509,26 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
                               void* scale_ws, hipStream_t s) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
-  const int MG = mixed_group(p, false);
+  int MG = mixed_group(p, false);
+  // the spin-2 32-map form is the parity-folded FP16 kernel, which needs the scale pass's
+  // workspace (jhtx_legendre_synth has none): otherwise 16 maps per recursion
+  if (p->spin == 2 && MG == 32 && !(p->mx_f16 && p->mx_synth_fold && scale_ws)) MG = 16;
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 2 && MG == 16) {
+  if (p->spin == 2 && (MG == 16 || MG == 32) && p->mx_f16 && p->mx_synth_fold && scale_ws) {
+    // parity folded into B: 16 maps x 128 rings (4 waves) or 32 maps x 256 rings (8 waves)
+    double* part = reinterpret_cast<double*>(scale_ws);
+    float* sB = reinterpret_cast<float*>(part + (size_t)kScaleBlocks * batch);
+    hipLaunchKernelGGL(k_alm_absmax, dim3(kScaleBlocks, batch), dim3(256), 0, s, alm, 2 * p->K, part);
+    hipLaunchKernelGGL(k_alm_scale, dim3(batch), dim3(256), 0, s, part, p->mx_gmax, sB);
+    RecArgs at = a;
+    const int rings = (MG == 32) ? 256 : 128;
+    at.nchunk = (p->T + rings - 1) / rings;
+    dim3 gt(Mpad * at.nchunk, (batch + MG - 1) / MG);
+    if (MG == 32)
+      hipLaunchKernelGGL((k_synth_s2p<8, 8, 1>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
+    else
+      hipLaunchKernelGGL((k_synth_s2p<4, 4, 3>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
+  } else if (p->spin == 2 && MG == 16) {
     RecArgs at = a;
     at.nchunk = (p->T + 127) / 128;
     dim3 gt(Mpad * at.nchunk, (batch + 15) / 16);

@@ -265,6 +265,37 @@ def test_mixed_f16_split_synthesis(nside, lmax, batch, amp, f16):
     assert es <= MIXED_TOL
 
 
+@pytest.mark.parametrize("group,fold", [(16, 1), (32, 1), (16, 2)])
+@pytest.mark.parametrize("nside,lmax,batch,amp", [(1, 2, 17, 1.0), (4, 8, 33, 1.0), (16, 32, 16, 1.0),
+                                                  (64, 128, 17, 1.0), (128, 256, 33, 1e-30), (128, 256, 32, 1e25),
+                                                  (256, 400, 40, 1.0), (512, 640, 32, 1.0)])
+def test_mixed_fold_synthesis(nside, lmax, batch, amp, group, fold):
+    """The spin-2 FP16-split synthesis with the N/S parity folded into B (fold 1: one
+    recursion sign per wave), 16 maps x 128 rings and 32 maps x 256 rings per work-group,
+    and the even/odd-accumulator form (fold 2, 16 maps), against the FP64 path: ragged map
+    groups, grids from 1 to 4 ring chunks, inputs far from unit scale, an all-zero map
+    (exact zeros)."""
+    j = _jht()
+    from jht_amd import healpix as H
+
+    with _tuned(nside, lmax, 2, "mixed", mx_synth_fold=fold, mx_synth_group=group):
+        rng = np.random.default_rng(4800 + nside + batch + group)
+        a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                      for _ in range(batch)]) * amp
+        a[1] = 0.0
+        at = torch.tensor(a, device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, 2, precision="f64").cpu().numpy()
+            smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed").cpu().numpy()
+            plan = H.get_plan(nside, lmax, 2, batch, H._prec("mixed"))
+            assert plan.tuning["mx_synth_fold"] == fold and plan.tuning["mx_synth_group"] == group
+    assert np.all(smx[1] == 0.0)
+    es = max(_rel(smx[i], s64[i]) for i in range(batch) if i != 1)
+    print(f"mixed fold={fold} group={group} nside={nside} lmax={lmax} B={batch} amp={amp:g}: synth {es:.2e}", flush=True)
+    assert es <= MIXED_TOL
+
+
 @pytest.mark.parametrize("f16", [1, 0])
 @pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
                                                   (128, 256, 16, 1e25), (256, 400, 32, 1.0)])
@@ -293,13 +324,15 @@ def test_mixed_f16_split_synthesis_spin0(nside, lmax, batch, amp, f16):
 
 
 @pytest.mark.parametrize("f16", [1, 0])
-@pytest.mark.parametrize("nside,lmax,batch,amp", [(16, 32, 16, 1.0), (64, 128, 17, 1.0), (128, 256, 16, 1e-30),
-                                                  (128, 256, 16, 1e25), (512, 640, 32, 1.0)])
+@pytest.mark.parametrize("nside,lmax,batch,amp", [(1, 2, 17, 1.0), (4, 8, 17, 1.0), (16, 32, 16, 1.0),
+                                                  (64, 128, 17, 1.0), (128, 256, 16, 1e-30), (128, 256, 16, 1e25),
+                                                  (512, 640, 32, 1.0)])
 def test_mixed_f16_split_adjoint(nside, lmax, batch, amp, f16):
     """The 16-map spin-2 column-block adjoint on the FP16 matrix cores (f16 1: hi + lo halves;
     the recursion values scaled by the plan's power of two, the V values by one per map and
-    work-group) and on the FP32 ones (f16 0) against the FP64 adjoint: ragged groups, inputs
-    far from unit scale, an all-zero map (exact zeros), and a grid with 4 ring chunks."""
+    work-group) and on the FP32 ones (f16 0) against the FP64 adjoint: ragged groups, tiny
+    grids, inputs far from unit scale, an all-zero map (exact zeros), and a grid with 4 ring
+    chunks."""
     j = _jht()
     from jht_amd import healpix as H
 

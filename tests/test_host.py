@@ -218,7 +218,7 @@ def test_tuning_validation_without_gpu():
     bad = L.Tuning(mx_adj_cb=3)
     rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
-    bad = L.Tuning(mx_synth_group=32)  # 32 maps per recursion: spin 0 only
+    bad = L.Tuning(mx_synth_group=32, mx_synth_fold=2)  # 32 maps per spin-2 recursion: only folded
     rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
     bad = L.Tuning(mx_single=3)
@@ -230,6 +230,10 @@ def test_tuning_validation_without_gpu():
     bad = L.Tuning(mx_adj_f16=3)
     rc = lib.jhtx_plan_create_tuned(16, 24, 0, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
     assert rc == L.JHTX_EINVAL and not h.value
+    for field in ("mx_synth_fold",):
+        bad = L.Tuning(**{field: 3})
+        rc = lib.jhtx_plan_create_tuned(16, 24, 2, 1, 1, 0, ctypes.byref(bad), ctypes.byref(h))
+        assert rc == L.JHTX_EINVAL and not h.value, field
 
 
 def test_offgrid_entry_validation_without_gpu():
