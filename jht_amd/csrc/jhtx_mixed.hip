@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
   __shared__ __align__(16) float s_a[CH][4][MG];  // [l][component: aE.re aE.im aB.re aB.im][map]
   __shared__ float s_ep[4][2][2][64][4];          // [wave][class][sign][ring][component]
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
   float* s_a = s_buf;                  // [l][16 cb + 4 g + component]
   float* tiles = s_buf + CH * AS;      // [wave][buffer][step][lane]
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r32 = lane & 31, sg = lane >> 5;  // ring within the wave, recursion sign
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(256) void k_synth_mx_s0(RecArgs A, const double2* _
   __shared__ __align__(16) float s_a[CH][4][NQ];  // [l][lane%4][q]
   __shared__ float s_ep[4][2][64][4];  // [wave][class][ring][lane%4]
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -709,7 +709,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const d
   uint32_t* s_b = s_buf;              // [l][2 column + {hi hi, lo lo}]
   uint32_t* tiles = s_buf + CH * ASH;  // [wave][buffer][step][lane]: (hi, lo)
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r32 = lane & 31, sg = lane >> 5;
@@ -913,7 +913,9 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const d
 // v_mfma_f32_16x16x32_f16 contracts 8 consecutive steps.  Operands are split into FP16
 // hi + lo halves after power-of-two scales, as in k_synth_tile_s2h: A halves (hi, lo, hi,
 // lo) of two steps from the wave's tile, B halves (hi, hi, lo, lo) of the same steps,
-// staged in LDS so that one 16-byte read is the B operand.
+// staged in LDS so that one 16-byte read is the B operand.  Work-groups take the grouped
+// order (decode_block<true>): the 16 consecutive orders m of one ring chunk run back to back
+// on one XCD, so their 8-byte stores F[ring][m] meet in that XCD's L2 as 128-byte runs.
 // ---------------------------------------------------------------------------------- //
 constexpr int kP_GS = 8;   // l steps per pipeline group: one matrix K block
 constexpr int kP_CH = 32;  // l steps per staged chunk of B
@@ -941,7 +943,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
   __shared__ float s_sc[MG];
   uint32_t* const s_b = s_buf;
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int sg = wave & 1;
@@ -1165,7 +1167,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
   float* s_a = s_buf;              // [l][2 map + (re, im)]
   float* tiles = s_buf + CH * AS;  // [wave][buffer][step][lane]
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ii = lane & 15, kk = lane >> 4;  // MFMA lane decomposition
@@ -1320,7 +1322,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0h(RecArgs A, const d
   uint32_t* s_b = s_buf;               // [l][2 column + {hi hi, lo lo}]
   uint32_t* tiles = s_buf + CH * ASH;  // [wave][buffer][step][lane]: (hi, lo)
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ii = lane & 15, kk = lane >> 4;

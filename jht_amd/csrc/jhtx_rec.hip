@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256, 2) void k_synth_s0(RecArgs A, const double2* _
   __shared__ double s_al[CH];
   __shared__ double2 s_a[CH];
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lmax = A.lmax;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_synth_s2(RecArgs A, const double2* __re
   __shared__ double2 s_e[CH];
   __shared__ double2 s_b[CH];
   int m, chunk;
-  if (!decode_block<false>(A, m, chunk)) return;
+  if (!decode_block<true>(A, m, chunk)) return;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lmax = A.lmax;
