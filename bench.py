@@ -4,7 +4,7 @@ Workload (the metric's configuration): spin 2, nside = 2048, lmax = 4096, E/B a_
 C_l^EE = (l+1)^-2, C_l^BB = 0.1 C_l^EE, l < 2 zero (synthetic, seeded).  One step is one
 synthesis S a -> (Q, U) followed by one adjoint S^T (Q, U) -> a' for every map of the
 batch; inputs are resident in HBM before the timed region.  `value` is maps/s of the
-selected precision (default: mixed, batch 16); the FP64 path (batch 1) is timed beside it
+selected precision (default: mixed, batch 32); the FP64 path (batch 1) is timed beside it
 and the mixed outputs of the first map are compared with the FP64 GPU outputs
 (max|x - x64| / max|x64|).  Prints one JSON line.
 
@@ -41,12 +41,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-# HBM bytes per call of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of
-# this script (each counter in its own run, scaled by the counters of a 1 GiB device copy:
-# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax, spin, batch, stage)
+# HBM bytes per call of the recursion kernels, measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
+# runs of this script (each counter in its own run, scaled by the counters of a 1 GiB device
+# copy in the same run: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); keyed by (precision, nside, lmax,
+# spin, batch, stage).  Not measured live: the JSON names the profile the value comes from, and
+# a kernel change needs a new profile.
+TRAFFIC_PROFILE = "profiles/kernel_stats_c3_spin2_mixed_batch32.md"
 MEASURED_TRAFFIC = {
-    ("mixed", 2048, 4096, 2, 16, "adj_rec"): 54.6e9,
-    ("mixed", 2048, 4096, 2, 16, "synth_rec"): 60.4e9,
+    ("mixed", 2048, 4096, 2, 32, "adj_rec"): 113.26e9,
+    ("mixed", 2048, 4096, 2, 32, "synth_rec"): 198.90e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X dense FP16 matrix
@@ -318,7 +321,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="mixed", choices=["f64", "mixed"])
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=4096)
     ap.add_argument("--spin", type=int, default=2)
@@ -450,8 +453,8 @@ def main():
     # one-map calls of a mixed plan run the FP64 vector-FMA recursion kernels (mx_single)
     fp64_kernels = prec == _lib.PREC_F64 or (B == 1 and plan.tuning["mx_single"] == 1)
     tun = plan.tuning
-    # the 16-map spin-2 kernels on the FP16 matrix cores (hi + lo split)
-    tile16 = prec != _lib.PREC_F64 and spin == 2 and B >= 16
+    # the 16- and 32-map kernels on the FP16 matrix cores (hi + lo split)
+    tile16 = prec != _lib.PREC_F64 and B >= 16
     f16 = {"synth_rec": tile16 and tun["mx_f16"] == 1,
            "adj_rec": tile16 and tun["mx_adj_f16"] == 1 and tun["mx_adj_cb"] == 1}
     peak_basis = None
@@ -485,7 +488,9 @@ def main():
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), **({"peak_basis": peak_basis} if peak_basis else {}),
-                     "traffic": MEASURED_TRAFFIC.get((args.precision, nside, lmax, spin, B, dom))},
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, nside, lmax, spin, B, dom)),
+                     **({"traffic_source": TRAFFIC_PROFILE}
+                        if (args.precision, nside, lmax, spin, B, dom) in MEASURED_TRAFFIC else {})},
         "stages_ms": {k: round(v[0], 3) for k, v in per.items()},
         "adjoint_identity": {"lhs": lhs, "rhs": rhs, "rel": ident},
         "plan_create_s": round(t_plan, 3),
