@@ -136,6 +136,48 @@ int jhtx_adjoint_noalloc(jhtx_plan* plan, const void* map, void* alm, int batch,
 /* Releases the plan's device tables (the plan must not be in use on any stream). */
 void jhtx_plan_destroy(jhtx_plan* plan);
 
+/* ---------------------------------------------------------------------------------- *
+ * Ragged groups: the transforms of several mixed-precision plans -- different nside, lmax
+ * and batch, one spin -- run as ONE kernel launch per stage.  The reference batches by
+ * jax.vmap over one geometry (scripts/gpu_check.py:150-153: vmap of jht.synthesis over
+ * realisations); a group also mixes geometries.  Every stage is a single launch whose
+ * work-groups cover all plans: synthesis = recursion + ring FFT (2 launches), adjoint =
+ * ring FFT + recursion + chunk reduction (3 launches), so a ragged batch captures into a
+ * HIP graph as five kernel nodes.  The contraction runs on the FP32 matrix cores (16 maps
+ * per recursion, FP32 in, FP32 accumulate, FP64 recursion); results agree with the plans'
+ * own FP32 matrix-core kernels (tuning mx_f16 = mx_adj_f16 = 2, 16-map groups) to FP32
+ * rounding and with the FP64 path to <= 1e-5.
+ * ---------------------------------------------------------------------------------- */
+typedef struct jhtx_group jhtx_group;
+
+/* A group over plans[0..n): n <= 16 HEALPix plans of precision JHTX_PREC_MIXED, one spin
+ * (0 or 2), nside <= 512 (every ring transform fits one 256 x 16 work-group), all on the
+ * current device; batch[i] (1 <= batch[i] <= max_batch of plan i) maps of plan i per call.
+ * The group uses the plans' tables, which must outlive it; it keeps no pointer to the
+ * arrays passed. */
+int jhtx_group_create(jhtx_plan* const* plans, const int* batch, int n, jhtx_group** group);
+
+/* Bytes of device workspace a group call needs (ring coefficients and chunk partials of
+ * every plan). */
+size_t jhtx_group_workspace_bytes(const jhtx_group* group);
+
+/* jhtx_synth of every plan: alm[i] (batch[i] x ncomp x K complex128) -> map[i]
+ * (batch[i] x ncomp x npix float64), in 2 kernel launches on `stream`. */
+int jhtx_group_synth(jhtx_group* group, const void* const* alm, void* const* map, void* workspace,
+                     void* stream);
+
+/* jhtx_adjoint of every plan: map[i] -> alm[i], in 3 kernel launches on `stream`. */
+int jhtx_group_adjoint(jhtx_group* group, const void* const* map, void* const* alm, void* workspace,
+                       void* stream);
+
+/* Group introspection: 0 plans, 1 spin, 2 kernel launches per synthesis call, 3 per adjoint
+ * call, 4 + s work-groups of stage s (s = 0..4 as in jhtx_plan_set_profiling). -1 if
+ * unknown. */
+long long jhtx_group_query(const jhtx_group* group, int what);
+
+/* Releases the group's descriptors (not its plans). */
+void jhtx_group_destroy(jhtx_group* group);
+
 /* Description of the last failure on the calling thread ("" if none). */
 const char* jhtx_last_error(void);
 

@@ -22,7 +22,7 @@ from .healpix import (  # noqa: F401
     synthesis,
     synthesis_batch,
 )
-from .graph import Instance, TransformGraph  # noqa: F401
+from .graph import Instance, RaggedGroup, TransformGraph  # noqa: F401
 from .offgrid import (  # noqa: F401
     adjoint_synthesis_general,
     adjoint_synthesis_general_real,

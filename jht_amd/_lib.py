@@ -37,6 +37,12 @@ EXPORTS = (
     "jhtx_nufft_spread",
     "jhtx_nufft_workspace_bytes",
     "jhtx_probe_peaks",
+    "jhtx_group_create",
+    "jhtx_group_workspace_bytes",
+    "jhtx_group_synth",
+    "jhtx_group_adjoint",
+    "jhtx_group_query",
+    "jhtx_group_destroy",
 )
 
 JHTX_OK = 0
@@ -49,6 +55,8 @@ PREC_F64 = 0
 PREC_MIXED = 1
 NOALLOC_SCRATCH_BYTES = 64 << 20  # JHTX_NOALLOC_SCRATCH_BYTES
 FLAG_BLUESTEIN_ALL = 1
+GROUP_MAX = 16  # plans per ragged group
+GROUP_NSIDE_MAX = 512
 
 
 TUNING_FIELDS = ("synth_rings", "adj_rings", "block", "mx_synth_group", "mx_synth_occ",
@@ -130,6 +138,18 @@ def load() -> ctypes.CDLL:
         fn.restype = ctypes.c_int
     lib.jhtx_nufft_workspace_bytes.argtypes = [ll, i, i]
     lib.jhtx_nufft_workspace_bytes.restype = ctypes.c_size_t
+    lib.jhtx_group_create.argtypes = [vp, vp, i, ctypes.POINTER(vp)]
+    lib.jhtx_group_create.restype = ctypes.c_int
+    lib.jhtx_group_workspace_bytes.argtypes = [vp]
+    lib.jhtx_group_workspace_bytes.restype = ctypes.c_size_t
+    for name in ("jhtx_group_synth", "jhtx_group_adjoint"):
+        fn = getattr(lib, name)
+        fn.argtypes = [vp, vp, vp, vp, vp]
+        fn.restype = ctypes.c_int
+    lib.jhtx_group_query.argtypes = [vp, i]
+    lib.jhtx_group_query.restype = ctypes.c_longlong
+    lib.jhtx_group_destroy.argtypes = [vp]
+    lib.jhtx_group_destroy.restype = None
     _LIB = lib
     return lib
 

@@ -13,7 +13,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["jhtx_api.hip", "jhtx_rec.hip", "jhtx_fft.hip", "jhtx_mixed.hip", "jhtx_offgrid.hip", "jhtx_peaks.hip"]
+SOURCES = ["jhtx_api.hip", "jhtx_rec.hip", "jhtx_fft.hip", "jhtx_mixed.hip", "jhtx_group.hip", "jhtx_offgrid.hip",
+           "jhtx_peaks.hip"]
 OUT = os.path.join(HERE, "libjhtx.so")
 OBJDIR = os.path.join(HERE, "_build")
 

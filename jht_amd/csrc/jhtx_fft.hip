@@ -472,21 +472,19 @@ __device__ __forceinline__ double2* fold_scratch(void* lds, int N) {
 
 // Work-group per mirror ring pair; template on the transform kind, spin and scalar type so
 // each instantiation holds one code path (register pressure).
-template <int BLOCK, int E, int SPIN, int KIND, typename T>
-__global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __restrict__ F,
-                                                      double* __restrict__ map) {
-  constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
+// Ring pair p, map b; spin 2 runs one ring per work-group (half 0 the north ring, 1 the
+// south one), spin 0 packs the pair into one complex transform.  LGLO..LGHI: the power-of-two
+// transform lengths the work-group shape covers.
+template <int BLOCK, int E, int LGLO, int LGHI, int SPIN, int KIND, typename T>
+__device__ __forceinline__ void ring_synth_body(const FftArgs& A, const cx<T>* __restrict__ F,
+                                                double* __restrict__ map, int p, int b, int half) {
   extern __shared__ double lds[];
   cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
-  // spin 2: one work-group per ring (blockIdx.x = 2 pair slot + member); spin 0 packs the
-  // pair into one complex transform
-  const int p = A.pairs[SPIN == 2 ? (blockIdx.x >> 1) : blockIdx.x];
-  const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int rn = p;
   const bool pair = p < 2 * A.nside - 1;
   const int rs = 4 * A.nside - 2 - p;
-  if (SPIN == 2 && (blockIdx.x & 1) && !pair) return;  // the equator has no mirror
+  if (SPIN == 2 && half && !pair) return;  // the equator has no mirror
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
@@ -515,7 +513,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
     const cx<T>* Fmb = Fpb + (long long)A.nrings * ringM;
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
-    const int r = (blockIdx.x & 1) ? rs : rn;
+    const int r = half ? rs : rn;
     ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
     const cx<T>* out = l2 + o;
@@ -529,18 +527,23 @@ __global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __
 }
 
 template <int BLOCK, int E, int SPIN, int KIND, typename T>
-__global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double* __restrict__ map,
-                                                        cx<T>* __restrict__ V) {
-  constexpr int LGLO = lg_lo(BLOCK, E), LGHI = lg_hi(BLOCK, E);
+__global__ __launch_bounds__(BLOCK) void k_ring_synth(FftArgs A, const cx<T>* __restrict__ F,
+                                                      double* __restrict__ map) {
+  const int slot = blockIdx.x;  // spin 2: 2 pair slot + member
+  const int p = A.pairs[SPIN == 2 ? (slot >> 1) : slot];
+  ring_synth_body<BLOCK, E, lg_lo(BLOCK, E), lg_hi(BLOCK, E), SPIN, KIND, T>(A, F, map, p, blockIdx.y, slot & 1);
+}
+
+template <int BLOCK, int E, int LGLO, int LGHI, int SPIN, int KIND, typename T>
+__device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double* __restrict__ map,
+                                                  cx<T>* __restrict__ V, int p, int b, int half) {
   extern __shared__ double lds[];
   cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
-  const int p = A.pairs[SPIN == 2 ? (blockIdx.x >> 1) : blockIdx.x];  // spin 2: one ring each
-  const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int rn = p;
   const bool pair = p < 2 * A.nside - 1;
   const int rs = 4 * A.nside - 2 - p;
-  if (SPIN == 2 && (blockIdx.x & 1) && !pair) return;  // the equator has no mirror
+  if (SPIN == 2 && half && !pair) return;  // the equator has no mirror
   const int N = ring_len(A.nside, rn);
   const bool sh = ring_shifted(A.nside, rn);
   int L;
@@ -577,7 +580,7 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   } else {
     const double* Q = map + (long long)b * 2 * A.npix;
     const double* U = Q + A.npix;
-    const bool h = blockIdx.x & 1;
+    const bool h = half;
     const int r = h ? rs : rn;
     const long long st = h ? ss : sn;
     for (int j = tid; j < N; j += BLOCK)
@@ -597,6 +600,14 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
       Vm[m] = cmul(cconj(W[(N - k) % N]), t);
     }
   }
+}
+
+template <int BLOCK, int E, int SPIN, int KIND, typename T>
+__global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double* __restrict__ map,
+                                                        cx<T>* __restrict__ V) {
+  const int slot = blockIdx.x;  // spin 2: 2 pair slot + member
+  const int p = A.pairs[SPIN == 2 ? (slot >> 1) : slot];
+  ring_adjoint_body<BLOCK, E, lg_lo(BLOCK, E), lg_hi(BLOCK, E), SPIN, KIND, T>(A, map, V, p, blockIdx.y, slot & 1);
 }
 
 // Chirp table c_n = e^{i pi (n^2 mod 2N) / N}, n < N, of each Bluestein ring pair.
@@ -734,6 +745,75 @@ void launch_spectrum(int lc, const FftArgs& a, dim3 grid, size_t sh) {
     case 1: hipLaunchKernelGGL((k_bluestein_spectrum<256, 16>), grid, dim3(256), sh, 0, a); break;
     case 2: hipLaunchKernelGGL((k_bluestein_spectrum<512, 16>), grid, dim3(512), sh, 0, a); break;
     default: hipLaunchKernelGGL((k_bluestein_spectrum<512, 32>), grid, dim3(512), sh, 0, a); break;
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// ragged groups: every ring of every plan in one launch of 256-thread work-groups
+// ---------------------------------------------------------------------------------- //
+// Per plan (device-resident).  Work-group shape 256 x 16 covers transform lengths 64..4096
+// (nside <= 512); each work-group takes its ring's kind at run time.
+constexpr int kGroupBlock = 256, kGroupE = 16, kGroupLgLo = 6, kGroupLgHi = 12;
+static_assert((1 << kGroupLgHi) == kGroupBlock * kGroupE, "the group shape holds L = 4096");
+
+struct GroupFft {
+  FftArgs A;
+  long long f_off;
+};
+
+// Work-group order within a plan: map b, then ring slots from the equator to the poles
+// (largest transforms first, so the cheap polar rings fill the launch's tail).
+__device__ __forceinline__ void group_ring_slot(const FftArgs& A, int spin, int local, int& p, int& b, int& half) {
+  const int per = (spin == 2 ? 4 : 2) * A.nside;  // spin 2: a work-group per ring of each pair
+  const int slot = per - 1 - local % per;
+  b = local / per;
+  p = spin == 2 ? (slot >> 1) : slot;
+  half = slot & 1;
+}
+
+template <int SPIN>
+__global__ __launch_bounds__(kGroupBlock) void k_group_ring_synth(const GroupFft* __restrict__ G, GroupCall C) {
+  int local;
+  const int i = group_member(C, local);
+  const FftArgs A = G[i].A;
+  int p, b, half;
+  group_ring_slot(A, SPIN, local, p, b, half);
+  const float2* F = reinterpret_cast<const float2*>(C.ws + G[i].f_off);
+  double* map = reinterpret_cast<double*>(C.out[i]);
+  int L;
+  switch (ring_kind(ring_len(A.nside, p), A.bluestein_all, &L)) {
+    case 0:
+      ring_synth_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 0, float>(A, F, map, p, b, half);
+      break;
+    case 1:
+      ring_synth_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 1, float>(A, F, map, p, b, half);
+      break;
+    default:
+      ring_synth_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 2, float>(A, F, map, p, b, half);
+      break;
+  }
+}
+
+template <int SPIN>
+__global__ __launch_bounds__(kGroupBlock) void k_group_ring_adjoint(const GroupFft* __restrict__ G, GroupCall C) {
+  int local;
+  const int i = group_member(C, local);
+  const FftArgs A = G[i].A;
+  int p, b, half;
+  group_ring_slot(A, SPIN, local, p, b, half);
+  const double* map = reinterpret_cast<const double*>(C.in[i]);
+  float2* V = reinterpret_cast<float2*>(C.ws + G[i].f_off);
+  int L;
+  switch (ring_kind(ring_len(A.nside, p), A.bluestein_all, &L)) {
+    case 0:
+      ring_adjoint_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 0, float>(A, map, V, p, b, half);
+      break;
+    case 1:
+      ring_adjoint_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 1, float>(A, map, V, p, b, half);
+      break;
+    default:
+      ring_adjoint_body<kGroupBlock, kGroupE, kGroupLgLo, kGroupLgHi, SPIN, 2, float>(A, map, V, p, b, half);
+      break;
   }
 }
 
@@ -916,6 +996,68 @@ hipError_t launch_ring_synth(const jhtx_plan* p, const void* F, double* map, int
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, int batch,
                                hipStream_t s) {
   return launch_ring(p, false, map, V, batch, s);
+}
+
+
+// ---- ragged groups (host) ----
+int group_fft_build(const GroupLayout& L, void** desc, int* start, int* lds_bytes) {
+  std::vector<GroupFft> h(L.n);
+  int lds = 512;
+  start[0] = 0;
+  for (int i = 0; i < L.n; ++i) {
+    const jhtx_plan* p = L.plan[i];
+    FftClass none;
+    h[i].A = make_fft_args(p, none);
+    h[i].A.pairs = nullptr;  // the group kernels enumerate every ring pair
+    h[i].f_off = L.f_off[i];
+    const int ball = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
+    for (int q = 0; q < 2 * p->nside; ++q) {
+      const int N = ring_len(p->nside, q);
+      int Lt;
+      const int kind = ring_kind(N, ball, &Lt);
+      if (Lt > kGroupBlock * kGroupE) {
+        set_error("ragged group: ring transform length " + std::to_string(Lt) + " above " +
+                  std::to_string(kGroupBlock * kGroupE) + " (nside <= " + std::to_string(kGroupNside) + ")");
+        return JHTX_EUNSUPPORTED;
+      }
+      // LDS as for the one-plan launch classes, at the group's 256-thread shape (complex64)
+      int nd = (kind == 0) ? 3 * N * 8 : Lt * 8;
+      if (N <= kGroupBlock) nd = std::max(nd, ((N * 8 + 15) & ~15) + 2 * kGroupBlock * 16);
+      lds = std::max(lds, std::max(nd, N * 8));
+    }
+    const long long n = (long long)start[i] + (long long)(p->spin == 2 ? 4 : 2) * p->nside * L.batch[i];
+    if (n > 0x7fffffffLL) {
+      set_error("ragged group: too many ring work-groups for one launch");
+      return JHTX_EINVAL;
+    }
+    start[i + 1] = (int)n;
+  }
+  for (int i = 0; i < L.n; ++i) h[i].A.lds_bytes = lds;
+  *lds_bytes = lds;
+  if (hipMalloc(desc, sizeof(GroupFft) * L.n) != hipSuccess) return JHTX_ENOMEM;
+  if (hipMemcpy(*desc, h.data(), sizeof(GroupFft) * L.n, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("ragged group: descriptor upload failed");
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+hipError_t group_fft_launch(bool synth, const GroupLayout& L, const void* desc, GroupCall C, int lds_bytes,
+                            hipStream_t s) {
+  const GroupFft* G = reinterpret_cast<const GroupFft*>(desc);
+  const dim3 grid((unsigned)C.start[C.n]);
+  if (synth) {
+    if (L.spin == 0)
+      hipLaunchKernelGGL(k_group_ring_synth<0>, grid, dim3(kGroupBlock), lds_bytes, s, G, C);
+    else
+      hipLaunchKernelGGL(k_group_ring_synth<2>, grid, dim3(kGroupBlock), lds_bytes, s, G, C);
+  } else {
+    if (L.spin == 0)
+      hipLaunchKernelGGL(k_group_ring_adjoint<0>, grid, dim3(kGroupBlock), lds_bytes, s, G, C);
+    else
+      hipLaunchKernelGGL(k_group_ring_adjoint<2>, grid, dim3(kGroupBlock), lds_bytes, s, G, C);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace jhtx

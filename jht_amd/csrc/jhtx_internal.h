@@ -157,6 +157,56 @@ hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, i
 int build_fft_tables(jhtx_plan* p);
 void free_fft_tables(jhtx_plan* p);
 
+// ---- Ragged groups: the transforms of several plans, one kernel launch per stage ----
+// A group call runs five stages (synthesis recursion, synthesis ring FFT; adjoint ring FFT,
+// adjoint recursion, chunk reduction), each as ONE launch whose work-groups cover every
+// plan: work-group w belongs to plan i when start[i] <= w < start[i + 1], and its index
+// within the plan's range plays the role of the plan's own launch coordinates.  Ranges of
+// the recursion stages start at multiples of 8, so the XCD-aware order inside each plan
+// (work-group & 7 = XCD) is kept.
+constexpr int kGroupMax = 16;   // plans per group
+constexpr int kGroupNside = 512;  // largest nside: every ring transform fits 256 x 16 elements
+
+// host-side layout of a group, in launch order (plans sorted by decreasing lmax)
+struct GroupLayout {
+  int n = 0, spin = 0;
+  const jhtx_plan* plan[kGroupMax] = {};
+  int batch[kGroupMax] = {};
+  long long f_off[kGroupMax] = {};     // bytes: the plan's F / V in the group workspace
+  long long part_off[kGroupMax] = {};  // bytes: its adjoint chunk partials
+};
+
+// Kernel argument of every group launch (by value): the stage's work-group ranges and the
+// call's buffers, per plan in launch order.
+struct GroupCall {
+  int n;
+  int start[kGroupMax + 1];
+  const void* in[kGroupMax];
+  void* out[kGroupMax];
+  char* ws;
+};
+
+// Plan of work-group w of a group launch, and w's index within that plan's range.
+__device__ __forceinline__ int group_member(const GroupCall& C, int& local) {
+  const int w = blockIdx.x;
+  int i = 0;
+  while (i + 1 < C.n && w >= C.start[i + 1]) ++i;
+  local = w - C.start[i];
+  return i;
+}
+
+// Stage builders (upload per-plan descriptors, fill the stage's work-group ranges) and
+// launchers; *desc is a device array freed with hipFree.
+int group_rec_build(const GroupLayout& L, void** desc, int* start_synth, int* start_adj);
+hipError_t group_rec_launch(bool synth, const GroupLayout& L, const void* desc, GroupCall C, hipStream_t s);
+int group_fft_build(const GroupLayout& L, void** desc, int* start, int* lds_bytes);
+hipError_t group_fft_launch(bool synth, const GroupLayout& L, const void* desc, GroupCall C, int lds_bytes,
+                            hipStream_t s);
+int group_fin_build(const GroupLayout& L, void** desc, int* start);
+hipError_t group_fin_launch(const void* desc, GroupCall C, hipStream_t s);
+// ring chunks of the group adjoint recursion (256 rings each) and of the synthesis
+int group_adj_chunks(const jhtx_plan* p);
+
 // ring geometry helpers (host and device)
 __host__ __device__ inline int ring_len(int nside, int r /*0-based*/) {
   const int i = r + 1;

@@ -15,6 +15,8 @@
 // rows against 4 rings per instruction, so the matrix core performs the ring reduction.  The
 // per-wave results are summed over the work-group in FP64 and written as one FP32 partial per
 // ring chunk (they are sums of FP32 tiles); the chunk reduction then finishes the sum in FP64.
+#include <vector>
+
 #include "jhtx_internal.h"
 
 namespace jhtx {
@@ -55,9 +57,11 @@ __device__ __forceinline__ double spin2_seed_L(const double* sp, double lc, doub
 // groups alternating between XCDs, and within a group the work-groups of one ring chunk
 // run back to back -- they read adjacent m of the same ring rows of V, so those cache
 // lines are shared in the XCD's L2 instead of being fetched once per m (7x less fetch).
+// P: the work-group's slot, blockIdx.x of a one-plan launch (a ragged group launch passes
+// the slot within its plan's range, which starts at a multiple of 8, so P & 7 is still the
+// XCD).
 template <bool GROUPED>
-__device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
-  const int P = blockIdx.x;
+__device__ __forceinline__ bool decode_block(const RecArgs& A, int P, int& m, int& chunk) {
   const int xcd = P & 7;
   const int slot = P >> 3;
   if (GROUPED) {
@@ -70,6 +74,11 @@ __device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chun
     m = (slot / A.nchunk) * 8 + xcd;
   }
   return m < A.M;
+}
+
+template <bool GROUPED>
+__device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chunk) {
+  return decode_block<GROUPED>(A, (int)blockIdx.x, m, chunk);
 }
 
 // ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
@@ -317,9 +326,9 @@ __global__ __launch_bounds__(256, MINB) void k_synth_mx_s2(RecArgs A, const doub
 // FP64 instructions per lane, for the same matrix-core work.
 constexpr int kTileTS = 72;  // tile row stride (floats): A reads at rows 2k+p, bank 16k+i
 
-template <int NCB, int MINB>
-__global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const double2* __restrict__ alm,
-                                                             float2* __restrict__ F, int batch) {
+template <int NCB>
+__device__ __forceinline__ void synth_tile_s2_body(const RecArgs& A, const double2* __restrict__ alm,
+                                                   float2* __restrict__ F, int batch, int P, int mg) {
   constexpr int MG = 4 * NCB;
   constexpr int AS = 4 * MG + 8;  // s_a row stride (floats): B reads at rows 2k+p, bank 16k+j
   constexpr int RINGS = 128;      // rings per work-group (4 waves x 32)
@@ -331,8 +340,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
   float* s_a = s_buf;                  // [l][16 cb + 4 g + component]
   float* tiles = s_buf + CH * AS;      // [wave][buffer][step][lane]
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
-  const int mg = blockIdx.y;
+  if (!decode_block<true>(A, P, m, chunk)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r32 = lane & 31, sg = lane >> 5;  // ring within the wave, recursion sign
   const int ii = lane & 15, kk = lane >> 4;   // MFMA lane decomposition
@@ -498,6 +506,12 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const do
     }
     wave_sync();
   }
+}
+
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s2(RecArgs A, const double2* __restrict__ alm,
+                                                             float2* __restrict__ F, int batch) {
+  synth_tile_s2_body<NCB>(A, alm, F, batch, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------- //
@@ -1154,9 +1168,9 @@ __device__ __forceinline__ void store_alm_s0(const double2 (&a)[MG / 4], double 
 }
 
 // NCB column blocks of 8 maps: 2 (16 maps, 64 accumulator registers) or 4 (32 maps, 128)
-template <int NCB, int MINB>
-__global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const double2* __restrict__ alm,
-                                                             float2* __restrict__ F, int batch) {
+template <int NCB>
+__device__ __forceinline__ void synth_tile_s0_body(const RecArgs& A, const double2* __restrict__ alm,
+                                                   float2* __restrict__ F, int batch, int P, int mg) {
   constexpr int MG = 8 * NCB, AS = 16 * NCB + 8, RINGS = 256;
   static_assert(AS % 64 == 8 || AS % 64 == 40, "B reads at rows 2k+p land on banks 16k+j");
   constexpr int EPS = 17;  // epilogue staging row stride (floats)
@@ -1167,8 +1181,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
   float* s_a = s_buf;              // [l][2 map + (re, im)]
   float* tiles = s_buf + CH * AS;  // [wave][buffer][step][lane]
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
-  const int mg = blockIdx.y;
+  if (!decode_block<true>(A, P, m, chunk)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ii = lane & 15, kk = lane >> 4;  // MFMA lane decomposition
   const int lmax = A.lmax;
@@ -1298,6 +1311,12 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const do
     }
     wave_sync();
   }
+}
+
+template <int NCB, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_synth_tile_s0(RecArgs A, const double2* __restrict__ alm,
+                                                             float2* __restrict__ F, int batch) {
+  synth_tile_s0_body<NCB>(A, alm, F, batch, blockIdx.x, blockIdx.y);
 }
 
 // k_synth_tile_s0<NCB, MINB> on the FP16 matrix cores, with the operand split, scales and
@@ -1735,15 +1754,14 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_adj_mx_s2(RecArgs A, const fl
 // of the group) over the 256 rings, reading all four tiles.  Every wave then does the same
 // matrix work and skips the same all-zero (ring block, sign) tiles, and its accumulators are
 // the chunk partials of its columns.
-__global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
-                                                        float2* __restrict__ part, int batch) {
+__device__ __forceinline__ void adj_mx_s2_cb_body(const RecArgs& A, const float2* __restrict__ V,
+                                                  float2* __restrict__ part, int batch, int P, int mg) {
   constexpr int NW = 4, MG = 16, RINGS = 256;
   __shared__ double2 s_ag[CH];
   __shared__ __align__(16) float tiles[NW][2][TL][TS2];  // [ring block][sign][row][col]
   __shared__ int s_act[NW];
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
-  const int mg = blockIdx.y;
+  if (!decode_block<true>(A, P, m, chunk)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
   const int lmin = max(m, 2);
@@ -1902,6 +1920,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2
       __syncthreads();
     }
   }
+}
+
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cb(RecArgs A, const float2* __restrict__ V,
+                                                        float2* __restrict__ part, int batch) {
+  adj_mx_s2_cb_body(A, V, part, batch, blockIdx.x, blockIdx.y);
 }
 
 // k_adj_mx_s2_cb<1> on the FP16 matrix cores.  The tiles hold (hi, lo) FP16 pairs of the
@@ -2221,15 +2244,14 @@ __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __re
 // matrix work and skips the same all-zero ring-block tiles; no cross-wave reduction.  Each
 // 64-ring block accumulates in FP32 (two chains of 32 consecutive rings) and the chains
 // are summed in FP64.
-__global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cb(RecArgs A, const float2* __restrict__ V,
-                                                         float2* __restrict__ part, int batch) {
+__device__ __forceinline__ void adj_mx_s0_cb_body(const RecArgs& A, const float2* __restrict__ V,
+                                                  float2* __restrict__ part, int batch, int P, int mg) {
   constexpr int NW = 4, MG = 16, RINGS = 256;
   __shared__ double s_al[CH];
   __shared__ __align__(16) float tiles[NW][TL][TS2];  // [ring block][row][col]
   __shared__ int s_act[NW];
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
-  const int mg = blockIdx.y;
+  if (!decode_block<true>(A, P, m, chunk)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
@@ -2333,6 +2355,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cb(RecArgs A, const float2
       __syncthreads();
     }
   }
+}
+
+__global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cb(RecArgs A, const float2* __restrict__ V,
+                                                         float2* __restrict__ part, int batch) {
+  adj_mx_s0_cb_body(A, V, part, batch, blockIdx.x, blockIdx.y);
 }
 
 // k_adj_mx_s0_cb on the FP16 matrix cores, with the split, the local V scale and the
@@ -2473,6 +2500,48 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cbh(RecArgs A, const float
   }
 }
 
+// ---------------------------------------------------------------------------------- //
+// ragged groups: the FP32 matrix-core kernels of several plans in one launch per stage
+// ---------------------------------------------------------------------------------- //
+// Per plan (device-resident): recursion arguments of the two stages (their ring-chunk
+// widths differ for spin 2) and the workspace offsets.
+struct GroupRec {
+  RecArgs synth, adj;
+  long long f_off, part_off;
+  int batch, synth_gx, adj_gx;
+};
+
+// 16 maps per recursion (the last group of a plan ragged): k_synth_tile_s0<2, 2> or
+// k_synth_tile_s2<4, 2> per work-group.
+template <int SPIN>
+__global__ __launch_bounds__(256, 2) void k_group_synth(const GroupRec* __restrict__ G, GroupCall C) {
+  int local;
+  const int i = group_member(C, local);
+  const RecArgs A = G[i].synth;
+  const int gx = G[i].synth_gx, batch = G[i].batch;
+  float2* F = reinterpret_cast<float2*>(C.ws + G[i].f_off);
+  const double2* alm = reinterpret_cast<const double2*>(C.in[i]);
+  if constexpr (SPIN == 0)
+    synth_tile_s0_body<2>(A, alm, F, batch, local % gx, local / gx);
+  else
+    synth_tile_s2_body<4>(A, alm, F, batch, local % gx, local / gx);
+}
+
+// k_adj_mx_s0_cb / k_adj_mx_s2_cb per work-group (16 maps, 256 rings)
+template <int SPIN>
+__global__ __launch_bounds__(256, 2) void k_group_adj(const GroupRec* __restrict__ G, GroupCall C) {
+  int local;
+  const int i = group_member(C, local);
+  const RecArgs A = G[i].adj;
+  const int gx = G[i].adj_gx, batch = G[i].batch;
+  const float2* V = reinterpret_cast<const float2*>(C.ws + G[i].f_off);
+  float2* part = reinterpret_cast<float2*>(C.ws + G[i].part_off);
+  if constexpr (SPIN == 0)
+    adj_mx_s0_cb_body(A, V, part, batch, local % gx, local / gx);
+  else
+    adj_mx_s2_cb_body(A, V, part, batch, local % gx, local / gx);
+}
+
 }  // namespace
 
 RecArgs make_args_mixed(const jhtx_plan* p) {
@@ -2591,6 +2660,61 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
     hipLaunchKernelGGL((k_adj_mx_s2<2, 1, 4>), grid, dim3(256), 0, s, a, V, part, batch);
   else
     hipLaunchKernelGGL((k_adj_mx_s2<1, 1, 4>), grid, dim3(256), 0, s, a, V, part, batch);
+  return hipGetLastError();
+}
+
+
+// ---- ragged groups (host) ----
+int group_adj_chunks(const jhtx_plan* p) { return (p->T + 255) / 256; }
+
+int group_rec_build(const GroupLayout& L, void** desc, int* start_synth, int* start_adj) {
+  std::vector<GroupRec> h(L.n);
+  start_synth[0] = start_adj[0] = 0;
+  for (int i = 0; i < L.n; ++i) {
+    const jhtx_plan* p = L.plan[i];
+    const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
+    const int groups = (L.batch[i] + 15) / 16;
+    GroupRec& g = h[i];
+    g.synth = make_args_mixed(p);
+    g.synth.nchunk = (p->T + (p->spin == 2 ? 127 : 255)) / (p->spin == 2 ? 128 : 256);
+    g.adj = make_args_mixed(p);
+    g.adj.nchunk = group_adj_chunks(p);
+    g.f_off = L.f_off[i];
+    g.part_off = L.part_off[i];
+    g.batch = L.batch[i];
+    g.synth_gx = Mpad * g.synth.nchunk;  // multiples of 8: every plan's range starts on XCD 0
+    g.adj_gx = Mpad * g.adj.nchunk;
+    const long long ns = (long long)start_synth[i] + (long long)g.synth_gx * groups;
+    const long long na = (long long)start_adj[i] + (long long)g.adj_gx * groups;
+    if (ns > 0x7fffffffLL || na > 0x7fffffffLL) {
+      set_error("ragged group: too many work-groups for one launch");
+      return JHTX_EINVAL;
+    }
+    start_synth[i + 1] = (int)ns;
+    start_adj[i + 1] = (int)na;
+  }
+  if (hipMalloc(desc, sizeof(GroupRec) * L.n) != hipSuccess) return JHTX_ENOMEM;
+  if (hipMemcpy(*desc, h.data(), sizeof(GroupRec) * L.n, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("ragged group: descriptor upload failed");
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+hipError_t group_rec_launch(bool synth, const GroupLayout& L, const void* desc, GroupCall C, hipStream_t s) {
+  const GroupRec* G = reinterpret_cast<const GroupRec*>(desc);
+  const dim3 grid((unsigned)C.start[C.n]);
+  if (synth) {
+    if (L.spin == 0)
+      hipLaunchKernelGGL(k_group_synth<0>, grid, dim3(256), 0, s, G, C);
+    else
+      hipLaunchKernelGGL(k_group_synth<2>, grid, dim3(256), 0, s, G, C);
+  } else {
+    if (L.spin == 0)
+      hipLaunchKernelGGL(k_group_adj<0>, grid, dim3(256), 0, s, G, C);
+    else
+      hipLaunchKernelGGL(k_group_adj<2>, grid, dim3(256), 0, s, G, C);
+  }
   return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@
 //   * only the active triangle l >= max(m, |s|) is iterated (no masked steps);
 //   * lanes own rings (R per lane); the even/odd (l - m) accumulators give the north
 //     value as their sum and the (-1)^m-reflected south value as their difference.
+#include <vector>
+
 #include "jhtx_internal.h"
 
 namespace jhtx {
@@ -701,11 +703,8 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const C* __restrict__
 // so short high-m columns waste no threads, and the chunk loads are issued 8 at a time.
 // Entries with l < |s| are zero; they sit in the columns m < |s|.
 template <typename P>
-__global__ __launch_bounds__(256) void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
-                                                    double2* __restrict__ alm) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  const int batch = gridDim.y;
+__device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int nchunk, const P* __restrict__ part,
+                                                double2* __restrict__ alm, long long idx, int b, int batch) {
   if (idx >= A.K) return;
   const int ncomp = spin ? 2 : 1;
   const double g = A.G[idx];
@@ -762,6 +761,31 @@ __global__ __launch_bounds__(256) void k_adj_finish(RecArgs A, int spin, int nch
   aE[idx] = make_double2(0.5 * (-p2.x - m2.x), 0.5 * (-p2.y - m2.y));
   const double2 d = p2 - m2;
   aB[idx] = make_double2(-0.5 * d.y, 0.5 * d.x);
+}
+
+template <typename P>
+__global__ __launch_bounds__(256) void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
+                                                    double2* __restrict__ alm) {
+  adj_finish_body<P>(A, spin, nchunk, part, alm, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
+                     gridDim.y);
+}
+
+// Ragged groups: the chunk reduction of every plan in one launch (per plan: ceil(K / 256)
+// work-groups per map).
+struct GroupFin {
+  RecArgs A;
+  long long part_off;
+  int batch, nchunk, spin, kblocks;
+};
+
+__global__ __launch_bounds__(256) void k_group_finish(const GroupFin* __restrict__ G, GroupCall C) {
+  int local;
+  const int i = group_member(C, local);
+  const RecArgs A = G[i].A;
+  const int kb = G[i].kblocks;
+  const float2* part = reinterpret_cast<const float2*>(C.ws + G[i].part_off);
+  adj_finish_body<float2>(A, G[i].spin, G[i].nchunk, part, reinterpret_cast<double2*>(C.out[i]),
+                          (long long)(local % kb) * 256 + threadIdx.x, local / kb, G[i].batch);
 }
 
 RecArgs make_args(const jhtx_plan* p, bool adjoint) {
@@ -873,6 +897,39 @@ hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* al
   const RecArgs a = make_args(p, true);
   hipLaunchKernelGGL(k_adj_finish<float2>, dim3((unsigned)((p->K + 255) / 256), batch), dim3(256), 0, s, a, p->spin,
                      nchunk, part, alm);
+  return hipGetLastError();
+}
+
+
+int group_fin_build(const GroupLayout& L, void** desc, int* start) {
+  std::vector<GroupFin> h(L.n);
+  start[0] = 0;
+  for (int i = 0; i < L.n; ++i) {
+    const jhtx_plan* p = L.plan[i];
+    h[i].A = make_args(p, true);
+    h[i].part_off = L.part_off[i];
+    h[i].batch = L.batch[i];
+    h[i].nchunk = group_adj_chunks(p);
+    h[i].spin = p->spin;
+    h[i].kblocks = (int)((p->K + 255) / 256);
+    const long long n = (long long)start[i] + (long long)h[i].kblocks * L.batch[i];
+    if (n > 0x7fffffffLL) {
+      set_error("ragged group: too many reduction work-groups for one launch");
+      return JHTX_EINVAL;
+    }
+    start[i + 1] = (int)n;
+  }
+  if (hipMalloc(desc, sizeof(GroupFin) * L.n) != hipSuccess) return JHTX_ENOMEM;
+  if (hipMemcpy(*desc, h.data(), sizeof(GroupFin) * L.n, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("ragged group: descriptor upload failed");
+    return JHTX_EDEVICE;
+  }
+  return JHTX_OK;
+}
+
+hipError_t group_fin_launch(const void* desc, GroupCall C, hipStream_t s) {
+  hipLaunchKernelGGL(k_group_finish, dim3((unsigned)C.start[C.n]), dim3(256), 0, s,
+                     reinterpret_cast<const GroupFin*>(desc), C);
   return hipGetLastError();
 }
 

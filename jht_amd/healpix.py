@@ -139,7 +139,8 @@ class Plan:
     """
 
     def __init__(self, nside: int, lmax: int, spin: int, max_batch: int = 1,
-                 precision: int = _lib.PREC_F64, flags: int = 0, device=None, tuning=None):
+                 precision: int = _lib.PREC_F64, flags: int = 0, device=None, tuning=None,
+                 workspace: bool = True):
         lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.nside, self.lmax, self.spin = nside, lmax, spin
@@ -159,8 +160,10 @@ class Plan:
                                                   _lib.ctypes.byref(t) if t is not None else None,
                                                   _lib.ctypes.byref(h)))
             self._h = h
-            nbytes = int(lib.jhtx_plan_workspace_bytes(h))
+            # workspace=False: the plan only lends its tables (e.g. to a RaggedGroup)
+            nbytes = int(lib.jhtx_plan_workspace_bytes(h)) if workspace else 0
             self.workspace = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=self.device)
+        self._has_workspace = workspace
         self._lib = lib
 
     def __del__(self):
@@ -192,13 +195,19 @@ class Plan:
         _lib.check(self._lib.jhtx_plan_read_profile(self._h, ms, cnt))
         return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
 
+    def _need_workspace(self):
+        if not self._has_workspace:
+            raise RuntimeError("this plan was created without a workspace (workspace=False)")
+
     def synth(self, alm: torch.Tensor, out: torch.Tensor, batch: int) -> torch.Tensor:
+        self._need_workspace()
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self._lib.jhtx_synth(self._h, alm.data_ptr(), out.data_ptr(), batch,
                                         self.workspace.data_ptr(), stream))
         return out
 
     def adjoint(self, maps: torch.Tensor, out: torch.Tensor, batch: int) -> torch.Tensor:
+        self._need_workspace()
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self._lib.jhtx_adjoint(self._h, maps.data_ptr(), out.data_ptr(), batch,
                                           self.workspace.data_ptr(), stream))

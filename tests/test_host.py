@@ -319,3 +319,26 @@ def test_tuning_struct_matches_header():
     assert tuple(fields) == L.TUNING_FIELDS
     assert [f[0] for f in L.Tuning._fields_] == fields
     assert ctypes.sizeof(L.Tuning) == 4 * len(fields)
+
+
+def test_ragged_group_validates_instances_on_the_host():
+    """RaggedGroup refuses what the group kernels cannot run before it touches a device:
+    mixed spins, FP64 instances, nside above 512, more than 16 instances, none."""
+    import pytest
+
+    from jht_amd import _lib as L
+    from jht_amd.graph import RaggedGroup
+
+    bad = [
+        [(16, 32, 0, 1, "mixed"), (16, 32, 2, 1, "mixed")],
+        [(16, 32, 0, 1, "f64")],
+        [(1024, 64, 0, 1, "mixed")],
+        [(16, 32, 0, 1, "mixed")] * (L.GROUP_MAX + 1),
+        [],
+    ]
+    for inst in bad:
+        with pytest.raises(ValueError):
+            RaggedGroup(inst)
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "jhtx.h")).read()
+    assert "n <= 16" in hdr and "nside <= 512" in hdr
+    assert L.GROUP_MAX == 16 and L.GROUP_NSIDE_MAX == 512
