@@ -64,7 +64,8 @@ int jhtx_plan_create(int nside, int lmax, int spin, int precision, int max_batch
  *                            or spin 2 with mx_synth_fold): spin 2 uses the value; spin 0
  *                            runs tile kernels for 16 and 32, else 8
  *   mx_synth_occ             its register budget in work-groups per CU {1,2,3} (spin 2)
- *   mx_adj_group, mx_adj_occ the same for the mixed adjoint {4,8,16}, {1,2}
+ *   mx_adj_group, mx_adj_occ the same for the mixed adjoint {4,8,16,32}, {1,2} (32: spin 2 with
+ *                            mx_adj_cb = mx_adj_f16 = 1, else 16)
  *   mx_adj_cb                16-map mixed spin-2 adjoint in column-block form (each wave
  *                            contracts 4 maps over the work-group's rings; the same bits)
  *                            {1 on (default), 2 off: per-wave tiles and a cross-wave sum}

@@ -450,7 +450,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
         (t.mx_synth_group == 0 || in(t.mx_synth_group, {4, 8, 16, 32})) &&
         (t.mx_synth_fold == 0 || in(t.mx_synth_fold, {1, 2})) &&
         (t.mx_synth_occ == 0 || in(t.mx_synth_occ, {1, 2, 3})) &&
-        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16})) && (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) &&
+        (t.mx_adj_group == 0 || in(t.mx_adj_group, {4, 8, 16, 32})) && (t.mx_adj_occ == 0 || in(t.mx_adj_occ, {1, 2})) &&
         (t.mx_adj_cb == 0 || in(t.mx_adj_cb, {1, 2})) && (t.mx_single == 0 || in(t.mx_single, {1, 2})) &&
         (t.mx_f16 == 0 || in(t.mx_f16, {1, 2})) && (t.mx_adj_f16 == 0 || in(t.mx_adj_f16, {1, 2}));
     if (!ok) {
@@ -477,6 +477,11 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   }
   if (p->spin == 2 && p->max_batch >= 32 && p->mx_synth_fold && p->mx_f16 && !(tuning && tuning->mx_synth_group))
     p->mx_synth_group = 32;
+  // spin 2: 32 maps per adjoint recursion (one sign per wave, double-buffered tiles; the
+  // bits of the 16-map kernel) once the plan holds 32: 111.8 -> 107.3 ms per 32 maps at
+  // nside 2048 / lmax 4096
+  if (p->spin == 2 && p->max_batch >= 32 && p->mx_adj_cb && p->mx_adj_f16 && !(tuning && tuning->mx_adj_group))
+    p->mx_adj_group = 32;
   p->block_cap = block_cap;
   auto shape = [&](int R, int& block, int& nchunk) {
     const int lanes = (p->T + R - 1) / R;

@@ -2140,6 +2140,214 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
   }
 }
 
+// k_adj_mx_s2_cbh for 32 maps: 8 waves over 256 rings, wave w running ONE recursion sign
+// (w & 1) for the 64 rings of ring block w >> 1 and contracting column block w (maps 4 w ..
+// 4 w + 3 of the group) over all eight (ring block, sign) tiles, with the operand layout,
+// split, local V scale and product pattern of k_adj_mx_s2_cbh.  The recursion is shared by
+// 32 maps instead of 16.  The tiles are double-buffered, so one barrier per 32 l separates
+// the recursion of tile j + 1 (into one buffer) from the contraction of tile j (from the
+// other), and the two waves of each SIMD (w and w + 4) run the two halves in opposite
+// order: one issues FP64 recursion while the other issues matrix instructions.
+__global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
+                                                   float2* __restrict__ part, int batch) {
+  constexpr int NW = 8, MG = 32, RINGS = 256;
+  __shared__ __align__(16) uint32_t tiles[2][4][2][TL][TS2];  // [buffer][ring block][sign][row][col]
+  __shared__ double2 s_ag[2][TL];                             // [buffer][step]: (alpha, gamma)
+  __shared__ int s_act[2][NW];                                // [buffer][ring block 2 + sign]
+  int m, chunk;
+  if (!decode_block<true>(A, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int sgw = wave & 1, rbw = wave >> 1;  // this wave's recursion: sign, ring block
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  if (lmin > lmax) return;
+  const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
+  const long long ringM = A.M;
+  const int t = chunk * RINGS + 64 * rbw + lane;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m + 4 * sgw;
+    double sv;
+    seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
+    mup = -sv;
+  }
+  // B operands of column block cw = wave (as k_adj_mx_s2_cbh): chunk ring 4 q + lane/16,
+  // column lane%16 -> map 4 cw + (lane%16)/4, component lane%4
+  const int g = (lane & 15) >> 2, cc = lane & 3;
+  const int bcol = mg * MG + 4 * wave + g;
+  uint32_t Bv[2][64];  // [sign][q]: (hi, lo) of the scaled value
+  double inv;          // 1 / (sA sB) of this lane's map
+  {
+    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const bool lo = cc < 2;
+    const int ri = cc & 1;
+    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
+    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
+    float mx = 0.f;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
+      double vp = 0.0, vm = 0.0;
+      if (tr < A.T && bcol < batch) {
+        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
+        const bool south = tr < A.T - 1;  // the equator ring has no mirror
+        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
+        if (lo) {
+          vp = (double)Vp[rn];
+          vm = south ? par0 * (double)Vp[rs] : 0.0;
+        } else {
+          vp = south ? par0 * (double)Vm[rs] : 0.0;
+          vm = (double)Vm[rn];
+        }
+      }
+      const float bp = (float)vp, bm = (float)vm;
+      mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
+      Bv[0][q] = __float_as_uint(bp);
+      Bv[1][q] = __float_as_uint(bm);
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 2));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    int e = 1;
+    if (mx > 0.f && mx < 3.0e38f) (void)frexpf(4096.f / mx, &e);
+    e = min(max(e - 1, -120), 120);
+    const float sB = ldexpf(1.f, e);
+    inv = 1.0 / (sA * (double)sB);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      Bv[0][q] = split_f16(__uint_as_float(Bv[0][q]) * sB);
+      Bv[1][q] = split_f16(__uint_as_float(Bv[1][q]) * sB);
+    }
+  }
+  float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
+  const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
+  const int tcol = s2_tile_col(lane);
+  const int ntile = (lmax - lmin + TL) / TL;
+  // 32 recursion steps of this wave's sign into its tile of buffer b (tile jt = steps
+  // lmin + TL jt ...); odd rows of the sign - tile negated, as in k_adj_mx_s2_cbh
+  auto rec_tile = [&](int b) {
+    uint32_t* mytile = &tiles[b][rbw][sgw][0][0];
+    int act = 0;
+#pragma unroll 1
+    for (int j0 = 0; j0 < TL; j0 += 8) {
+      const int anydeep = __ballot(valid && ex < 0) != 0ull;
+      act |= __ballot(valid && ex == 0) != 0ull;
+      const double f = (ex == 0 && valid) ? sA : 0.0;
+      double2 ag[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[b][j0 + jj];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const double c = fma(ag[jj].x, x, sgw ? -ag[jj].y : ag[jj].y);
+        const double n = fma(c, mu, -mup);
+        mup = mu;
+        mu = n;
+        mytile[s2_tile_row(j0 + jj) * TS2 + tcol] = split_f16((float)(n * ((sgw && (jj & 1)) ? -f : f)));
+      }
+      if (anydeep) rescale_pair(mu, mup, ex);
+    }
+    if (lane == 0) s_act[b][wave] = act;
+  };
+  const uint32_t* abase0 = &tiles[0][0][0][0][0] + (lane & 15) * TS2 + 16 * (lane >> 4);
+  // contraction of tile jt (buffer b) for this wave's column block, and its partials
+  auto contract_tile = [&](int jt, int b) {
+    int actv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) actv[w] = s_act[b][w];
+    // pz is always 0 but unknown to the compiler (l is not bounded below 2^30 for it): the
+    // B words stay built in the loop instead of being hoisted as 256 more registers
+    const uint32_t pz = (uint32_t)(lmin + TL * jt > (1 << 30));
+    const uint32_t sel_hh = 0x01000100u | pz, sel_ll = 0x03020302u | pz;
+    const uint32_t* abase = abase0 + b * (4 * 2 * TL * TS2);
+    floatx4 D[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) D[p][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        if (!actv[2 * rb + sg]) continue;
+        uint4 a[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const uint4* src = reinterpret_cast<const uint4*>(abase + ((rb * 2 + sg) * TL + 16 * p) * TS2);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) a[p][n] = src[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          uint32_t hh[4], ll[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint32_t w = Bv[sg][16 * rb + 4 * n + i];
+            hh[i] = __builtin_amdgcn_perm(w, w, sel_hh);
+            ll[i] = __builtin_amdgcn_perm(w, w, sel_ll);
+          }
+          const halfx8 bh = __builtin_bit_cast(halfx8, make_uint4(hh[0], hh[1], hh[2], hh[3]));
+          const halfx8 bl = __builtin_bit_cast(halfx8, make_uint4(ll[0], ll[1], ll[2], ll[3]));
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const halfx8 av = __builtin_bit_cast(halfx8, a[p][n]);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bh, D[p][rb], 0, 0, 0);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bl, D[p][rb], 0, 0, 0);
+          }
+        }
+      }
+    if (bcol < batch) {
+      const int l0 = lmin + TL * jt;
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int l = l0 + 2 * (4 * (lane >> 4) + r) + p;
+          double v = 0.0;
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb) v += (double)D[p][rb][r];
+          v *= inv;
+          if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * (float)v : (float)v;
+        }
+    }
+  };
+  // (alpha, gamma) of step j of tile jt (zero past lmax, and at lmin: the seed step)
+  auto ag_at = [&](int jt, int j) {
+    const int l = lmin + TL * jt + j;
+    return (l <= lmax && l != lmin) ? make_double2(A.alpha[cb + l], A.gamma[cb + l]) : make_double2(0.0, 0.0);
+  };
+  if (tid < TL) {
+    s_ag[0][tid] = ag_at(0, tid);
+    s_ag[1][tid] = ag_at(1, tid);
+  }
+  double2 pf = (tid < TL) ? ag_at(2, tid) : make_double2(0.0, 0.0);
+  __syncthreads();
+  rec_tile(0);
+  __syncthreads();
+  const bool rec_first = wave < 4;  // waves w and w + 4 share a SIMD
+  for (int jt = 0; jt < ntile; ++jt) {
+    const int b = jt & 1;
+    // s_ag[b] held tile jt, read by the recursion before the last barrier: now tile jt + 2
+    if (tid < TL) {
+      s_ag[b][tid] = pf;
+      pf = ag_at(jt + 3, tid);
+    }
+    const bool more = jt + 1 < ntile;
+    if (rec_first) {
+      if (more) rec_tile(b ^ 1);
+      contract_tile(jt, b);
+    } else {
+      contract_tile(jt, b);
+      if (more) rec_tile(b ^ 1);
+    }
+    __syncthreads();
+  }
+}
+
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
 __global__ __launch_bounds__(256) void k_adj_mx_s0(RecArgs A, const float2* __restrict__ V,
                                                    float2* __restrict__ part, int batch) {
@@ -2640,9 +2848,14 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   RecArgs a = make_args_mixed(p);
   a.nchunk = mixed_adj_chunks(p);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
-  const int MG = mixed_group(p, true);
+  int MG = mixed_group(p, true);
+  // 32 maps per adjoint recursion: the FP16-split column-block kernel only
+  if (MG == 32 && !(p->spin == 2 && p->mx_adj_cb && p->mx_adj_f16)) MG = 16;
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
+  if (p->spin == 2 && MG == 32 && p->mx_adj_cb && p->mx_adj_f16)
+    hipLaunchKernelGGL(k_adj_s2p, dim3(Mpad * a.nchunk, (batch + 31) / 32), dim3(512), 0, s, a, V, p->mx_sA, part,
+                       batch);
+  else if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
     hipLaunchKernelGGL(k_adj_mx_s0_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
   else if (p->spin == 0 && MG == 16)
     hipLaunchKernelGGL(k_adj_mx_s0_cb, grid, dim3(256), 0, s, a, V, part, batch);

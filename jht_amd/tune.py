@@ -26,9 +26,9 @@ _CANDIDATES = {
     ("f64", 0): ([{"synth_rings": r} for r in (1, 4, 8)], [{"adj_rings": r} for r in (4, 8, 1)]),
     ("f64", 2): ([{"synth_rings": r} for r in (1, 2, 4)], [{"adj_rings": r} for r in (4, 2, 1)]),
     ("mixed", 0): ([{"mx_synth_group": g} for g in (32, 16, 8)], [{"mx_adj_group": g} for g in (16, 8)]),
-    ("mixed", 2): ([{"mx_synth_group": 16, "mx_f16": f} for f in (1, 2)] +
+    ("mixed", 2): ([{"mx_synth_group": 32}] + [{"mx_synth_group": 16, "mx_f16": f} for f in (1, 2)] +
                    [{"mx_synth_group": g, "mx_synth_occ": o} for g, o in ((8, 2), (4, 1), (8, 1))],
-                   [{"mx_adj_group": 16, "mx_adj_occ": 2, "mx_adj_cb": 1, "mx_adj_f16": 1}] +
+                   [{"mx_adj_group": g, "mx_adj_occ": 2, "mx_adj_cb": 1, "mx_adj_f16": 1} for g in (32, 16)] +
                    [{"mx_adj_group": g, "mx_adj_occ": o, "mx_adj_cb": c, "mx_adj_f16": 2}
                     for g, o, c in ((16, 2, 1), (16, 2, 2), (8, 2, 2), (4, 1, 2), (8, 1, 2))]),
 }

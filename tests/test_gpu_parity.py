@@ -429,6 +429,35 @@ def test_mixed_adjoint_ring_chunks(batch, cb):
     assert ea <= MIXED_TOL
 
 
+@pytest.mark.parametrize("nside,lmax,batch", [(32, 40, 7), (64, 128, 32), (512, 640, 40), (128, 300, 33)])
+def test_mixed_adjoint_32_maps_bitwise(nside, lmax, batch):
+    """The 32-map spin-2 adjoint (one recursion sign per wave, double-buffered tiles) gives
+    the bits of the 16-map column-block kernel (the same recursion values, scales, operand
+    words and accumulation order), on one and four ring chunks and with ragged groups, and
+    matches the FP64 adjoint."""
+    j = _jht()
+    from jht_amd import _lib
+    from jht_amd.healpix import Plan
+
+    rng = np.random.default_rng(700 + batch)
+    v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
+    out = {}
+    for g in (32, 16):
+        p = Plan(nside, lmax, 2, batch, _lib.PREC_MIXED, 0,
+                 tuning=dict(mx_adj_group=g, mx_adj_cb=1, mx_adj_f16=1, mx_single=2))
+        assert p.tuning["mx_adj_group"] == g
+        b = torch.empty((batch, 2, O.alm_size(lmax)), dtype=torch.complex128, device="cuda")
+        p.adjoint(v, b, batch)
+        torch.cuda.synchronize()
+        out[g] = b
+    assert torch.equal(out[32], out[16])
+    b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
+    bmx = out[32].cpu().numpy()
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"32-map adjoint nside={nside} lmax={lmax} batch={batch}: bitwise = 16-map, vs FP64 {ea:.2e}", flush=True)
+    assert ea <= MIXED_TOL
+
+
 @pytest.mark.parametrize("group", [32, 16, 8])
 @pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
 def test_mixed_spin0_groups(nside, lmax, group):
