@@ -1546,6 +1546,35 @@ constexpr int TS2 = 68;
 __device__ __forceinline__ int s2_tile_row(int j) { return 16 * (j & 1) + (j >> 1); }
 __device__ __forceinline__ int s2_tile_col(int r) { return 16 * (r & 3) + (r >> 2); }
 
+// The ring values a column-block adjoint lane holds for the whole l loop: component ri of
+// field `field` of map bcol at chunk ring 4 (q0 + q) + lane / 16 (n[q]) and at its mirror
+// (s[q]), q < NQ, as raw bits.  All 2 NQ loads are issued before any is used: the addresses
+// are clamped to valid rings and maps and the caller applies each value's validity (a load
+// behind a branch waits for the one before it: 128 memory latencies per work-group).
+template <typename W, int NQ>
+__device__ __forceinline__ void load_ring_values(const RecArgs& A, const float2* __restrict__ V, int bcol,
+                                                 int batch, int ncomp, int field, int ri, int m, int chunk,
+                                                 int lane, int q0, W (&n)[NQ], W (&s)[NQ]) {
+  const long long ringM = A.M;
+  const float* Vf = reinterpret_cast<const float*>(
+      V + ((long long)min(bcol, batch - 1) * ncomp + field) * A.nrings * ringM);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int tr = min(chunk * 256 + 4 * (q0 + q) + (lane >> 4), A.T - 1);
+    n[q] = __builtin_bit_cast(W, Vf[2 * ((long long)tr * ringM + m) + ri]);
+    s[q] = __builtin_bit_cast(W, Vf[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri]);
+  }
+}
+
+// validity of the ring pair of q: the ring exists (ok), and it has a mirror (south: not the
+// equator)
+__device__ __forceinline__ void ring_valid(const RecArgs& A, int bcol, int batch, int chunk, int q, int lane,
+                                           bool& ok, bool& south) {
+  const int tr = chunk * 256 + 4 * q + (lane >> 4);
+  ok = bcol < batch && tr < A.T;
+  south = ok && tr < A.T - 1;
+}
+
 // act: wave-uniform bit sg set when the sign-sg tile holds a nonzero value (an all-zero
 // sign tile contributes nothing and is skipped).  The operands of the next (parity, sign)
 // step are read while the MFMAs of the current one run.
@@ -1789,29 +1818,21 @@ __device__ __forceinline__ void adj_mx_s2_cb_body(const RecArgs& A, const float2
   const int bcol = mg * MG + 4 * cw + g;  // the map of this lane's column
   float Bv[2][64];                          // [sign][q]
   {
-    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const float par0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
     const bool lo = cc < 2;
-    const int ri = cc & 1;
-    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
-    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
+    // sign + column (p): V+ north and par0 V+ south; sign - column (n): the reverse
+    load_ring_values(A, V, bcol, batch, 2, lo ? 0 : 1, cc & 1, m, chunk, lane, 0, Bv[0], Bv[1]);
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
-      double bp = 0.0, bm = 0.0;
-      if (tr < A.T && bcol < batch) {
-        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
-        const bool south = tr < A.T - 1;  // the equator ring has no mirror
-        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
-        if (lo) {
-          bp = (double)Vp[rn];
-          bm = south ? par0 * (double)Vp[rs] : 0.0;
-        } else {
-          bp = south ? par0 * (double)Vm[rs] : 0.0;
-          bm = (double)Vm[rn];
-        }
-      }
-      Bv[0][q] = (float)bp;
-      Bv[1][q] = (float)bm;
+      bool ok, south;
+      ring_valid(A, bcol, batch, chunk, q, lane, ok, south);
+      const float vn = ok ? Bv[0][q] : 0.f;
+      const float vs = south ? par0 * Bv[1][q] : 0.f;
+      Bv[0][q] = lo ? vn : vs;
+      Bv[1][q] = lo ? vs : vn;
+      // materialised here: otherwise the compiler recomputes them from the raw values in
+      // the l loop and holds more registers there
+      asm volatile("" : "+v"(Bv[0][q]), "+v"(Bv[1][q]));
     }
   }
   float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
@@ -1971,37 +1992,19 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
   uint32_t Bv[2][64];  // [sign][q]: (hi, lo) of the scaled value
   double inv;          // 1 / (sA sB) of this lane's map
   {
-    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const float par0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
     const bool lo = cc < 2;
-    const int ri = cc & 1;
-    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
-    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
-    // the lane's two B values of ring 4 q + lane/16 (signs +, -)
-    auto load_v = [&](int q, float& bp, float& bm) {
-      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
-      double vp = 0.0, vm = 0.0;
-      if (tr < A.T && bcol < batch) {
-        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
-        const bool south = tr < A.T - 1;  // the equator ring has no mirror
-        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
-        if (lo) {
-          vp = (double)Vp[rn];
-          vm = south ? par0 * (double)Vp[rs] : 0.0;
-        } else {
-          vp = south ? par0 * (double)Vm[rs] : 0.0;
-          vm = (double)Vm[rn];
-        }
-      }
-      bp = (float)vp;
-      bm = (float)vm;
-    };
-    // one pass: the values go into Bv as float bits, then are split in place once the
-    // maximum is known
+    // the values go into Bv as float bits (sign + column: V+ north and par0 V+ south; sign -:
+    // the reverse), then are split in place once the maximum is known
+    load_ring_values(A, V, bcol, batch, 2, lo ? 0 : 1, cc & 1, m, chunk, lane, 0, Bv[0], Bv[1]);
     float mx = 0.f;
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      float bp, bm;
-      load_v(q, bp, bm);
+      bool ok, south;
+      ring_valid(A, bcol, batch, chunk, q, lane, ok, south);
+      const float vn = ok ? __uint_as_float(Bv[0][q]) : 0.f;
+      const float vs = south ? par0 * __uint_as_float(Bv[1][q]) : 0.f;
+      const float bp = lo ? vn : vs, bm = lo ? vs : vn;
       mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
       Bv[0][q] = __float_as_uint(bp);
       Bv[1][q] = __float_as_uint(bm);
@@ -2182,29 +2185,19 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   uint32_t Bv[2][64];  // [sign][q]: (hi, lo) of the scaled value
   double inv;          // 1 / (sA sB) of this lane's map
   {
-    const double par0 = ((lmin - m) & 1) ? -1.0 : 1.0;
+    const float par0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
     const bool lo = cc < 2;
-    const int ri = cc & 1;
-    const float* Vp = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 0) * A.nrings * ringM);
-    const float* Vm = reinterpret_cast<const float*>(V + ((long long)bcol * 2 + 1) * A.nrings * ringM);
+    // the values go into Bv as float bits (sign + column: V+ north and par0 V+ south; sign -:
+    // the reverse), then are split in place once the maximum is known
+    load_ring_values(A, V, bcol, batch, 2, lo ? 0 : 1, cc & 1, m, chunk, lane, 0, Bv[0], Bv[1]);
     float mx = 0.f;
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
-      double vp = 0.0, vm = 0.0;
-      if (tr < A.T && bcol < batch) {
-        const long long rn = 2 * ((long long)tr * ringM + m) + ri;
-        const bool south = tr < A.T - 1;  // the equator ring has no mirror
-        const long long rs = 2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri;
-        if (lo) {
-          vp = (double)Vp[rn];
-          vm = south ? par0 * (double)Vp[rs] : 0.0;
-        } else {
-          vp = south ? par0 * (double)Vm[rs] : 0.0;
-          vm = (double)Vm[rn];
-        }
-      }
-      const float bp = (float)vp, bm = (float)vm;
+      bool ok, south;
+      ring_valid(A, bcol, batch, chunk, q, lane, ok, south);
+      const float vn = ok ? __uint_as_float(Bv[0][q]) : 0.f;
+      const float vs = south ? par0 * __uint_as_float(Bv[1][q]) : 0.f;
+      const float bp = lo ? vn : vs, bm = lo ? vs : vn;
       mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
       Bv[0][q] = __float_as_uint(bp);
       Bv[1][q] = __float_as_uint(bm);
@@ -2481,21 +2474,21 @@ __device__ __forceinline__ void adj_mx_s0_cb_body(const RecArgs& A, const float2
   const int j = lane & 15;
   const int bcol = mg * MG + 8 * cw + (j >> 1);
   float Bv[64];
-  {
-    const float* Vb = reinterpret_cast<const float*>(V + (long long)bcol * A.nrings * ringM);
-    const int ri = j & 1;
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
-      double u = 0.0;
-      if (tr < A.T && bcol < batch) {
-        const double vn = (double)Vb[2 * ((long long)tr * ringM + m) + ri];
-        // the equator ring has no mirror
-        const double vs = (tr < A.T - 1) ? (double)Vb[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri] : 0.0;
-        u = pw ? vn - vs : vn + vs;
-      }
-      Bv[q] = (float)u;
+  for (int h = 0; h < 4; ++h) {  // four quarters of 32 loads each: fewer registers in flight
+    float vn[16], vs[16];
+    load_ring_values(A, V, bcol, batch, 1, 0, j & 1, m, chunk, lane, 16 * h, vn, vs);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      bool ok, south;  // south: not the equator, which has no mirror
+      ring_valid(A, bcol, batch, chunk, 16 * h + q, lane, ok, south);
+      const double n = ok ? (double)vn[q] : 0.0, sv = south ? (double)vs[q] : 0.0;
+      Bv[16 * h + q] = (float)(pw ? n - sv : n + sv);
+      // materialised here: otherwise the compiler recomputes it from both raw values in the
+      // l loop and holds twice the registers there (occupancy 3 -> 2)
+      asm volatile("" : "+v"(Bv[16 * h + q]));
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep the next quarter's loads behind this one's use
   }
   float* pcol = reinterpret_cast<float*>(part + ((long long)chunk * batch + bcol) * A.K) + (j & 1);
   float* mytile = &tiles[wave][0][0];
@@ -2605,22 +2598,21 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cbh(RecArgs A, const float
   uint32_t Bv[64];  // (hi, lo) of the scaled value
   double inv;
   {
-    const float* Vb = reinterpret_cast<const float*>(V + (long long)bcol * A.nrings * ringM);
-    const int ri = j & 1;
     float mx = 0.f;
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-      const int tr = chunk * RINGS + 4 * q + (lane >> 4);
-      double u = 0.0;
-      if (tr < A.T && bcol < batch) {
-        const double vn = (double)Vb[2 * ((long long)tr * ringM + m) + ri];
-        // the equator ring has no mirror
-        const double vs = (tr < A.T - 1) ? (double)Vb[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + ri] : 0.0;
-        u = pw ? vn - vs : vn + vs;
+    for (int h = 0; h < 2; ++h) {  // two halves of 64 loads each: fewer registers in flight
+      float vn[32], vs[32];
+      load_ring_values(A, V, bcol, batch, 1, 0, j & 1, m, chunk, lane, 32 * h, vn, vs);
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        bool ok, south;  // south: not the equator, which has no mirror
+        ring_valid(A, bcol, batch, chunk, 32 * h + q, lane, ok, south);
+        const double n = ok ? (double)vn[q] : 0.0, sv = south ? (double)vs[q] : 0.0;
+        const float uf = (float)(pw ? n - sv : n + sv);
+        mx = fmaxf(mx, fabsf(uf));
+        Bv[32 * h + q] = __float_as_uint(uf);
       }
-      const float uf = (float)u;
-      mx = fmaxf(mx, fabsf(uf));
-      Bv[q] = __float_as_uint(uf);
+      __builtin_amdgcn_sched_barrier(0);  // keep the second half's loads behind the first's use
     }
     // the map's maximum over its 8 lanes (re / im x 4 ring residues): bits 0, 4 and 5
     mx = fmaxf(mx, __shfl_xor(mx, 1));
