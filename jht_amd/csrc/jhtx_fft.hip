@@ -391,28 +391,44 @@ __device__ __forceinline__ double2 ld64(const cx<T>* p, int m) {
   return make_double2((double)v.x, (double)v.y);
 }
 
+// Loads are issued in groups (clamped, always-valid addresses; each value's validity applied
+// after) so that a thread waits for one memory latency per group rather than per term: the
+// large-ring launch classes run one work-group per CU and cannot hide a latency per term.
 template <typename T>
-__device__ void ring_fold(const cx<T>* __restrict__ Fp, const cx<T>* __restrict__ Fm, int M,
+__device__ __forceinline__ void ring_fold(const cx<T>* __restrict__ Fp, const cx<T>* __restrict__ Fm, int M,
                           int N, bool sh, bool real0, bool combine, cx<T>* dst,
                           double2* scratch, int tid, int nthr) {
+  constexpr int G = sizeof(T) == 4 ? 8 : 4;  // terms in flight per thread and field
   if (N <= nthr) {
     const int S = N * (nthr / N);
     if (tid < S) {
       double2 ap = make_double2(0.0, 0.0), am = ap;
       const int km = (N - tid % N) % N;
-      for (int m = tid; m < M; m += S) {
-        const int q = m / N;
-        const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
-        double2 v = ld64<T>(Fp, m);
-        if (m == 0 && real0) v.y = 0.0;
-        ap.x += sg * v.x;
-        ap.y += sg * v.y;
-        if (m >= 1) {
-          const int qq = (m + km) / N;
-          const double sm = (sh && (qq & 1)) ? -1.0 : 1.0;
-          const double2 w = ld64<T>(Fm, m);
-          am.x += sm * w.x;
-          am.y -= sm * w.y;
+      for (int m0 = tid; m0 < M; m0 += G * S) {
+        cx<T> fp[G], fm[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int mc = min(m0 + u * S, M - 1);
+          fp[u] = Fp[mc];
+          fm[u] = Fm[mc];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int m = m0 + u * S;
+          if (m < M) {
+            const int q = m / N;
+            const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
+            double2 v = make_double2((double)fp[u].x, (double)fp[u].y);
+            if (m == 0 && real0) v.y = 0.0;
+            ap.x += sg * v.x;
+            ap.y += sg * v.y;
+            if (m >= 1) {
+              const int qq = (m + km) / N;
+              const double sm = (sh && (qq & 1)) ? -1.0 : 1.0;
+              am.x += sm * (double)fm[u].x;
+              am.y -= sm * (double)fm[u].y;
+            }
+          }
         }
       }
       scratch[tid] = ap;
@@ -429,6 +445,54 @@ __device__ void ring_fold(const cx<T>* __restrict__ Fp, const cx<T>* __restrict_
         dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
       } else {
         dst[k] = to_cx<T>(acc);
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  // N > nthr: bin k = tid + nthr e sums Fp at m = k + q N (sign (-1)^q when shifted) and
+  // conj(Fm) at m = N - k + (q - 1) N (q >= 1, the same sign).  With M <= 2 N every bin has
+  // at most two terms of each field, and the terms of G bins are loaded at once.
+  if (M <= 2 * N) {
+    const double s1 = sh ? -1.0 : 1.0;  // the q = 1 sign
+    for (int k0 = tid; k0 < N; k0 += G * nthr) {
+      cx<T> p0[G], p1[G], q0[G], q1[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int kc = min(k0 + u * nthr, N - 1);
+        p0[u] = Fp[min(kc, M - 1)];
+        p1[u] = Fp[min(kc + N, M - 1)];
+        q0[u] = Fm[min(N - kc, M - 1)];
+        q1[u] = Fm[min(2 * N - kc, M - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int k = k0 + u * nthr;
+        if (k >= N) break;
+        double2 acc = make_double2(0.0, 0.0);
+        if (k < M) {  // m = k (q = 0)
+          acc.x += (double)p0[u].x;
+          acc.y += (k == 0 && real0) ? 0.0 : (double)p0[u].y;
+        }
+        if (k + N < M) {  // m = k + N (q = 1)
+          acc.x += s1 * (double)p1[u].x;
+          acc.y += s1 * (double)p1[u].y;
+        }
+        if (N - k < M) {  // m = N - k (q = 1)
+          acc.x += s1 * (double)q0[u].x;
+          acc.y -= s1 * (double)q0[u].y;
+        }
+        if (2 * N - k < M) {  // m = 2 N - k (q = 2)
+          acc.x += (double)q1[u].x;
+          acc.y -= (double)q1[u].y;
+        }
+        if (sh) acc = cmul(acc, cispi((double)k / (double)N));
+        if (combine) {
+          const cx<T> d = dst[k];
+          dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
+        } else {
+          dst[k] = to_cx<T>(acc);
+        }
       }
     }
     __syncthreads();
@@ -555,10 +619,19 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
   const long long ss = ring_start(A.nside, rs);
   if constexpr (SPIN == 0) {
     const double* mb = map + (long long)b * A.npix;
-    // the map is read once: streamed past the caches
-    for (int j = tid; j < N; j += BLOCK)
-      l2[j] = mk2((T)__builtin_nontemporal_load(&mb[sn + j]),
-                  (T)(pair ? __builtin_nontemporal_load(&mb[ss + j]) : 0.0));
+    // the map is read once: streamed past the caches, 8 loads per ring in flight per thread
+    for (int j0 = tid; j0 < N; j0 += 8 * BLOCK) {
+      double vn[8], vs[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int jc = min(j0 + u * BLOCK, N - 1);
+        vn[u] = __builtin_nontemporal_load(&mb[sn + jc]);
+        vs[u] = pair ? __builtin_nontemporal_load(&mb[ss + jc]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vn[u], (T)vs[u]);
+    }
     __syncthreads();
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
@@ -583,8 +656,18 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
     const bool h = half;
     const int r = h ? rs : rn;
     const long long st = h ? ss : sn;
-    for (int j = tid; j < N; j += BLOCK)
-      l2[j] = mk2((T)__builtin_nontemporal_load(&Q[st + j]), (T)__builtin_nontemporal_load(&U[st + j]));
+    for (int j0 = tid; j0 < N; j0 += 8 * BLOCK) {  // 8 loads per field in flight per thread
+      double vq[8], vu[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int jc = min(j0 + u * BLOCK, N - 1);
+        vq[u] = __builtin_nontemporal_load(&Q[st + jc]);
+        vu[u] = __builtin_nontemporal_load(&U[st + jc]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vq[u], (T)vu[u]);
+    }
     __syncthreads();
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
