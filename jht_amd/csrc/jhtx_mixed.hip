@@ -81,6 +81,23 @@ __device__ __forceinline__ bool decode_block(const RecArgs& A, int& m, int& chun
   return decode_block<GROUPED>(A, (int)blockIdx.x, m, chunk);
 }
 
+// Tiled order: each XCD runs tiles of MB consecutive orders m x CB consecutive ring chunks
+// (m fastest), the MB-blocks of m alternating between XCDs.  The work-groups resident on
+// an XCD at one time then share the a_lm columns of their m across CB chunks (one HBM read
+// per CB chunks instead of per chunk) and write F[ring][m] for MB adjacent m (runs of
+// 8 MB bytes per ring).  The grid holds Mpad x ceil(nchunk / CB) x CB slots, Mpad a
+// multiple of 8 MB; decode_tile<kMGroup, 1> is decode_block<true>.
+template <int MB, int CB>
+__device__ __forceinline__ bool decode_tile(const RecArgs& A, int P, int& m, int& chunk) {
+  const int xcd = P & 7;
+  const int slot = P >> 3;
+  const int in = slot % (MB * CB), blk = slot / (MB * CB);
+  const int ncb = (A.nchunk + CB - 1) / CB;
+  chunk = (blk % ncb) * CB + in / MB;
+  m = ((blk / ncb) * 8 + xcd) * MB + in % MB;
+  return m < A.M && chunk < A.nchunk;
+}
+
 // ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
 __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
 
@@ -935,7 +952,7 @@ constexpr int kP_GS = 8;   // l steps per pipeline group: one matrix K block
 constexpr int kP_CH = 32;  // l steps per staged chunk of B
 constexpr int kP_TS = 72;  // tile row stride (dwords): 2 kP_TS = 16 mod 64, conflict-free A reads
 
-template <int NCB, int NW, int MINB>
+template <int NCB, int NW, int MINB, int MB, int CB>
 __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const double2* __restrict__ alm,
                                                             const float* __restrict__ sB, double sA,
                                                             float2* __restrict__ F, int batch) {
@@ -957,7 +974,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
   __shared__ float s_sc[MG];
   uint32_t* const s_b = s_buf;
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
+  if (!decode_tile<MB, CB>(A, (int)blockIdx.x, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int sg = wave & 1;
@@ -2143,20 +2160,33 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
   }
 }
 
-// k_adj_mx_s2_cbh for 32 maps: 8 waves over 256 rings, wave w running ONE recursion sign
-// (w & 1) for the 64 rings of ring block w >> 1 and contracting column block w (maps 4 w ..
-// 4 w + 3 of the group) over all eight (ring block, sign) tiles, with the operand layout,
-// split, local V scale and product pattern of k_adj_mx_s2_cbh.  The recursion is shared by
-// 32 maps instead of 16.  The tiles are double-buffered, so one barrier per 32 l separates
-// the recursion of tile j + 1 (into one buffer) from the contraction of tile j (from the
-// other), and the two waves of each SIMD (w and w + 4) run the two halves in opposite
-// order: one issues FP64 recursion while the other issues matrix instructions.
+// The spin-2 adjoint for 32 maps per recursion: 8 waves over 256 rings, wave w running ONE
+// recursion sign (w & 1) for the 64 rings of ring block w >> 1 and contracting column block
+// w (maps 4 w .. 4 w + 3 of the group, components p.re, p.im, n.re, n.im) over all eight
+// (ring block, sign) tiles, with the N/S parity fold, the local V scale and the FP16 hi + lo
+// split of k_adj_mx_s2_cbh.  The contraction's K index is the ring: the tiles hold the hi
+// and lo halves of the recursion values in separate planes ([row][ring]), and each lane's
+// B operands are the hi and lo halves of its V values packed ring-pairwise, built once per
+// work-group, so the four products come from four instructions with no per-step operand
+// rebuilding: hi.hi, hi.lo, lo.hi and lo.lo over 32 rings each.  The tiles are
+// double-buffered (one barrier per 32 l between the recursion of tile j + 1 and the
+// contraction of tile j), and the two waves of each SIMD (w and w + 4) run the two halves in
+// opposite order: one issues FP64 recursion while the other issues matrix instructions.
+constexpr int kQ_RS = 72;  // plane row stride (halves): 144 bytes, rows 4 banks apart
+
+// ring (within the 256-ring chunk) of entry t of a lane's B operands: t = 16 rb + 8 sl + e
+// is ring 64 rb + 32 sl + 8 k4 + e, k4 = lane / 16 (the lane's K block of 8 rings)
+__device__ __forceinline__ int q_ring(int t, int lane) {
+  return 64 * (t >> 4) + 32 * ((t >> 3) & 1) + 8 * (lane >> 4) + (t & 7);
+}
+
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
                                                    float2* __restrict__ part, int batch) {
   constexpr int NW = 8, MG = 32, RINGS = 256;
-  __shared__ __align__(16) uint32_t tiles[2][4][2][TL][TS2];  // [buffer][ring block][sign][row][col]
-  __shared__ double2 s_ag[2][TL];                             // [buffer][step]: (alpha, gamma)
-  __shared__ int s_act[2][NW];                                // [buffer][ring block 2 + sign]
+  // [buffer][ring block][sign][hi / lo][row][ring]
+  __shared__ __align__(16) uint16_t tp[2][4][2][2][TL][kQ_RS];
+  __shared__ double2 s_ag[2][2][TL];  // [buffer][sign][step]: (alpha, +-gamma)
+  __shared__ int s_act[2][NW];        // [buffer][ring block 2 + sign]
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -2178,30 +2208,39 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
     mup = -sv;
   }
-  // B operands of column block cw = wave (as k_adj_mx_s2_cbh): chunk ring 4 q + lane/16,
-  // column lane%16 -> map 4 cw + (lane%16)/4, component lane%4
+  // B operands of column block cw = wave: column lane % 16 -> map 4 cw + (lane % 16) / 4,
+  // component lane % 4; Bq[sign][2 rb + sl][hi / lo] holds the halves of the lane's 8 rings
+  // q_ring(16 rb + 8 sl + e), e = 0..7, in ring pairs
   const int g = (lane & 15) >> 2, cc = lane & 3;
   const int bcol = mg * MG + 4 * wave + g;
-  uint32_t Bv[2][64];  // [sign][q]: (hi, lo) of the scaled value
-  double inv;          // 1 / (sA sB) of this lane's map
+  uint4 Bq[2][8][2];
+  double inv;  // 1 / (sA sB) of this lane's map
   {
     const float par0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
     const bool lo = cc < 2;
-    // the values go into Bv as float bits (sign + column: V+ north and par0 V+ south; sign -:
-    // the reverse), then are split in place once the maximum is known
-    load_ring_values(A, V, bcol, batch, 2, lo ? 0 : 1, cc & 1, m, chunk, lane, 0, Bv[0], Bv[1]);
+    const float* Vs =
+        reinterpret_cast<const float*>(V + ((long long)min(bcol, batch - 1) * 2 + (lo ? 0 : 1)) * A.nrings * ringM);
+    float bf[2][64];  // [sign][t]; every load in flight before any is used (clamped addresses)
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+      const int tr = min(chunk * RINGS + q_ring(q, lane), A.T - 1);
+      bf[0][q] = Vs[2 * ((long long)tr * ringM + m) + (cc & 1)];                   // north ring
+      bf[1][q] = Vs[2 * ((long long)(A.nrings - 1 - tr) * ringM + m) + (cc & 1)];  // its mirror
+    }
     float mx = 0.f;
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-      bool ok, south;
-      ring_valid(A, bcol, batch, chunk, q, lane, ok, south);
-      const float vn = ok ? __uint_as_float(Bv[0][q]) : 0.f;
-      const float vs = south ? par0 * __uint_as_float(Bv[1][q]) : 0.f;
-      const float bp = lo ? vn : vs, bm = lo ? vs : vn;
-      mx = fmaxf(mx, fmaxf(fabsf(bp), fabsf(bm)));
-      Bv[0][q] = __float_as_uint(bp);
-      Bv[1][q] = __float_as_uint(bm);
+      const int tr = chunk * RINGS + q_ring(q, lane);
+      const bool ok = bcol < batch && tr < A.T;
+      const bool south = ok && tr < A.T - 1;  // the equator ring has no mirror
+      const float vn = ok ? bf[0][q] : 0.f;
+      const float vs = south ? par0 * bf[1][q] : 0.f;
+      // sign + column (p): V+ north and par0 V+ south; sign - column (n): the reverse
+      bf[0][q] = lo ? vn : vs;
+      bf[1][q] = lo ? vs : vn;
+      mx = fmaxf(mx, fmaxf(fabsf(bf[0][q]), fabsf(bf[1][q])));
     }
+    // the map's maximum over its 16 lanes (4 components x 4 K blocks): bits 0, 1, 4, 5
     mx = fmaxf(mx, __shfl_xor(mx, 1));
     mx = fmaxf(mx, __shfl_xor(mx, 2));
     mx = fmaxf(mx, __shfl_xor(mx, 16));
@@ -2212,51 +2251,57 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     const float sB = ldexpf(1.f, e);
     inv = 1.0 / (sA * (double)sB);
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-      Bv[0][q] = split_f16(__uint_as_float(Bv[0][q]) * sB);
-      Bv[1][q] = split_f16(__uint_as_float(Bv[1][q]) * sB);
-    }
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = split_f16(bf[s][8 * u + k] * sB);
+        // hi halves of rings 2i, 2i + 1 -> dword i; lo halves likewise
+        Bq[s][u][0] = make_uint4(__builtin_amdgcn_perm(w[1], w[0], 0x05040100u), __builtin_amdgcn_perm(w[3], w[2], 0x05040100u),
+                                 __builtin_amdgcn_perm(w[5], w[4], 0x05040100u), __builtin_amdgcn_perm(w[7], w[6], 0x05040100u));
+        Bq[s][u][1] = make_uint4(__builtin_amdgcn_perm(w[1], w[0], 0x07060302u), __builtin_amdgcn_perm(w[3], w[2], 0x07060302u),
+                                 __builtin_amdgcn_perm(w[5], w[4], 0x07060302u), __builtin_amdgcn_perm(w[7], w[6], 0x07060302u));
+      }
   }
   float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
   const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
-  const int tcol = s2_tile_col(lane);
   const int ntile = (lmax - lmin + TL) / TL;
-  // 32 recursion steps of this wave's sign into its tile of buffer b (tile jt = steps
-  // lmin + TL jt ...); odd rows of the sign - tile negated, as in k_adj_mx_s2_cbh
+  // 32 recursion steps of this wave's sign into its planes of buffer b; odd rows of the
+  // sign - tile negated (one B serves both parities, as in k_adj_mx_s2_cbh)
   auto rec_tile = [&](int b) {
-    uint32_t* mytile = &tiles[b][rbw][sgw][0][0];
+    uint16_t* ph = &tp[b][rbw][sgw][0][0][lane];
+    uint16_t* pl = &tp[b][rbw][sgw][1][0][lane];
     int act = 0;
 #pragma unroll 1
     for (int j0 = 0; j0 < TL; j0 += 8) {
       const int anydeep = __ballot(valid && ex < 0) != 0ull;
       act |= __ballot(valid && ex == 0) != 0ull;
       const double f = (ex == 0 && valid) ? sA : 0.0;
+      const double fo = sgw ? -f : f;  // odd steps
       double2 ag[8];
 #pragma unroll
-      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[b][j0 + jj];
+      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[b][sgw][j0 + jj];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        const double c = fma(ag[jj].x, x, sgw ? -ag[jj].y : ag[jj].y);
-        const double n = fma(c, mu, -mup);
+        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
         mup = mu;
         mu = n;
-        mytile[s2_tile_row(j0 + jj) * TS2 + tcol] = split_f16((float)(n * ((sgw && (jj & 1)) ? -f : f)));
+        const uint32_t w = split_f16((float)(n * ((jj & 1) ? fo : f)));
+        const int row = s2_tile_row(j0 + jj) * kQ_RS;
+        ph[row] = (uint16_t)w;
+        pl[row] = (uint16_t)(w >> 16);
       }
       if (anydeep) rescale_pair(mu, mup, ex);
     }
     if (lane == 0) s_act[b][wave] = act;
   };
-  const uint32_t* abase0 = &tiles[0][0][0][0][0] + (lane & 15) * TS2 + 16 * (lane >> 4);
-  // contraction of tile jt (buffer b) for this wave's column block, and its partials
+  // A operands: lane (row i = lane % 16, K block k4 = lane / 16) reads 8 rings of a plane row
+  const int a_off = (lane & 15) * kQ_RS + 8 * (lane >> 4);
   auto contract_tile = [&](int jt, int b) {
     int actv[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) actv[w] = s_act[b][w];
-    // pz is always 0 but unknown to the compiler (l is not bounded below 2^30 for it): the
-    // B words stay built in the loop instead of being hoisted as 256 more registers
-    const uint32_t pz = (uint32_t)(lmin + TL * jt > (1 << 30));
-    const uint32_t sel_hh = 0x01000100u | pz, sel_ll = 0x03020302u | pz;
-    const uint32_t* abase = abase0 + b * (4 * 2 * TL * TS2);
     floatx4 D[2][4];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -2267,32 +2312,31 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
 #pragma unroll
       for (int sg = 0; sg < 2; ++sg) {
         if (!actv[2 * rb + sg]) continue;
-        uint4 a[2][4];
+        uint4 ah[2][2], al[2][2];  // [parity][32-ring slice]
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          const uint4* src = reinterpret_cast<const uint4*>(abase + ((rb * 2 + sg) * TL + 16 * p) * TS2);
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int n = 0; n < 4; ++n) a[p][n] = src[n];
-        }
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-          uint32_t hh[4], ll[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const uint32_t w = Bv[sg][16 * rb + 4 * n + i];
-            hh[i] = __builtin_amdgcn_perm(w, w, sel_hh);
-            ll[i] = __builtin_amdgcn_perm(w, w, sel_ll);
+          for (int sl = 0; sl < 2; ++sl) {
+            const int o = a_off + 16 * p * kQ_RS + 32 * sl;
+            ah[p][sl] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][0][0][o]);
+            al[p][sl] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][1][0][o]);
           }
-          const halfx8 bh = __builtin_bit_cast(halfx8, make_uint4(hh[0], hh[1], hh[2], hh[3]));
-          const halfx8 bl = __builtin_bit_cast(halfx8, make_uint4(ll[0], ll[1], ll[2], ll[3]));
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          const halfx8 bh = __builtin_bit_cast(halfx8, Bq[sg][2 * rb + sl][0]);
+          const halfx8 bl = __builtin_bit_cast(halfx8, Bq[sg][2 * rb + sl][1]);
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
-            const halfx8 av = __builtin_bit_cast(halfx8, a[p][n]);
-            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bh, D[p][rb], 0, 0, 0);
-            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bl, D[p][rb], 0, 0, 0);
+            const halfx8 xh = __builtin_bit_cast(halfx8, ah[p][sl]);
+            const halfx8 xl = __builtin_bit_cast(halfx8, al[p][sl]);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, D[p][rb], 0, 0, 0);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, D[p][rb], 0, 0, 0);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, D[p][rb], 0, 0, 0);
+            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bl, D[p][rb], 0, 0, 0);
           }
         }
       }
+    // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p
     if (bcol < batch) {
       const int l0 = lmin + TL * jt;
 #pragma unroll
@@ -2313,9 +2357,13 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     const int l = lmin + TL * jt + j;
     return (l <= lmax && l != lmin) ? make_double2(A.alpha[cb + l], A.gamma[cb + l]) : make_double2(0.0, 0.0);
   };
+  auto stage_ag = [&](int b, double2 v) {
+    s_ag[b][0][tid] = v;
+    s_ag[b][1][tid] = make_double2(v.x, -v.y);
+  };
   if (tid < TL) {
-    s_ag[0][tid] = ag_at(0, tid);
-    s_ag[1][tid] = ag_at(1, tid);
+    stage_ag(0, ag_at(0, tid));
+    stage_ag(1, ag_at(1, tid));
   }
   double2 pf = (tid < TL) ? ag_at(2, tid) : make_double2(0.0, 0.0);
   __syncthreads();
@@ -2326,7 +2374,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     const int b = jt & 1;
     // s_ag[b] held tile jt, read by the recursion before the last barrier: now tile jt + 2
     if (tid < TL) {
-      s_ag[b][tid] = pf;
+      stage_ag(b, pf);
       pf = ag_at(jt + 3, tid);
     }
     const bool more = jt + 1 < ntile;
@@ -2794,11 +2842,12 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     RecArgs at = a;
     const int rings = (MG == 32) ? 256 : 128;
     at.nchunk = (p->T + rings - 1) / rings;
-    dim3 gt(Mpad * at.nchunk, (batch + MG - 1) / MG);
+    constexpr int MB = kMGroup, CB = 1;  // the decode_tile shape of both forms
+    dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
     if (MG == 32)
-      hipLaunchKernelGGL((k_synth_s2p<8, 8, 1>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
+      hipLaunchKernelGGL((k_synth_s2p<8, 8, 1, MB, CB>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
     else
-      hipLaunchKernelGGL((k_synth_s2p<4, 4, 3>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
+      hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, MB, CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
   } else if (p->spin == 2 && MG == 16) {
     RecArgs at = a;
     at.nchunk = (p->T + 127) / 128;

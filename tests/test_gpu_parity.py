@@ -430,11 +430,11 @@ def test_mixed_adjoint_ring_chunks(batch, cb):
 
 
 @pytest.mark.parametrize("nside,lmax,batch", [(32, 40, 7), (64, 128, 32), (512, 640, 40), (128, 300, 33)])
-def test_mixed_adjoint_32_maps_bitwise(nside, lmax, batch):
-    """The 32-map spin-2 adjoint (one recursion sign per wave, double-buffered tiles) gives
-    the bits of the 16-map column-block kernel (the same recursion values, scales, operand
-    words and accumulation order), on one and four ring chunks and with ragged groups, and
-    matches the FP64 adjoint."""
+def test_mixed_adjoint_32_maps(nside, lmax, batch):
+    """The 32-map spin-2 adjoint (one recursion sign per wave, hi / lo planes, double-buffered
+    tiles) against the 16-map column-block kernel (the same split and scales, another
+    summation order: <= 5e-6 of the largest coefficient) and against the FP64 adjoint, on one
+    and four ring chunks and with ragged groups."""
     j = _jht()
     from jht_amd import _lib
     from jht_amd.healpix import Plan
@@ -449,12 +449,12 @@ def test_mixed_adjoint_32_maps_bitwise(nside, lmax, batch):
         b = torch.empty((batch, 2, O.alm_size(lmax)), dtype=torch.complex128, device="cuda")
         p.adjoint(v, b, batch)
         torch.cuda.synchronize()
-        out[g] = b
-    assert torch.equal(out[32], out[16])
+        out[g] = b.cpu().numpy()
     b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
-    bmx = out[32].cpu().numpy()
-    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
-    print(f"32-map adjoint nside={nside} lmax={lmax} batch={batch}: bitwise = 16-map, vs FP64 {ea:.2e}", flush=True)
+    e16 = max(_rel(out[32][i], out[16][i]) for i in range(batch))
+    ea = max(_rel(out[32][i], b64[i]) for i in range(batch))
+    print(f"32-map adjoint nside={nside} lmax={lmax} batch={batch}: vs 16-map {e16:.2e}, vs FP64 {ea:.2e}", flush=True)
+    assert e16 <= 5e-6
     assert ea <= MIXED_TOL
 
 
