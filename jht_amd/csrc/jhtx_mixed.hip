@@ -969,9 +969,16 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
   constexpr int NE = kP_CH * MG / NT;    // staged (l, map) entries per thread
   static_assert(NE * NT == kP_CH * MG, "the threads cover a chunk's entries exactly");
   static_assert(kP_CH % kP_GS == 0, "groups tile the chunk");
-  __shared__ __align__(16) uint32_t s_buf[BUF_DW];
-  __shared__ double2 s_ag[kP_CH][2];  // (alpha, +gamma), (alpha, -gamma)
-  __shared__ float s_sc[MG];
+  // one LDS block, (alpha, gamma) first: its per-step reads then take immediate offsets
+  struct Smem {
+    double2 ag[kP_CH][2];  // (alpha, +gamma), (alpha, -gamma)
+    float sc[MG];
+    __align__(16) uint32_t buf[BUF_DW];
+  };
+  __shared__ Smem sm;
+  auto& s_ag = sm.ag;
+  float* const s_sc = sm.sc;
+  uint32_t* const s_buf = sm.buf;
   uint32_t* const s_b = s_buf;
   int m, chunk;
   if (!decode_tile<MB, CB>(A, (int)blockIdx.x, m, chunk)) return;
@@ -2183,10 +2190,16 @@ __device__ __forceinline__ int q_ring(int t, int lane) {
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
                                                    float2* __restrict__ part, int batch) {
   constexpr int NW = 8, MG = 32, RINGS = 256;
-  // [buffer][ring block][sign][hi / lo][row][ring]
-  __shared__ __align__(16) uint16_t tp[2][4][2][2][TL][kQ_RS];
-  __shared__ double2 s_ag[2][2][TL];  // [buffer][sign][step]: (alpha, +-gamma)
-  __shared__ int s_act[2][NW];        // [buffer][ring block 2 + sign]
+  // one LDS block, (alpha, gamma) first: its per-step reads then take immediate offsets
+  struct Smem {
+    double2 ag[2][2][TL];  // [buffer][sign][step]: (alpha, +-gamma)
+    int act[2][NW];        // [buffer][ring block 2 + sign]
+    __align__(16) uint16_t tp[2][4][2][2][TL][kQ_RS];  // [buffer][ring block][sign][hi / lo][row][ring]
+  };
+  __shared__ Smem sm;
+  auto& s_ag = sm.ag;
+  auto& s_act = sm.act;
+  auto& tp = sm.tp;
   int m, chunk;
   if (!decode_block<true>(A, m, chunk)) return;
   const int mg = blockIdx.y;
@@ -2287,7 +2300,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
         const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
         mup = mu;
         mu = n;
-        const uint32_t w = split_f16((float)(n * ((jj & 1) ? fo : f)));
+        uint32_t w = split_f16((float)(n * ((jj & 1) ? fo : f)));
+        // one packed conversion, then its two halves stored (ds_write_b16 / _d16_hi): left
+        // visible, the compiler converts each half separately
+        asm volatile("" : "+v"(w));
         const int row = s2_tile_row(j0 + jj) * kQ_RS;
         ph[row] = (uint16_t)w;
         pl[row] = (uint16_t)(w >> 16);
