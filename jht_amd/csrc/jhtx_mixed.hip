@@ -1082,6 +1082,47 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
       for (int rb = 0; rb < 4; ++rb) D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[rb], bv, D[rb][c], 0, 0, 0);
     }
   };
+  // mfma_group(jg, tl_m) and rec_group(jg + kP_GS, tl_r) interleaved by hand: column block
+  // c's four matrix instructions, then recursion step(s) c, with block c + 1's B operand read
+  // during block c.  sched_barrier keeps the blocks in this order (left to itself the
+  // compiler issues every matrix instruction first, each B read just before its use, and the
+  // recursion after them).
+  auto fused_group = [&](int jg, const uint32_t* tl_m, uint32_t* tl_r) {
+    static_assert(kP_GS % NCB == 0, "whole recursion steps per column block");
+    constexpr int SPB = kP_GS / NCB;  // recursion steps per column block
+    const int jr = jg + kP_GS;
+    const double f = (ex == 0) ? sA : 0.0;
+    double2 ag[kP_GS];
+#pragma unroll
+    for (int jj = 0; jj < kP_GS; ++jj) ag[jj] = s_ag[jr + jj][sg];
+    halfx8 av[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const uint32_t a0 = tl_m[(2 * kk) * kP_TS + 16 * rb + ii];
+      const uint32_t a1 = tl_m[(2 * kk + 1) * kP_TS + 16 * rb + ii];
+      av[rb] = __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1));
+    }
+    const uint32_t* brow = s_b + sg * NLP * BROW + ((jg >> 1) + kk) * BROW + 4 * ii;
+    halfx8 bv = __builtin_bit_cast(halfx8, *reinterpret_cast<const uint4*>(brow));
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      halfx8 bn = bv;
+      if (c + 1 < NCB) bn = __builtin_bit_cast(halfx8, *reinterpret_cast<const uint4*>(brow + 64 * (c + 1)));
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[rb], bv, D[rb][c], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < SPB; ++u) {
+        const int jj = c * SPB + u;
+        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+        mup = mu;
+        mu = n;
+        tl_r[jj * kP_TS + lane] = split_f16((float)(n * f));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      bv = bn;
+    }
+    rescale_pair(mu, mup, ex);
+  };
   if (lmin <= lmax) prefetch(lmin);
   for (int l0 = lmin; l0 <= lmax; l0 += kP_CH) {
     __syncthreads();
@@ -1113,8 +1154,12 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
       wave_sync();
       bool odd = false;
       for (; j0 + kP_GS < nst; j0 += kP_GS) {
-        mfma_group(j0, odd ? tile1 : tile0);
-        rec_group(j0 + kP_GS, odd ? tile0 : tile1);
+        if constexpr (NCB == 8) {  // the 16-map form keeps its registers (3 work-groups per CU)
+          fused_group(j0, odd ? tile1 : tile0, odd ? tile0 : tile1);
+        } else {
+          mfma_group(j0, odd ? tile1 : tile0);
+          rec_group(j0 + kP_GS, odd ? tile0 : tile1);
+        }
         wave_sync();
         odd = !odd;
       }
