@@ -718,6 +718,15 @@ __device__ __forceinline__ uint32_t split_f16(float v) {
   return __builtin_bit_cast(uint32_t, halfx2{(_Float16)hi, (_Float16)lo});
 }
 
+// The same pair with hi rounded to nearest (|lo| <= 2^-11 |v|, either sign).  Where one
+// operand of a product is split this way, the left-out lo.lo term has a random sign rather
+// than the sign of the product (split_f16's lo always has the sign of v).
+__device__ __forceinline__ uint32_t split_f16_rn(float v) {
+  const _Float16 hi = (_Float16)v;
+  const float lo = v - (float)hi;
+  return __builtin_bit_cast(uint32_t, halfx2{hi, (_Float16)lo});
+}
+
 // (hi, hi) and (lo, lo) from a split_f16 pair
 __device__ __forceinline__ uint32_t dup_lo(uint32_t w) { return (w & 0xffffu) | (w << 16); }
 __device__ __forceinline__ uint32_t dup_hi(uint32_t w) { return (w >> 16) | (w & 0xffff0000u); }
@@ -2219,9 +2228,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
 // split of k_adj_mx_s2_cbh.  The contraction's K index is the ring: the tiles hold the hi
 // and lo halves of the recursion values in separate planes ([row][ring]), and each lane's
 // B operands are the hi and lo halves of its V values packed ring-pairwise, built once per
-// work-group, so the four products come from four instructions with no per-step operand
-// rebuilding: hi.hi, hi.lo, lo.hi and lo.lo over 32 rings each.  The tiles are
-// double-buffered (one barrier per 32 l between the recursion of tile j + 1 and the
+// work-group, so the split products come from one instruction each with no per-step operand
+// rebuilding: hi.hi, hi.lo and lo.hi over 32 rings each.  lo.lo is left out: the halves are
+// below 2^-10 and 2^-11 of their values, so the product is below 2^-21 of a.b, and with V's
+// hi rounded to nearest its sign is random (a quarter fewer matrix instructions).  The tiles
+// are double-buffered (one barrier per 32 l between the recursion of tile j + 1 and the
 // contraction of tile j), and the two waves of each SIMD (w and w + 4) run the two halves in
 // opposite order: one issues FP64 recursion while the other issues matrix instructions.
 constexpr int kQ_RS = 72;  // plane row stride (halves): 144 bytes, rows 4 banks apart
@@ -2314,7 +2325,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
       for (int u = 0; u < 8; ++u) {
         uint32_t w[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = split_f16(bf[s][8 * u + k] * sB);
+        for (int k = 0; k < 8; ++k) w[k] = split_f16_rn(bf[s][8 * u + k] * sB);
         // hi halves of rings 2i, 2i + 1 -> dword i; lo halves likewise
         Bq[s][u][0] = make_uint4(__builtin_amdgcn_perm(w[1], w[0], 0x05040100u), __builtin_amdgcn_perm(w[3], w[2], 0x05040100u),
                                  __builtin_amdgcn_perm(w[5], w[4], 0x05040100u), __builtin_amdgcn_perm(w[7], w[6], 0x05040100u));
@@ -2393,7 +2404,6 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, D[p][rb], 0, 0, 0);
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, D[p][rb], 0, 0, 0);
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, D[p][rb], 0, 0, 0);
-            D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bl, D[p][rb], 0, 0, 0);
           }
         }
       }
