@@ -6,7 +6,9 @@
 // Adjoint (healpix.py:293-305, :332-355): W = unnormalised forward DFT of the ring,
 //   V+_m = W[m mod N] e^{-i m phi0},  V-_m = conj(W[-m mod N]) e^{-i m phi0}.
 // phi0 is either 0 or pi/N (a half-sample shift), so e^{i m phi0} = (-1)^q e^{i pi k / N}
-// for m = k + q N and the phase is applied once per bin.
+// for m = k + q N: the sign per term, the phase once per bin, read from a table (the twiddle
+// table for power-of-two rings, whose length 2N divides; folded into the chirp of chirp-z
+// rings, e^{i pi (k^2 + k) / N} with an exact integer exponent).
 //
 // One work-group handles a mirror ring pair (r, 4 nside - 2 - r): the pair shares N and
 // phi0.  Spin 0 packs the two real rings into one complex transform.  Transforms:
@@ -62,6 +64,11 @@ __device__ __forceinline__ double2 chirp(long long n, int N, int s) {
   const double2 c = cispi((double)r / (double)N);
   return s > 0 ? c : cconj(c);
 }
+// e^{i pi (n^2 + n) / N}: the chirp times the half-sample shift e^{i pi n / N}
+__device__ __forceinline__ double2 chirp_shifted(long long n, int N) {
+  const long long r = (n * n + n) % (2LL * N);
+  return cispi((double)r / (double)N);
+}
 
 struct FftArgs {
   int nside, lmax, M, nrings, spin;
@@ -76,6 +83,7 @@ struct FftArgs {
   const long long* cofs;  // per pair p: offset of its chirp table c_n = e^{i pi n^2/N}, n < N
   const double2* chirps;
   const float2* chirpsf;
+  long long chirp_shift;  // offset of the shifted chirp tables
   int lds_bytes;
   int bluestein_all;
 };
@@ -295,9 +303,12 @@ static_assert(lg_lo(1024, 16) == 14 && lg_hi(1024, 16) == 14, "the 1024-thread c
 // length-N DFT  X_j = sum_k in_k e^{s 2 pi i j k / N}, in = LDS cx<T>[N] at offset 0.
 // Returns the complex offset of the output (N values) in LDS.  Ends with a barrier.
 // --------------------------------------------------------------------------------- //
+// Chirp-z rings with the half-sample shift (sh) take the shifted chirp where the shift
+// belongs: the synthesis multiplies bin k of its input by e^{i pi k / N}, the adjoint bin j
+// of its output by e^{-i pi j / N}.
 template <int E, int LGLO, int LGHI, int KIND, typename T>
 __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, long long bofs,
-                                        const cx<T>* __restrict__ ch, void* lds, int tid, int nthr) {
+                                        const cx<T>* __restrict__ ch, bool sh, void* lds, int tid, int nthr) {
   cx<T>* l2 = reinterpret_cast<cx<T>*>(lds);
   if constexpr (KIND == 0) {
     cx<T>* roots = l2 + N;
@@ -334,12 +345,14 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
     return 0;
   }
   // Bluestein: a_k = in_k c_k, b_n = conj(c_n), c_n = e^{s i pi n^2/N}; X_j = c_j (a * b)_j
+  const cx<T>* cpre = (sh && s > 0) ? ch + A.chirp_shift : ch;
+  const cx<T>* cpost = (sh && s < 0) ? ch + A.chirp_shift : ch;
   if (tid < nt) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
       if (i < N) {
-        const cx<T> c = ch[i];
+        const cx<T> c = cpre[i];
         x[e] = cmul(l2[i], s > 0 ? c : cconj(c));
       } else {
         x[e] = mk2((T)0, (T)0);
@@ -372,7 +385,7 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
     for (int e = 0; e < E; ++e) {
       const int i = tid + nt * e;
       if (i < N) {
-        const cx<T> c = ch[i];
+        const cx<T> c = cpost[i];
         l2[i] = cmul(cscale(x[e], invL), s > 0 ? c : cconj(c));
       }
       if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -385,6 +398,16 @@ __device__ __forceinline__ int ring_dft(const FftArgs& A, int N, int L, int s, l
 // --------------------------------------------------------------------------------- //
 // synthesis fold: D_k (k < N) into LDS dst; combine -> dst_k += i D_k.  Sums in FP64.
 // --------------------------------------------------------------------------------- //
+// e^{i pi k / N}, the half-sample shift of bin k: phase 1 reads it from the twiddle table
+// (exp(-2 pi i k tstep / twL) with twL = 2 N tstep, conjugated), phase 2 computes it
+__device__ __forceinline__ double2 bin_phase(int phase, const double2* __restrict__ tw, int tstep, int k, int N) {
+  if (phase == 1) {
+    const double2 w = tw[k * tstep];
+    return make_double2(w.x, -w.y);
+  }
+  return cispi((double)k / (double)N);
+}
+
 template <typename T>
 __device__ __forceinline__ double2 ld64(const cx<T>* p, int m) {
   const cx<T> v = p[m];
@@ -395,9 +418,11 @@ __device__ __forceinline__ double2 ld64(const cx<T>* p, int m) {
 // after) so that a thread waits for one memory latency per group rather than per term: the
 // large-ring launch classes run one work-group per CU and cannot hide a latency per term.
 template <typename T>
+// sh: the ring is shifted (signs (-1)^q per term); phase: how bin k's e^{i pi k / N} is
+// applied (0 not here -- unshifted, or carried by a chirp-z ring's chirp; 1, 2: bin_phase).
 __device__ __forceinline__ void ring_fold(const cx<T>* __restrict__ Fp, const cx<T>* __restrict__ Fm, int M,
-                          int N, bool sh, bool real0, bool combine, cx<T>* dst,
-                          double2* scratch, int tid, int nthr) {
+                          int N, bool sh, int phase, const double2* __restrict__ tw, int tstep,
+                          bool real0, bool combine, cx<T>* dst, double2* scratch, int tid, int nthr) {
   constexpr int G = sizeof(T) == 4 ? 8 : 4;  // terms in flight per thread and field
   if (N <= nthr) {
     const int S = N * (nthr / N);
@@ -439,7 +464,7 @@ __device__ __forceinline__ void ring_fold(const cx<T>* __restrict__ Fp, const cx
       double2 acc = make_double2(0.0, 0.0);
       for (int j = k; j < S; j += N) acc = cadd(acc, scratch[j]);
       for (int j = (N - k) % N; j < S; j += N) acc = cadd(acc, scratch[S + j]);
-      if (sh) acc = cmul(acc, cispi((double)k / (double)N));
+      if (phase) acc = cmul(acc, bin_phase(phase, tw, tstep, k, N));
       if (combine) {
         const cx<T> d = dst[k];
         dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
@@ -486,7 +511,7 @@ __device__ __forceinline__ void ring_fold(const cx<T>* __restrict__ Fp, const cx
           acc.x += (double)q1[u].x;
           acc.y -= (double)q1[u].y;
         }
-        if (sh) acc = cmul(acc, cispi((double)k / (double)N));
+        if (phase) acc = cmul(acc, bin_phase(phase, tw, tstep, k, N));
         if (combine) {
           const cx<T> d = dst[k];
           dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
@@ -515,7 +540,7 @@ __device__ __forceinline__ void ring_fold(const cx<T>* __restrict__ Fp, const cx
       acc.x += sg * w.x;
       acc.y -= sg * w.y;
     }
-    if (sh) acc = cmul(acc, cispi((double)k / (double)N));
+    if (phase) acc = cmul(acc, bin_phase(phase, tw, tstep, k, N));
     if (combine) {
       const cx<T> d = dst[k];
       dst[k] = mk2((T)(d.x - (T)acc.y), (T)(d.y + (T)acc.x));
@@ -557,11 +582,18 @@ __device__ __forceinline__ void ring_synth_body(const FftArgs& A, const cx<T>* _
   const cx<T>* ch = (A.cofs[p] >= 0) ? chirps_of<T>(A) + A.cofs[p] : nullptr;
   const long long ringM = A.M;
   double2* scratch = fold_scratch<T>(lds, N);  // fold partials (only when N <= BLOCK)
+  // the bin phase of a shifted ring: in the chirp (chirp-z), the twiddle table (power of
+  // two), computed (direct DFT)
+  const int phase = (!sh || KIND == 2) ? 0 : (KIND == 1 ? 1 : 2);
+  const int tstep = (KIND == 1) ? A.twL / (2 * N) : 0;
   if constexpr (SPIN == 0) {
     const cx<T>* Fb = F + (long long)b * A.nrings * ringM;
-    ring_fold<T>(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, true, false, l2, scratch, tid, BLOCK);
-    if (pair) ring_fold<T>(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, true, true, l2, scratch, tid, BLOCK);
-    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
+    ring_fold<T>(Fb + rn * ringM, Fb + rn * ringM, A.M, N, sh, phase, A.tw, tstep, true, false, l2, scratch, tid,
+                 BLOCK);
+    if (pair)
+      ring_fold<T>(Fb + rs * ringM, Fb + rs * ringM, A.M, N, sh, phase, A.tw, tstep, true, true, l2, scratch, tid,
+                   BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, sh, lds, tid, BLOCK);
     const cx<T>* out = l2 + o;
     double* mb = map + (long long)b * A.npix;
     const long long sn = ring_start(A.nside, rn);
@@ -578,8 +610,9 @@ __device__ __forceinline__ void ring_synth_body(const FftArgs& A, const cx<T>* _
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
     const int r = half ? rs : rn;
-    ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, false, false, l2, scratch, tid, BLOCK);
-    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, lds, tid, BLOCK);
+    ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, phase, A.tw, tstep, false, false, l2, scratch, tid,
+                 BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, sh, lds, tid, BLOCK);
     const cx<T>* out = l2 + o;
     const long long st = ring_start(A.nside, r);
     for (int j = tid; j < N; j += BLOCK) {
@@ -617,6 +650,12 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
   const long long ringM = A.M;
   const long long sn = ring_start(A.nside, rn);
   const long long ss = ring_start(A.nside, rs);
+  // e^{-i pi k / N} of a shifted ring: in the chirp (chirp-z), the twiddle table (power of
+  // two), computed (direct DFT).  A shifted chirp-z ring's output bin j carries e^{-i pi j / N}
+  // already, so the conjugated mirror bin N - k carries -e^{-i pi k / N} for k > 0: nflip.
+  const int phase = (!sh || KIND == 2) ? 0 : (KIND == 1 ? 1 : 2);
+  const int tstep = (KIND == 1) ? A.twL / (2 * N) : 0;
+  const bool nflip = KIND == 2 && sh;
   if constexpr (SPIN == 0) {
     const double* mb = map + (long long)b * A.npix;
     // the map is read once: streamed past the caches, 8 loads per ring in flight per thread
@@ -633,16 +672,18 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
         if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vn[u], (T)vs[u]);
     }
     __syncthreads();
-    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, sh, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
     cx<T>* Vn = V + ((long long)b * A.nrings + rn) * ringM;
     cx<T>* Vs = V + ((long long)b * A.nrings + rs) * ringM;
     for (int m = tid; m < A.M; m += BLOCK) {
       const int k = m % N, q = m / N;
-      const cx<T> wk = W[k], wr = cconj(W[(N - k) % N]);
+      const cx<T> wk = W[k];
+      cx<T> wr = cconj(W[(N - k) % N]);
+      if (nflip && k) wr = mk2(-wr.x, -wr.y);
       const double sg = (sh && (q & 1)) ? -0.5 : 0.5;
       double2 t64 = make_double2(sg, 0.0);
-      if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
+      if (phase) t64 = cscale(cconj(bin_phase(phase, A.tw, tstep, k, N)), sg);
       const cx<T> t = to_cx<T>(t64);
       Vn[m] = cmul(cadd(wk, wr), t);
       if (pair) {
@@ -669,7 +710,7 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
         if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vq[u], (T)vu[u]);
     }
     __syncthreads();
-    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, lds, tid, BLOCK);
+    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, sh, lds, tid, BLOCK);
     const cx<T>* W = l2 + o;
     cx<T>* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
     cx<T>* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
@@ -677,10 +718,12 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
       const int k = m % N, q = m / N;
       const double sg = (sh && (q & 1)) ? -1.0 : 1.0;
       double2 t64 = make_double2(sg, 0.0);
-      if (sh) t64 = cscale(cispi(-(double)k / (double)N), sg);
+      if (phase) t64 = cscale(cconj(bin_phase(phase, A.tw, tstep, k, N)), sg);
       const cx<T> t = to_cx<T>(t64);
       Vp[m] = cmul(W[k], t);
-      Vm[m] = cmul(cconj(W[(N - k) % N]), t);
+      cx<T> wr = cconj(W[(N - k) % N]);
+      if (nflip && k) wr = mk2(-wr.x, -wr.y);
+      Vm[m] = cmul(wr, t);
     }
   }
 }
@@ -693,14 +736,18 @@ __global__ __launch_bounds__(BLOCK) void k_ring_adjoint(FftArgs A, const double*
   ring_adjoint_body<BLOCK, E, lg_lo(BLOCK, E), lg_hi(BLOCK, E), SPIN, KIND, T>(A, map, V, p, blockIdx.y, slot & 1);
 }
 
-// Chirp table c_n = e^{i pi (n^2 mod 2N) / N}, n < N, of each Bluestein ring pair.
+// Chirp table c_n = e^{i pi (n^2 mod 2N) / N}, n < N, of each Bluestein ring pair, and its
+// shifted copy e^{i pi ((n^2 + n) mod 2N) / N}.
 __global__ void k_chirp_table(FftArgs A) {
   const int p = A.pairs[blockIdx.x];
   const long long o = A.cofs[p];
   if (o < 0) return;
   const int N = ring_len(A.nside, p);
   double2* c = const_cast<double2*>(A.chirps) + o;
-  for (int n = threadIdx.x; n < N; n += blockDim.x) c[n] = chirp(n, N, +1);
+  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+    c[n] = chirp(n, N, +1);
+    c[A.chirp_shift + n] = chirp_shifted(n, N);
+  }
 }
 
 // Kernel spectrum of a Bluestein ring: FFT_L of b_n = e^{-i pi n^2 / N} (n < N, mirrored),
@@ -761,6 +808,7 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   a.cofs = p->fft.cofs;
   a.chirps = p->fft.chirps;
   a.chirpsf = p->fft.chirpsf;
+  a.chirp_shift = p->fft.chirp_shift;
   a.lds_bytes = c.lds_bytes;
   a.bluestein_all = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
   return a;
@@ -936,6 +984,7 @@ int build_fft_tables(jhtx_plan* p) {
     need[c] = std::max(need[c], nd);
     need64[c] = std::max(need64[c], (kind == 0) ? 3 * N * 16 : L * 16);
     if (kind >= 1) twL = std::max(twL, L);
+    if (kind == 1 && ring_shifted(nside, q)) twL = std::max(twL, 2 * N);  // bin phases from the table
     if (kind == 2) {
       bofs[q] = blen;
       blen += L;
@@ -976,11 +1025,12 @@ int build_fft_tables(jhtx_plan* p) {
   e = hipMalloc(&p->fft.cofs, sizeof(long long) * npairs);
   if (e != hipSuccess) return JHTX_ENOMEM;
   (void)hipMemcpy(p->fft.cofs, cofs.data(), sizeof(long long) * npairs, hipMemcpyHostToDevice);
+  p->fft.chirp_shift = clen;  // the shifted tables follow the plain ones
   if (clen > 0) {
-    e = hipMalloc(&p->fft.chirps, sizeof(double2) * clen);
+    e = hipMalloc(&p->fft.chirps, sizeof(double2) * 2 * clen);
     if (e != hipSuccess) return JHTX_ENOMEM;
     if (f32) {
-      e = hipMalloc(&p->fft.chirpsf, sizeof(float2) * clen);
+      e = hipMalloc(&p->fft.chirpsf, sizeof(float2) * 2 * clen);
       if (e != hipSuccess) return JHTX_ENOMEM;
     }
   }
@@ -1019,7 +1069,8 @@ int build_fft_tables(jhtx_plan* p) {
       launch_spectrum(fc.lc, a, dim3(fc.npairs), fc.lds64_bytes);
     }
     if (f32) {
-      hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)p->fft.chirps, p->fft.chirpsf, clen);
+      hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)p->fft.chirps, p->fft.chirpsf,
+                         2 * clen);
       hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)p->fft.bspec, p->fft.bspecf, blen);
     }
     e = hipDeviceSynchronize();

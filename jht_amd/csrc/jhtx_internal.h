@@ -78,14 +78,18 @@ struct FftTables {
   float2* twf = nullptr;  // FP32 copies of tw, bspec and chirps (mixed-precision plans)
   float2* bspecf = nullptr;
   float2* chirpsf = nullptr;
-  int lmax_fft = 0;       // largest power-of-two transform length in the plan
+  int lmax_fft = 0;       // twiddle table length: the largest power-of-two transform length,
+                          // at least 2N for a shifted power-of-two ring of N pixels
   FftClass cls[12];
   int ncls = 0;
   long long* bofs = nullptr;  // device, per ring pair: offset of its Bluestein kernel spectrum or -1
   double2* bspec = nullptr;   // device, concatenated kernel spectra
   long long bspec_len = 0;
   long long* cofs = nullptr;  // device, per ring pair: offset of its chirp table or -1
-  double2* chirps = nullptr;  // device, concatenated chirp tables e^{i pi n^2 / N}
+  // device, concatenated chirp tables e^{i pi n^2 / N}, then (from offset chirp_shift) the
+  // same tables with the half-sample ring shift folded in, e^{i pi (n^2 + n) / N}
+  double2* chirps = nullptr;
+  long long chirp_shift = 0;
 };
 
 }  // namespace jhtx
