@@ -956,6 +956,8 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const d
 // staged in LDS so that one 16-byte read is the B operand.  Work-groups take the grouped
 // order (decode_block<true>): the 16 consecutive orders m of one ring chunk run back to back
 // on one XCD, so their 8-byte stores F[ring][m] meet in that XCD's L2 as 128-byte runs.
+// Instantiated for 16 maps (4 waves, 128 rings, 3 work-groups per CU); 32 maps per
+// recursion run k_synth_s2q below.
 // ---------------------------------------------------------------------------------- //
 constexpr int kP_GS = 8;   // l steps per pipeline group: one matrix K block
 constexpr int kP_CH = 32;  // l steps per staged chunk of B
@@ -1091,47 +1093,6 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
       for (int rb = 0; rb < 4; ++rb) D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[rb], bv, D[rb][c], 0, 0, 0);
     }
   };
-  // mfma_group(jg, tl_m) and rec_group(jg + kP_GS, tl_r) interleaved by hand: column block
-  // c's four matrix instructions, then recursion step(s) c, with block c + 1's B operand read
-  // during block c.  sched_barrier keeps the blocks in this order (left to itself the
-  // compiler issues every matrix instruction first, each B read just before its use, and the
-  // recursion after them).
-  auto fused_group = [&](int jg, const uint32_t* tl_m, uint32_t* tl_r) {
-    static_assert(kP_GS % NCB == 0, "whole recursion steps per column block");
-    constexpr int SPB = kP_GS / NCB;  // recursion steps per column block
-    const int jr = jg + kP_GS;
-    const double f = (ex == 0) ? sA : 0.0;
-    double2 ag[kP_GS];
-#pragma unroll
-    for (int jj = 0; jj < kP_GS; ++jj) ag[jj] = s_ag[jr + jj][sg];
-    halfx8 av[4];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-      const uint32_t a0 = tl_m[(2 * kk) * kP_TS + 16 * rb + ii];
-      const uint32_t a1 = tl_m[(2 * kk + 1) * kP_TS + 16 * rb + ii];
-      av[rb] = __builtin_bit_cast(halfx8, make_uint4(a0, a0, a1, a1));
-    }
-    const uint32_t* brow = s_b + sg * NLP * BROW + ((jg >> 1) + kk) * BROW + 4 * ii;
-    halfx8 bv = __builtin_bit_cast(halfx8, *reinterpret_cast<const uint4*>(brow));
-#pragma unroll
-    for (int c = 0; c < NCB; ++c) {
-      halfx8 bn = bv;
-      if (c + 1 < NCB) bn = __builtin_bit_cast(halfx8, *reinterpret_cast<const uint4*>(brow + 64 * (c + 1)));
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb) D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[rb], bv, D[rb][c], 0, 0, 0);
-#pragma unroll
-      for (int u = 0; u < SPB; ++u) {
-        const int jj = c * SPB + u;
-        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
-        mup = mu;
-        mu = n;
-        tl_r[jj * kP_TS + lane] = split_f16((float)(n * f));
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      bv = bn;
-    }
-    rescale_pair(mu, mup, ex);
-  };
   if (lmin <= lmax) prefetch(lmin);
   for (int l0 = lmin; l0 <= lmax; l0 += kP_CH) {
     __syncthreads();
@@ -1163,12 +1124,8 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
       wave_sync();
       bool odd = false;
       for (; j0 + kP_GS < nst; j0 += kP_GS) {
-        if constexpr (NCB == 8) {  // the 16-map form keeps its registers (3 work-groups per CU)
-          fused_group(j0, odd ? tile1 : tile0, odd ? tile0 : tile1);
-        } else {
-          mfma_group(j0, odd ? tile1 : tile0);
-          rec_group(j0 + kP_GS, odd ? tile0 : tile1);
-        }
+        mfma_group(j0, odd ? tile1 : tile0);
+        rec_group(j0 + kP_GS, odd ? tile0 : tile1);
         wave_sync();
         odd = !odd;
       }
@@ -1200,6 +1157,255 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
           const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
           if (lmin <= lmax) {
             const double inv = 1.0 / (sA * (double)s_sc[4 * c + g]);  // a power of two: exact
+            const float* e = ep + lane * 16 + 4 * g;
+            Fx[rn] = f2(inv * e[0], inv * e[1]);
+            if (t < A.T - 1) Fy[rs] = f2(inv * e[2], inv * e[3]);
+          } else {
+            Fx[rn] = make_float2(0.f, 0.f);
+            if (t < A.T - 1) Fy[rs] = make_float2(0.f, 0.f);
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// Spin-2 synthesis for 32 maps per recursion with hi / lo planes (the layout of the 32-map
+// adjoint k_adj_s2p, transposed).  The ring / sign / parity-fold arrangement is that of
+// k_synth_s2p (with 8 waves): wave w runs recursion sign w & 1 for the 64 rings of ring block
+// w >> 1 and contracts them against all 128 B columns (32 maps x [X.re, X.im, s Y.re,
+// s Y.im]).  What changes is the K index of a matrix instruction: 32 consecutive l steps.
+// The recursion values of a 32-step chunk go to the wave's tile as FP16 hi and lo planes
+// ([ring][step], 8 steps per 16-byte store), G_l a_lm to B planes ([column][step]) likewise,
+// and each lane's A and B operands are single 16-byte reads, so the split products are one
+// instruction each: hi.hi, hi.lo and lo.hi over 32 steps (three instructions where
+// k_synth_s2p issues four; lo.lo, below 2^-21 of a product, is left out, and B's hi is
+// rounded to nearest so the dropped term has no sign bias).  The B planes are
+// double-buffered, one barrier per chunk; the matrix instructions of chunk j and the
+// recursion of chunk j + 1 (into the tile the wave has just read into registers) form one
+// block, interleaved by column block.
+// ---------------------------------------------------------------------------------- //
+constexpr int kQ_CH = 32;  // l steps per chunk: one matrix K block
+constexpr int kQ_PS = 72;  // plane row stride (halves): hi 0..31, lo 32..63, pad; 144 bytes
+
+// (hi, hi) and (lo, lo) halves of two FP32 values: hi by mantissa mask (truncation)
+__device__ __forceinline__ void split2_f16(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const float ha = __uint_as_float(__float_as_uint(a) & 0xffffe000u);
+  const float hb = __uint_as_float(__float_as_uint(b) & 0xffffe000u);
+  hi = __builtin_bit_cast(uint32_t, halfx2{(_Float16)ha, (_Float16)hb});
+  lo = __builtin_bit_cast(uint32_t, halfx2{(_Float16)(a - ha), (_Float16)(b - hb)});
+}
+// the same with hi rounded to nearest (lo of either sign)
+__device__ __forceinline__ void split2_f16_rn(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const halfx2 h = halfx2{(_Float16)a, (_Float16)b};
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, halfx2{(_Float16)(a - (float)h[0]), (_Float16)(b - (float)h[1])});
+}
+
+template <int MB, int CB>
+__global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* __restrict__ alm,
+                                                     const float* __restrict__ sB, double sA,
+                                                     float2* __restrict__ F, int batch) {
+  constexpr int NCB = 8, NW = 8, MG = 32, NCOL = 4 * MG, NT = 64 * NW, RINGS = 32 * NW;
+  struct Smem {
+    double2 ag[2][kQ_CH][2];                       // [buffer][step][sign]: (alpha, +-gamma)
+    float sc[MG];
+    __align__(16) uint16_t bp[2][2][NCOL][kQ_PS];  // [buffer][sign][column][hi | lo]
+    __align__(16) uint16_t tp[NW][64][kQ_PS];      // [wave][ring][hi | lo]
+  };
+  static_assert(sizeof(uint16_t) * NW * 64 * kQ_PS >= sizeof(float) * NW * 64 * 16, "the epilogue fits the tiles");
+  __shared__ Smem sm;
+  int m, chunk;
+  if (!decode_tile<MB, CB>(A, (int)blockIdx.x, m, chunk)) return;
+  const int mg = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int sg = wave & 1;
+  const int ii = lane & 15, kk = lane >> 4;
+  const int lmax = A.lmax;
+  const int lmin = max(m, 2);
+  const long long cb0 = (long long)m * (2 * lmax + 1 - m) / 2;
+  const int t = chunk * RINGS + 64 * (wave >> 1) + lane;
+  const bool valid = t < A.T;
+  double x = 0.0, mu = 0.0, mup = 0.0;
+  int ex = 0;
+  if (valid && lmin <= lmax) {
+    x = A.x[t];
+    const double* sd = A.seed + 8 * m + 4 * sg;
+    double sv;
+    seed_split(spin2_seed_L(sd, A.lcos[t], A.lsin[t]), sd[3], sv, ex);
+    mup = -sv;
+  }
+  floatx4 D[4][NCB];  // [row block][column block]
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) D[rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  if (tid < MG) sm.sc[tid] = (mg * MG + tid < batch) ? sB[mg * MG + tid] : 0.f;
+  // staging: thread (step pair jp = tid % 16, map g = tid / 16) takes steps 2 jp, 2 jp + 1
+  const int sjp = tid & 15, sgm = tid >> 4;
+  double2 pf_e[2], pf_b[2];
+  double pf_g[2];
+  auto prefetch = [&](int l0n) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int l = l0n + 2 * sjp + e, b = mg * MG + sgm;
+      const bool ok = l <= lmax && b < batch;
+      pf_g[e] = ok ? A.G[cb0 + l] : 0.0;
+      pf_e[e] = ok ? alm[((long long)b * 2 + 0) * A.K + cb0 + l] : make_double2(0.0, 0.0);
+      pf_b[e] = ok ? alm[((long long)b * 2 + 1) * A.K + cb0 + l] : make_double2(0.0, 0.0);
+    }
+  };
+  // (alpha, gamma) of step j of the chunk at l0c (zero past lmax, and at lmin: the seed step)
+  auto ag_at = [&](int l0c, int j) {
+    const int l = l0c + j;
+    return (l <= lmax && l != lmin) ? make_double2(A.alpha[cb0 + l], A.gamma[cb0 + l]) : make_double2(0.0, 0.0);
+  };
+  auto stage_ag = [&](int buf, double2 v) {
+    sm.ag[buf][tid][0] = v;
+    sm.ag[buf][tid][1] = make_double2(v.x, -v.y);
+  };
+  auto store_b = [&](int l0c, int buf) {
+    const double gsc = (double)sm.sc[sgm];
+    float vp[2][4], vm[2][4];  // [step][column]: sign + and sign - B values
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double gs = pf_g[e] * gsc;
+      const double er = pf_e[e].x * gs, ei = pf_e[e].y * gs, br = pf_b[e].x * gs, bi = pf_b[e].y * gs;
+      // bp = -(E + iB), bm = -(E - iB); s = (-1)^(l+m) on the folded copies
+      const float s = (((l0c + 2 * sjp + e + m) & 1) != 0) ? -1.f : 1.f;
+      const float bpr = (float)(bi - er), bpi = (float)(-ei - br), bmr = (float)(-er - bi), bmi = (float)(br - ei);
+      vp[e][0] = bpr, vp[e][1] = bpi, vp[e][2] = s * bmr, vp[e][3] = s * bmi;
+      vm[e][0] = bmr, vm[e][1] = bmi, vm[e][2] = s * bpr, vm[e][3] = s * bpi;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t h, l;
+      split2_f16_rn(vp[0][c], vp[1][c], h, l);
+      *reinterpret_cast<uint32_t*>(&sm.bp[buf][0][4 * sgm + c][2 * sjp]) = h;
+      *reinterpret_cast<uint32_t*>(&sm.bp[buf][0][4 * sgm + c][kQ_CH + 2 * sjp]) = l;
+      split2_f16_rn(vm[0][c], vm[1][c], h, l);
+      *reinterpret_cast<uint32_t*>(&sm.bp[buf][1][4 * sgm + c][2 * sjp]) = h;
+      *reinterpret_cast<uint32_t*>(&sm.bp[buf][1][4 * sgm + c][kQ_CH + 2 * sjp]) = l;
+    }
+  };
+  uint16_t* const trow = &sm.tp[wave][lane][0];  // this lane's ring row
+  // 8 recursion steps (group q of the chunk in buffer abuf) into the tile; groups where no lane
+  // of the wave is active (every value below 2^-256) store zeros and only advance the recursion
+  auto rec8 = [&](int abuf, int q, bool& act) {
+    const bool on = __ballot(valid && ex == 0) != 0ull;
+    const bool deep = __ballot(valid && ex < 0) != 0ull;  // exponents never decrease
+    act |= on;
+    const double f = (ex == 0) ? sA : 0.0;
+    uint32_t hw[4], lw[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const double2 a0 = sm.ag[abuf][8 * q + 2 * p][sg];
+      const double2 a1 = sm.ag[abuf][8 * q + 2 * p + 1][sg];
+      const double n0 = fma(fma(a0.x, x, a0.y), mu, -mup);
+      const double n1 = fma(fma(a1.x, x, a1.y), n0, -mu);
+      mup = n0;
+      mu = n1;
+      split2_f16((float)(n0 * f), (float)(n1 * f), hw[p], lw[p]);
+    }
+    *reinterpret_cast<uint4*>(trow + 8 * q) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    *reinterpret_cast<uint4*>(trow + kQ_CH + 8 * q) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    if (deep) rescale_pair(mu, mup, ex);
+  };
+  const int nch = (lmin <= lmax) ? (lmax - lmin + kQ_CH) / kQ_CH : 0;
+  bool act_cur = false;
+  if (nch > 0) {
+    // chunk 0's B and (alpha, gamma) and chunk 1's (alpha, gamma); then chunk 1's a_lm in flight
+    prefetch(lmin);
+    if (tid < kQ_CH) {
+      stage_ag(0, ag_at(lmin, tid));
+      stage_ag(1, ag_at(lmin + kQ_CH, tid));
+    }
+    __syncthreads();  // sm.sc
+    store_b(lmin, 0);
+    if (nch > 1) prefetch(lmin + kQ_CH);
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) rec8(0, q, act_cur);
+    wave_sync();
+  }
+  const uint16_t* const arow = &sm.tp[wave][ii][8 * kk];  // A operands: ring 16 rb + ii, steps 8 kk..
+#pragma unroll 1
+  for (int jc = 0; jc < nch; ++jc) {
+    const int b = jc & 1;
+    const int l0 = lmin + kQ_CH * jc;
+    const bool more = jc + 1 < nch;
+    __syncthreads();  // chunk jc's B and chunk jc + 1's (alpha, gamma) visible; buffer b ^ 1 free
+    if (more) store_b(l0 + kQ_CH, b ^ 1);
+    if (jc + 2 < nch) prefetch(l0 + 2 * kQ_CH);
+    if (tid < kQ_CH) stage_ag(b, ag_at(l0 + 2 * kQ_CH, tid));  // chunk jc + 2 (read after the next barrier)
+    // this chunk's A operands, read before the recursion of chunk jc + 1 overwrites the tile
+    uint4 ah[4], al[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      ah[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS);
+      al[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS + kQ_CH);
+    }
+    const uint16_t* brow = &sm.bp[b][sg][ii][8 * kk];
+    bool act_next = false;
+    const bool mma = act_cur;
+    // column block c: its three matrix instructions per row block (block c + 1's B read
+    // first), then 4 recursion steps of chunk jc + 1 (a group of 8 stored after every second
+    // block)
+    uint4 bq[2][2];  // [block parity][hi / lo]: no register copies between blocks
+    bq[0][0] = bq[0][1] = bq[1][0] = bq[1][1] = make_uint4(0u, 0u, 0u, 0u);
+    if (mma) {
+      bq[0][0] = *reinterpret_cast<const uint4*>(brow);
+      bq[0][1] = *reinterpret_cast<const uint4*>(brow + kQ_CH);
+    }
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      if (mma && c + 1 < NCB) {
+        bq[(c + 1) & 1][0] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS);
+        bq[(c + 1) & 1][1] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS + kQ_CH);
+      }
+      if (mma) {
+        const halfx8 bh = __builtin_bit_cast(halfx8, bq[c & 1][0]);
+        const halfx8 bl = __builtin_bit_cast(halfx8, bq[c & 1][1]);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const halfx8 xh = __builtin_bit_cast(halfx8, ah[rb]);
+          const halfx8 xl = __builtin_bit_cast(halfx8, al[rb]);
+          D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, D[rb][c], 0, 0, 0);
+          D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, D[rb][c], 0, 0, 0);
+          D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, D[rb][c], 0, 0, 0);
+        }
+      }
+      if (more && (c & 1)) rec8(b ^ 1, c >> 1, act_next);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    wave_sync();
+    act_cur = act_next;
+  }
+  // epilogue: per column block, the wave's D through LDS to (ring, map) rows; the scales
+  // are undone in FP64
+  __syncthreads();
+  float* ep = reinterpret_cast<float*>(&sm.tp[0][0][0]) + wave * 64 * 16;
+  const long long ringM = A.M;
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ep[(16 * rb + 4 * kk + r) * 16 + ii] = D[rb][c][r];
+    wave_sync();
+    if (valid) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int bm = mg * MG + 4 * c + g;
+        if (bm < batch) {
+          float2* Fx = F + ((long long)bm * 2 + sg) * A.nrings * ringM;        // the field of this sign
+          float2* Fy = F + ((long long)bm * 2 + (sg ^ 1)) * A.nrings * ringM;  // the other field
+          const long long rn = (long long)t * ringM + m;
+          const long long rs = (long long)(A.nrings - 1 - t) * ringM + m;
+          if (lmin <= lmax) {
+            const double inv = 1.0 / (sA * (double)sm.sc[4 * c + g]);  // a power of two: exact
             const float* e = ep + lane * 16 + 4 * g;
             Fx[rn] = f2(inv * e[0], inv * e[1]);
             if (t < A.T - 1) Fy[rs] = f2(inv * e[2], inv * e[3]);
@@ -2905,7 +3111,8 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
   if (p->spin == 2 && MG == 32 && !(p->mx_f16 && p->mx_synth_fold && scale_ws)) MG = 16;
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
   if (p->spin == 2 && (MG == 16 || MG == 32) && p->mx_f16 && p->mx_synth_fold && scale_ws) {
-    // parity folded into B: 16 maps x 128 rings (4 waves) or 32 maps x 256 rings (8 waves)
+    // parity folded into B: 16 maps x 128 rings (k_synth_s2p, 4 waves) or 32 maps x 256 rings
+    // (k_synth_s2q, 8 waves, hi / lo planes)
     double* part = reinterpret_cast<double*>(scale_ws);
     float* sB = reinterpret_cast<float*>(part + (size_t)kScaleBlocks * batch);
     hipLaunchKernelGGL(k_alm_absmax, dim3(kScaleBlocks, batch), dim3(256), 0, s, alm, 2 * p->K, part);
@@ -2916,7 +3123,7 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     constexpr int MB = kMGroup, CB = 1;  // the decode_tile shape of both forms
     dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
     if (MG == 32)
-      hipLaunchKernelGGL((k_synth_s2p<8, 8, 1, MB, CB>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
+      hipLaunchKernelGGL((k_synth_s2q<MB, CB>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
     else
       hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, MB, CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
   } else if (p->spin == 2 && MG == 16) {
