@@ -339,13 +339,16 @@ hipError_t launch_adjoint_rec_finish(const jhtx_plan* p, const void* F, void* pa
   hipError_t e;
   double2* out = reinterpret_cast<double2*>(alm);
   if (p->precision == JHTX_PREC_MIXED && !single_route(p, batch)) {
-    float2* part32 = reinterpret_cast<float2*>(part);
+    // the first-stored-l table (if the kernel keeps one), then the partials
+    const size_t lfb = adj_lfirst_bytes(p);
+    int* lfirst = lfb ? reinterpret_cast<int*>(part) : nullptr;
+    float2* part32 = reinterpret_cast<float2*>(reinterpret_cast<char*>(part) + lfb);
     StageTimer t3(p, s, 3);
-    e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, batch, s);
+    e = launch_adj_mixed(p, reinterpret_cast<const float2*>(F), part32, lfirst, batch, s);
     t3.done();
     if (e != hipSuccess) return e;
     StageTimer t4(p, s, 4);
-    e = launch_adj_finish(p, part32, out, batch, mixed_adj_chunks(p), s);
+    e = launch_adj_finish(p, part32, lfirst, out, batch, mixed_adj_chunks(p), s);
     t4.done();
   } else {
     double2* part64 = reinterpret_cast<double2*>(part);
@@ -367,7 +370,7 @@ size_t adjoint_scratch_bytes(const jhtx_plan* p, int batch, size_t* part_offset)
   const size_t v_bytes = align256(f_elem * (size_t)batch * p->ncomp * p->nrings * p->M);
   size_t part;
   if (mixed && !single_route(p, batch))
-    part = sizeof(float2) * (size_t)mixed_adj_chunks(p) * batch * p->ncomp * p->K;
+    part = adj_lfirst_bytes(p) + sizeof(float2) * (size_t)mixed_adj_chunks(p) * batch * p->ncomp * p->K;
   else
     part = sizeof(double2) * (size_t)adj_rec_chunks(p) * batch * p->ncomp * p->K;
   if (part_offset) *part_offset = v_bytes;
@@ -497,7 +500,8 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   // calls through the FP64 kernels keep FP64 partials)
   size_t part_bytes;
   if (p->precision == JHTX_PREC_MIXED) {
-    part_bytes = sizeof(float2) * (size_t)jhtx::mixed_adj_chunks(p) * p->max_batch * p->ncomp * p->K;
+    part_bytes = jhtx::adj_lfirst_bytes(p) + sizeof(float2) * (size_t)jhtx::mixed_adj_chunks(p) * p->max_batch *
+                                                 p->ncomp * p->K;
     if (p->mx_single) part_bytes = std::max<size_t>(part_bytes, sizeof(double2) * (size_t)p->adj_nchunk * p->ncomp * p->K);
   } else {
     part_bytes = sizeof(double2) * (size_t)p->adj_nchunk * p->max_batch * p->ncomp * p->K;

@@ -142,13 +142,17 @@ hipError_t launch_adj_rec(const jhtx_plan* p, const float2* V, double2* part, in
 int adj_rec_chunks(const jhtx_plan* p);
 hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* alm, int batch,
                              int nchunk, hipStream_t s);
-hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
+// lfirst: the mixed adjoint's first-stored-l table (or null), adj_lfirst_bytes() before the partials
+hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, const int* lfirst, double2* alm, int batch,
                              int nchunk, hipStream_t s);
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
                               void* scale_ws, hipStream_t s);
 constexpr int kScaleBlocks = 256;  // partial maxima per map of the FP16 split's scale pass
-hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int* lfirst, int batch,
                             hipStream_t s);
+// bytes of the mixed adjoint's first-stored-l table (0 where its kernel stores every partial);
+// it sits at the start of the partials region
+size_t adj_lfirst_bytes(const jhtx_plan* p);
 // ring chunks (partials per map) of the mixed adjoint
 int mixed_adj_chunks(const jhtx_plan* p);
 // F / V: complex128 (FP64 plans) or complex64 (mixed-precision plans)

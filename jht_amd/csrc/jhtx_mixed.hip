@@ -2449,8 +2449,11 @@ __device__ __forceinline__ int q_ring(int t, int lane) {
   return 64 * (t >> 4) + 32 * ((t >> 3) & 1) + 8 * (lane >> 4) + (t & 7);
 }
 
+// Tiles before the first tile in which any of the work-group's recursions is active (values
+// at or above 2^-256: activity only grows with l) contribute exactly zero: their partials are
+// not stored, and lfirst[chunk * M + m] records the first stored l for the chunk reduction.
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
-                                                   float2* __restrict__ part, int batch) {
+                                                   float2* __restrict__ part, int* __restrict__ lfirst, int batch) {
   constexpr int NW = 8, MG = 32, RINGS = 256;
   // one LDS block, (alpha, gamma) first: its per-step reads then take immediate offsets
   struct Smem {
@@ -2469,7 +2472,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   const int sgw = wave & 1, rbw = wave >> 1;  // this wave's recursion: sign, ring block
   const int lmax = A.lmax;
   const int lmin = max(m, 2);
-  if (lmin > lmax) return;
+  if (lmin > lmax) {
+    if (mg == 0 && tid == 0) lfirst[(long long)chunk * A.M + m] = lmax + 1;
+    return;
+  }
   const long long cb = (long long)m * (2 * lmax + 1 - m) / 2;
   const long long ringM = A.M;
   const int t = chunk * RINGS + 64 * rbw + lane;
@@ -2542,6 +2548,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
   const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
   const int ntile = (lmax - lmin + TL) / TL;
+  int jf = ntile;  // first active tile (work-group uniform)
   // 32 recursion steps of this wave's sign into its planes of buffer b; odd rows of the
   // sign - tile negated (one B serves both parities, as in k_adj_mx_s2_cbh)
   auto rec_tile = [&](int b) {
@@ -2578,8 +2585,13 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   const int a_off = (lane & 15) * kQ_RS + 8 * (lane >> 4);
   auto contract_tile = [&](int jt, int b) {
     int actv[NW];
+    bool any = false;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) actv[w] = s_act[b][w];
+    for (int w = 0; w < NW; ++w) {
+      actv[w] = s_act[b][w];
+      any |= actv[w] != 0;
+    }
+    if (any && jf == ntile) jf = jt;
     floatx4 D[2][4];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -2614,7 +2626,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
         }
       }
     // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p
-    if (bcol < batch) {
+    if (bcol < batch && jf <= jt) {
       const int l0 = lmin + TL * jt;
 #pragma unroll
       for (int p = 0; p < 2; ++p)
@@ -2664,6 +2676,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     }
     __syncthreads();
   }
+  if (mg == 0 && tid == 0) lfirst[(long long)chunk * A.M + m] = lmin + TL * jf;  // above lmax if none
 }
 
 // spin 0: columns j = 2 g + (re, im), MG = 8 maps; U0 = Vn + Vs (even l - m), U1 = Vn - Vs
@@ -3093,6 +3106,18 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
 // ring chunks of the mixed adjoint: every kernel's work-group covers 256 rings
 int mixed_adj_chunks(const jhtx_plan* p) { return (p->T + 255) / 256; }
 
+int mixed_group(const jhtx_plan* p, bool adjoint);
+
+// the 32-map spin-2 adjoint k_adj_s2p runs for this plan
+static bool adj_s2p(const jhtx_plan* p) {
+  return p->spin == 2 && mixed_group(p, true) == 32 && p->mx_adj_cb && p->mx_adj_f16;
+}
+
+size_t adj_lfirst_bytes(const jhtx_plan* p) {
+  if (p->precision != JHTX_PREC_MIXED || !adj_s2p(p)) return 0;
+  return ((sizeof(int) * (size_t)mixed_adj_chunks(p) * p->M) + 255) & ~(size_t)255;
+}
+
 int mixed_group(const jhtx_plan* p, bool adjoint) {
   if (p->spin == 0) {
     const int g = adjoint ? p->mx_adj_group : p->mx_synth_group;
@@ -3162,18 +3187,18 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
   return hipGetLastError();
 }
 
-hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int batch,
+hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int* lfirst, int batch,
                             hipStream_t s) {
   RecArgs a = make_args_mixed(p);
   a.nchunk = mixed_adj_chunks(p);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   int MG = mixed_group(p, true);
   // 32 maps per adjoint recursion: the FP16-split column-block kernel only
-  if (MG == 32 && !(p->spin == 2 && p->mx_adj_cb && p->mx_adj_f16)) MG = 16;
+  if (MG == 32 && !adj_s2p(p)) MG = 16;
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
-  if (p->spin == 2 && MG == 32 && p->mx_adj_cb && p->mx_adj_f16)
+  if (MG == 32)
     hipLaunchKernelGGL(k_adj_s2p, dim3(Mpad * a.nchunk, (batch + 31) / 32), dim3(512), 0, s, a, V, p->mx_sA, part,
-                       batch);
+                       lfirst, batch);
   else if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
     hipLaunchKernelGGL(k_adj_mx_s0_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
   else if (p->spin == 0 && MG == 16)

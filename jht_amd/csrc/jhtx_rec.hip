@@ -702,23 +702,48 @@ __global__ __launch_bounds__(256) void k_adj_s2(RecArgs A, const C* __restrict__
 // (fixed).  One thread per (map, packed index): the grid runs over the flat triangle,
 // so short high-m columns waste no threads, and the chunk loads are issued 8 at a time.
 // Entries with l < |s| are zero; they sit in the columns m < |s|.
+// lfirst (or null): lfirst[c * M + m] is the first l at which ring chunk c's partial of column
+// m was stored; below it the partial is zero and is neither stored nor read.
+// idx(l, m) = col_base(m) + l; column m holds l = m .. lmax, so it starts at col_base(m) + m
+__device__ __forceinline__ long long col_base(int m, int lmax) { return (long long)m * (2 * lmax + 1 - m) / 2; }
+
+// the order m of packed index idx: the largest m with col_base(m) + m = m (2 lmax + 3 - m) / 2
+// at or below idx (the root of that quadratic, then corrected by at most a step or two)
+__device__ __forceinline__ int m_of_index(long long idx, int lmax) {
+  const double a = 2.0 * lmax + 3.0;
+  int m = (int)(0.5 * (a - sqrt(fmax(a * a - 8.0 * (double)idx, 0.0))));
+  m = max(0, min(m, lmax));
+  while (m > 0 && col_base(m, lmax) + m > idx) --m;
+  while (m < lmax && col_base(m + 1, lmax) + m + 1 <= idx) ++m;
+  return m;
+}
+
 template <typename P>
 __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int nchunk, const P* __restrict__ part,
-                                                double2* __restrict__ alm, long long idx, int b, int batch) {
+                                                const int* __restrict__ lfirst, double2* __restrict__ alm,
+                                                long long idx, int b, int batch) {
   if (idx >= A.K) return;
   const int ncomp = spin ? 2 : 1;
   const double g = A.G[idx];
   // part[((c * batch + b) * ncomp + comp) * K + idx]
   const P* p0 = part + ((long long)b * ncomp) * A.K + idx;
   const long long cstride = (long long)batch * ncomp * A.K;
+  int mi = 0, li = 0;
+  if (lfirst) {
+    mi = m_of_index(idx, A.lmax);
+    li = (int)(idx - col_base(mi, A.lmax));
+  }
+  // chunk c holds a stored partial at this index
+  auto stored = [&](int c) { return !lfirst || li >= lfirst[(long long)c * A.M + mi]; };
   double2 s0 = make_double2(0.0, 0.0), s1 = s0;
   int c = 0;
   for (; c + 8 <= nchunk; c += 8) {
     P u[8], v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      u[k] = p0[(c + k) * cstride];
-      if (spin) v[k] = p0[(c + k) * cstride + A.K];
+      const bool ok = stored(c + k);
+      u[k] = ok ? p0[(c + k) * cstride] : P{};
+      if (spin) v[k] = ok ? p0[(c + k) * cstride + A.K] : P{};
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -731,6 +756,7 @@ __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int 
     }
   }
   for (; c < nchunk; ++c) {
+    if (!stored(c)) continue;
     const P u = p0[c * cstride];
     s0.x += (double)u.x;
     s0.y += (double)u.y;
@@ -765,8 +791,8 @@ __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int 
 
 template <typename P>
 __global__ __launch_bounds__(256) void k_adj_finish(RecArgs A, int spin, int nchunk, const P* __restrict__ part,
-                                                    double2* __restrict__ alm) {
-  adj_finish_body<P>(A, spin, nchunk, part, alm, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
+                                                    const int* __restrict__ lfirst, double2* __restrict__ alm) {
+  adj_finish_body<P>(A, spin, nchunk, part, lfirst, alm, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
                      gridDim.y);
 }
 
@@ -784,7 +810,7 @@ __global__ __launch_bounds__(256) void k_group_finish(const GroupFin* __restrict
   const RecArgs A = G[i].A;
   const int kb = G[i].kblocks;
   const float2* part = reinterpret_cast<const float2*>(C.ws + G[i].part_off);
-  adj_finish_body<float2>(A, G[i].spin, G[i].nchunk, part, reinterpret_cast<double2*>(C.out[i]),
+  adj_finish_body<float2>(A, G[i].spin, G[i].nchunk, part, nullptr, reinterpret_cast<double2*>(C.out[i]),
                           (long long)(local % kb) * 256 + threadIdx.x, local / kb, G[i].batch);
 }
 
@@ -888,15 +914,15 @@ hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* a
                              int nchunk, hipStream_t s) {
   const RecArgs a = make_args(p, true);
   hipLaunchKernelGGL(k_adj_finish<double2>, dim3((unsigned)((p->K + 255) / 256), batch), dim3(256), 0, s, a, p->spin,
-                     nchunk, part, alm);
+                     nchunk, part, (const int*)nullptr, alm);
   return hipGetLastError();
 }
 
-hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, double2* alm, int batch,
+hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, const int* lfirst, double2* alm, int batch,
                              int nchunk, hipStream_t s) {
   const RecArgs a = make_args(p, true);
   hipLaunchKernelGGL(k_adj_finish<float2>, dim3((unsigned)((p->K + 255) / 256), batch), dim3(256), 0, s, a, p->spin,
-                     nchunk, part, alm);
+                     nchunk, part, lfirst, alm);
   return hipGetLastError();
 }
 
