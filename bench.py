@@ -48,14 +48,15 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # a kernel change needs a new profile.
 TRAFFIC_PROFILE = "profiles/kernel_stats_c3_spin2_mixed_batch32.md"
 MEASURED_TRAFFIC = {
-    ("mixed", 2048, 4096, 2, 32, "adj_rec"): 105.56e9,
-    ("mixed", 2048, 4096, 2, 32, "synth_rec"): 196.22e9,
+    ("mixed", 2048, 4096, 2, 32, "adj_rec"): 91.06e9,
+    ("mixed", 2048, 4096, 2, 32, "synth_rec"): 149.40e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X dense FP16 matrix
-# the FP16-split kernels spend 4 FP16 products (hi hi, lo hi, hi lo, lo lo) per FP32-equivalent
-# product: their ceiling for the FP32-equivalent contraction flops is a quarter of the FP16 peak
-F16_SPLIT_PRODUCTS = 4
+# FP16 products per FP32-equivalent product of the FP16-split kernels: the 16-map forms issue
+# all four (hi hi, lo hi, hi lo, lo lo), the 32-map spin-2 forms leave lo lo out; their ceiling
+# for the FP32-equivalent contraction flops is the FP16 peak divided by that count
+F16_SPLIT_PRODUCTS = {16: 4, 32: 3}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -475,8 +476,10 @@ def main():
     if fp64_kernels:
         flops, peak, bound = tri * (26 if spin else 8) * B, PEAK_FP64_TFLOPS, "mfma"
     elif f16[dom]:
-        flops, peak, bound = tri * 16 * B, PEAK_FP16_MFMA_TFLOPS / F16_SPLIT_PRODUCTS, "mfma"
-        peak_basis = "fp16 dense 2500 TFLOP/s / 4 fp16 products per fp32-equivalent product"
+        group = tun["mx_adj_group"] if dom == "adj_rec" else tun["mx_synth_group"]
+        nprod = F16_SPLIT_PRODUCTS[32 if (spin == 2 and group >= 32) else 16]
+        flops, peak, bound = tri * 16 * B, round(PEAK_FP16_MFMA_TFLOPS / nprod, 1), "mfma"
+        peak_basis = f"fp16 dense 2500 TFLOP/s / {nprod} fp16 products per fp32-equivalent product"
     else:
         flops, peak, bound = tri * (16 if spin else 4) * B, PEAK_FP32_MFMA_TFLOPS, "mfma"
     achieved = flops / (per[dom][0] * 1e-3) / 1e12
