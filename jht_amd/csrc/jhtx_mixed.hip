@@ -1350,9 +1350,8 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     const uint16_t* brow = &sm.bp[b][sg][ii][8 * kk];
     bool act_next = false;
     const bool mma = act_cur;
-    // column block c: its three matrix instructions per row block (block c + 1's B read
-    // first), then 4 recursion steps of chunk jc + 1 (a group of 8 stored after every second
-    // block)
+    // column block c: its three matrix instructions per row block, block c + 1's B read, then
+    // 4 recursion steps of chunk jc + 1 (a group of 8 stored after every second block)
     uint4 bq[2][2];  // [block parity][hi / lo]: no register copies between blocks
     bq[0][0] = bq[0][1] = bq[1][0] = bq[1][1] = make_uint4(0u, 0u, 0u, 0u);
     if (mma) {
@@ -1361,10 +1360,6 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     }
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
-      if (mma && c + 1 < NCB) {
-        bq[(c + 1) & 1][0] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS);
-        bq[(c + 1) & 1][1] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS + kQ_CH);
-      }
       if (mma) {
         const halfx8 bh = __builtin_bit_cast(halfx8, bq[c & 1][0]);
         const halfx8 bl = __builtin_bit_cast(halfx8, bq[c & 1][1]);
@@ -1376,6 +1371,13 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
           D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, D[rb][c], 0, 0, 0);
           D[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, D[rb][c], 0, 0, 0);
         }
+      }
+      // block c + 1's B, read after this block's matrix instructions were issued: the
+      // recursion steps below cover its latency (LDS reads complete in order, so a read
+      // issued before them would hold up this block's instructions)
+      if (mma && c + 1 < NCB) {
+        bq[(c + 1) & 1][0] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS);
+        bq[(c + 1) & 1][1] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS + kQ_CH);
       }
       if (more && (c & 1)) rec8(b ^ 1, c >> 1, act_next);
       __builtin_amdgcn_sched_barrier(0);
