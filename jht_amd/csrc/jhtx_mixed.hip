@@ -2599,34 +2599,40 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) D[p][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (any) {
+      // 16 units u = (ring block rb, sign sg, 32-ring slice sl) of 6 matrix instructions
+      // each; unit u + 1's A operands are read after unit u's instructions are issued (LDS
+      // reads complete in order: read before them they would hold them up)
+      uint4 ah[2][2], al[2][2];  // [buffer][parity]
+      auto load_unit = [&](int u, int buf) {
+        const int rb = u >> 2, sg = (u >> 1) & 1, sl = u & 1;
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+        for (int p = 0; p < 2; ++p) {
+          const int o = a_off + 16 * p * kQ_RS + 32 * sl;
+          ah[buf][p] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][0][0][o]);
+          al[buf][p] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][1][0][o]);
+        }
+      };
+      load_unit(0, 0);
 #pragma unroll
-      for (int sg = 0; sg < 2; ++sg) {
-        if (!actv[2 * rb + sg]) continue;
-        uint4 ah[2][2], al[2][2];  // [parity][32-ring slice]
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int sl = 0; sl < 2; ++sl) {
-            const int o = a_off + 16 * p * kQ_RS + 32 * sl;
-            ah[p][sl] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][0][0][o]);
-            al[p][sl] = *reinterpret_cast<const uint4*>(&tp[b][rb][sg][1][0][o]);
-          }
-#pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
+      for (int u = 0; u < 16; ++u) {
+        const int rb = u >> 2, sg = (u >> 1) & 1, sl = u & 1;
+        if (actv[2 * rb + sg]) {
           const halfx8 bh = __builtin_bit_cast(halfx8, Bq[sg][2 * rb + sl][0]);
           const halfx8 bl = __builtin_bit_cast(halfx8, Bq[sg][2 * rb + sl][1]);
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
-            const halfx8 xh = __builtin_bit_cast(halfx8, ah[p][sl]);
-            const halfx8 xl = __builtin_bit_cast(halfx8, al[p][sl]);
+            const halfx8 xh = __builtin_bit_cast(halfx8, ah[u & 1][p]);
+            const halfx8 xl = __builtin_bit_cast(halfx8, al[u & 1][p]);
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, D[p][rb], 0, 0, 0);
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, D[p][rb], 0, 0, 0);
             D[p][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, D[p][rb], 0, 0, 0);
           }
         }
+        if (u + 1 < 16) load_unit(u + 1, (u + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
       }
+    }
     // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p
     if (bcol < batch && jf <= jt) {
       const int l0 = lmin + TL * jt;
