@@ -2,7 +2,8 @@
 library's static per-GPU scratch array -- against jhtx_adjoint with a caller workspace,
 through the C ABI.  Two calls in a row, the second with a smaller batch and new inputs, must
 give jhtx_adjoint's results bit for bit (the same kernels on the same inputs), on an FP64
-spin-2 plan and a mixed spin-2 plan (the 16-map FP16-split kernels, then a ragged group);
+spin-2 plan and mixed spin-2 plans (the 16-map FP16-split kernels, then a ragged group; the
+32-map kernel, whose first-stored-l table also lives in the scratch, then 7 maps);
 the existing entry point keeps working between them; a call above the scratch size is
 refused before any launch.
 """
@@ -42,7 +43,8 @@ def _adjoint_noalloc(lib, plan, maps, batch, stream):
     return out
 
 
-@pytest.mark.parametrize("precision,nside,lmax,batches", [("f64", 64, 128, (3, 1)), ("mixed", 128, 256, (16, 5))])
+@pytest.mark.parametrize("precision,nside,lmax,batches", [("f64", 64, 128, (3, 1)), ("mixed", 128, 256, (16, 5)),
+                                                      ("mixed", 64, 128, (32, 7))])
 def test_adjoint_noalloc_matches_workspace_entry(precision, nside, lmax, batches):
     from jht_amd import _lib
     from jht_amd import healpix as H
