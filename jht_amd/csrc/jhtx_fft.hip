@@ -84,6 +84,8 @@ struct FftArgs {
   const double2* chirps;
   const float2* chirpsf;
   long long chirp_shift;  // offset of the shifted chirp tables
+  const long long* qofs;  // per pair p: offset of its split-transform tables in qtab, -1 if none
+  const float2* qtab;
   int lds_bytes;
   int bluestein_all;
 };
@@ -809,6 +811,8 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   a.chirps = p->fft.chirps;
   a.chirpsf = p->fft.chirpsf;
   a.chirp_shift = p->fft.chirp_shift;
+  a.qofs = p->fft.qofs;
+  a.qtab = p->fft.qtab;
   a.lds_bytes = c.lds_bytes;
   a.bluestein_all = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
   return a;
@@ -876,6 +880,297 @@ void launch_spectrum(int lc, const FftArgs& a, dim3 grid, size_t sh) {
     case 1: hipLaunchKernelGGL((k_bluestein_spectrum<256, 16>), grid, dim3(256), sh, 0, a); break;
     case 2: hipLaunchKernelGGL((k_bluestein_spectrum<512, 16>), grid, dim3(512), sh, 0, a); break;
     default: hipLaunchKernelGGL((k_bluestein_spectrum<512, 32>), grid, dim3(512), sh, 0, a); break;
+  }
+}
+
+// ---------------------------------------------------------------------------------- //
+// Chirp-z transforms of the polar rings split four ways (mixed-precision spin-2 plans).  A
+// polar ring has N = 4 i pixels.  With bin k = 4 k' + r and pixel j = j1 + i j2 (j1 < i,
+// j2 < 4), e^{2 pi i j k / N} = e^{2 pi i j1 k' / i} e^{2 pi i j1 r / N} i^{j2 r}, so
+//   synthesis  x_{j1 + i j2} = sum_r i^{j2 r} u_{j1}^r Y_r[j1],
+//              Y_r[j1] = sum_k' D_{4 k' + r} e^{i pi 4 k' / N} e^{2 pi i j1 k' / i},
+//              u_j = e^{i pi (2 j + 1) / N}  (the ring's half-sample shift, 1/N of a turn per
+//              bin, split as e^{i pi 4 k' / N} before the sub-transform and u's 1 after it);
+//   adjoint    W'_{k1 + i k2} = e^{-i pi k2 / 4} sum_s (-i)^{k2 s} (w_{k1}^*)^s Z'_s[k1],
+//              Z'_s[k1] = e^{-i pi k1 / N} sum_j' x_{4 j' + s} e^{-2 pi i k1 j' / i},
+//              w_k = e^{2 pi i k / N},
+// with W'_k = W_k e^{-i pi k / N} the shifted spectrum.  The length-i sub-transforms are
+// chirp-z transforms of length Lq = 2048 or 4096 (the next power of two >= 2 i - 1) where
+// the ring's own would need 8192 or 16384: a quarter of the tables, a fifth fewer butterflies,
+// and 16 or 32 KB of LDS per work-group instead of 64 or 128 KB.  The sub-transforms run in
+// turn; each is folded into the four output quarters (synthesis) or output bins (adjoint)
+// as it completes, so no LDS holds the intermediate results.
+// Per ring pair p the tables at qtab + qofs[p] are: S (Lq, the kernel spectrum, FFT of
+// e^{-i pi n^2 / i}), then i entries each of
+//   T1_k = e^{i pi (4 k^2 + 4 k) / N}, T2_j = e^{i pi 4 j^2 / N} (= e^{i pi j^2 / i}),
+//   T3_j = e^{i pi (2 j + 1) / N}, T4_k = e^{i pi (4 k^2 + k) / N}, T5_k = e^{i pi 2 k / N},
+// every exponent reduced exactly mod 2 N in integers.
+// ---------------------------------------------------------------------------------- //
+constexpr int kQuadMinI = 513;  // polar rings with i >= this (N > 2048) run split
+
+__host__ __device__ inline int quad_len(int i) {  // Lq: power of two >= 2 i - 1
+  int l = 128;
+  while (l < 2 * i - 1) l <<= 1;
+  return l;
+}
+
+struct QuadTabs {
+  const float2 *S, *T1, *T2, *T3, *T4, *T5;
+};
+
+__device__ __forceinline__ QuadTabs quad_tabs(const FftArgs& A, int p, int i, int Lq) {
+  const float2* base = A.qtab + A.qofs[p];
+  QuadTabs t;
+  t.S = base;
+  t.T1 = base + Lq;
+  t.T2 = t.T1 + i;
+  t.T3 = t.T2 + i;
+  t.T4 = t.T3 + i;
+  t.T5 = t.T4 + i;
+  return t;
+}
+
+// e^{i pi (e mod 2N) / N} for an exponent given exactly as an integer
+__device__ __forceinline__ double2 cispi_int(long long e, int N) {
+  const long long r = e % (2LL * N);
+  return cispi((double)r / (double)N);
+}
+
+// z times i^n (n mod 4)
+__device__ __forceinline__ float2 rot_i(float2 z, int n) {
+  switch (n & 3) {
+    case 0: return z;
+    case 1: return make_float2(-z.y, z.x);
+    case 2: return make_float2(-z.x, -z.y);
+    default: return make_float2(z.y, -z.x);
+  }
+}
+
+// Plan-time tables of a split ring, in FP64 (rounded to FP32 afterwards)
+template <int BLOCK, int E>
+__global__ __launch_bounds__(BLOCK) void k_quad_tables(FftArgs A, double2* __restrict__ q64) {
+  extern __shared__ double lds[];
+  const int p = A.pairs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int N = ring_len(A.nside, p), i = N >> 2;
+  const int Lq = quad_len(i);
+  const int lgL = ilog2(Lq);
+  double2* S = q64 + A.qofs[p];
+  double2* T = S + Lq;
+  // the kernel spectrum: forward FFT of b_n = e^{-i pi n^2 / i} (n < i, mirrored)
+  const int nt = Lq / E;
+  double2 x[E];
+  if (tid < nt) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = tid + nt * e;
+      int n = -1;
+      if (k < i) n = k;
+      else if (k > Lq - i) n = Lq - k;
+      x[e] = (n >= 0) ? chirp(n, i, -1) : make_double2(0.0, 0.0);
+    }
+  }
+  fft_pow2<E, lg_lo(BLOCK, E), lg_hi(BLOCK, E), double>(x, lgL, tid, -1, A.tw, A.twL, lds, A.lds_bytes);
+  if (tid < nt) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) S[tid + nt * e] = x[e];
+  }
+  for (int k = tid; k < i; k += BLOCK) {
+    const long long kk = k;
+    T[k] = cispi_int(4 * kk * kk + 4 * kk, N);
+    T[i + k] = cispi_int(4 * kk * kk, N);
+    T[2 * i + k] = cispi_int(2 * kk + 1, N);
+    T[3 * i + k] = cispi_int(4 * kk * kk + kk, N);
+    T[4 * i + k] = cispi_int(2 * kk, N);
+  }
+}
+
+// folded spectrum D_k of one spin-2 ring with M <= 2 N (no half-sample phase), FP64 sums
+__device__ __forceinline__ double2 fold_bin2(const float2* __restrict__ Fp, const float2* __restrict__ Fm, int k,
+                                             int N, int M) {
+  // terms: Fp at m = k (+), m = k + N (-); conj(Fm) at m = N - k (-), m = 2 N - k (+)
+  const float2 p0 = Fp[min(k, M - 1)], p1 = Fp[min(k + N, M - 1)];
+  const float2 q0 = Fm[min(N - k, M - 1)], q1 = Fm[min(2 * N - k, M - 1)];
+  double2 acc = make_double2(0.0, 0.0);
+  if (k < M) {
+    acc.x += (double)p0.x;
+    acc.y += (double)p0.y;
+  }
+  if (k + N < M) {
+    acc.x -= (double)p1.x;
+    acc.y -= (double)p1.y;
+  }
+  if (N - k < M) {
+    acc.x -= (double)q0.x;
+    acc.y += (double)q0.y;
+  }
+  if (2 * N - k < M) {
+    acc.x += (double)q1.x;
+    acc.y -= (double)q1.y;
+  }
+  return acc;
+}
+
+// one length-Lq chirp-z convolution in registers: forward FFT, the kernel spectrum (conj for
+// the adjoint), inverse FFT; the 1/Lq scale is left to the caller
+template <int E, int LG>
+__device__ __forceinline__ void quad_convolve(const FftArgs& A, float2 (&x)[E], const float2* __restrict__ S,
+                                              bool adjoint, void* lds, int tid) {
+  constexpr int nt = (1 << LG) / E;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    int tp = tid;
+    asm volatile("" : "+v"(tp));
+    fft_pow2<E, LG, LG, float>(x, LG, tp, pass ? +1 : -1, A.twf, A.twL, lds, A.lds_bytes);
+    if (pass == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 s = S[tp + nt * e];
+        x[e] = cmul(x[e], adjoint ? cconj(s) : s);
+        if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bound in-flight loads
+      }
+    }
+  }
+}
+
+// Synthesis of one polar ring (map b, half 0 north / 1 south), split four ways
+// 4 waves per SIMD (at most 128 VGPRs), 1024 / BLOCK work-groups per CU: the phases of one
+// ring's work-group overlap another's
+template <int BLOCK, int E, int LG>
+__global__ __launch_bounds__(BLOCK, 4) void k_ring_synth_q(FftArgs A, const float2* __restrict__ F,
+                                                        double* __restrict__ map) {
+  extern __shared__ double lds[];
+  constexpr int L = 1 << LG, nt = L / E, NY = E / 2;
+  static_assert(nt == BLOCK, "one element column per thread");
+  const int slot = blockIdx.x;
+  const int p = A.pairs[slot >> 1], half = slot & 1, b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int N = ring_len(A.nside, p), i = N >> 2;
+  const QuadTabs tb = quad_tabs(A, p, i, L);
+  const int r = half ? 4 * A.nside - 2 - p : p;
+  const long long ringM = A.M;
+  const float2* Fp = F + ((long long)b * 2 * A.nrings + r) * ringM;
+  const float2* Fm = Fp + (long long)A.nrings * ringM;
+  const float invL = 1.f / (float)L;
+  float2 X[4][NY], up[NY];  // output quarters j2 and u_j1^r (j1 = tid + nt e)
+#pragma unroll
+  for (int e = 0; e < NY; ++e) {
+    up[e] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X[q][e] = make_float2(0.f, 0.f);
+  }
+#pragma unroll 1
+  for (int r4 = 0; r4 < 4; ++r4) {
+    float2 x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = tid + nt * e;
+      if (k < i) {
+        const double2 d = fold_bin2(Fp, Fm, 4 * k + r4, N, A.M);
+        x[e] = cmul(make_float2((float)d.x, (float)d.y), tb.T1[k]);
+      } else {
+        x[e] = make_float2(0.f, 0.f);
+      }
+      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    quad_convolve<E, LG>(A, x, tb.S, false, lds, tid);
+#pragma unroll
+    for (int e = 0; e < NY; ++e) {
+      const int j1 = tid + nt * e;
+      if (j1 < i) {
+        const float2 y = cmul(cmul(cscale(x[e], invL), tb.T2[j1]), up[e]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) X[q][e] = cadd(X[q][e], rot_i(y, q * r4));
+        up[e] = cmul(up[e], tb.T3[j1]);
+      }
+    }
+  }
+  double* Q = map + (long long)b * 2 * A.npix;
+  double* U = Q + A.npix;
+  const long long st = ring_start(A.nside, r);
+#pragma unroll
+  for (int e = 0; e < NY; ++e) {
+    const int j1 = tid + nt * e;
+    if (j1 < i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __builtin_nontemporal_store((double)X[q][e].x, &Q[st + j1 + (long long)i * q]);
+        __builtin_nontemporal_store((double)X[q][e].y, &U[st + j1 + (long long)i * q]);
+      }
+    }
+  }
+}
+
+// Adjoint of one polar ring, split four ways; writes V+ and V- of every order m directly
+template <int BLOCK, int E, int LG>
+__global__ __launch_bounds__(BLOCK, 4) void k_ring_adjoint_q(FftArgs A, const double* __restrict__ map,
+                                                          float2* __restrict__ V) {
+  extern __shared__ double lds[];
+  constexpr int L = 1 << LG, nt = L / E, NY = E / 2;
+  static_assert(nt == BLOCK, "one element column per thread");
+  const int slot = blockIdx.x;
+  const int p = A.pairs[slot >> 1], half = slot & 1, b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int N = ring_len(A.nside, p), i = N >> 2;
+  const QuadTabs tb = quad_tabs(A, p, i, L);
+  const int r = half ? 4 * A.nside - 2 - p : p;
+  const double* Q = map + (long long)b * 2 * A.npix;
+  const double* U = Q + A.npix;
+  const long long st = ring_start(A.nside, r);
+  const float invL = 1.f / (float)L;
+  float2 W[4][NY], up[NY];  // output bins k1 + i k2 and (w_k1^*)^s (k1 = tid + nt e)
+#pragma unroll
+  for (int e = 0; e < NY; ++e) {
+    up[e] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) W[q][e] = make_float2(0.f, 0.f);
+  }
+#pragma unroll 1
+  for (int s4 = 0; s4 < 4; ++s4) {
+    float2 x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int jp = tid + nt * e;
+      // clamped: validity after.  Cached loads: the four sub-transforms read every 4th sample
+      // of the same lines (a streaming load would fetch each line from HBM four times)
+      const long long jc = st + 4 * (long long)min(jp, i - 1) + s4;
+      const double vq = Q[jc];
+      const double vu = U[jc];
+      x[e] = (jp < i) ? cmul(make_float2((float)vq, (float)vu), cconj(tb.T2[jp])) : make_float2(0.f, 0.f);
+      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    quad_convolve<E, LG>(A, x, tb.S, true, lds, tid);
+#pragma unroll
+    for (int e = 0; e < NY; ++e) {
+      const int k1 = tid + nt * e;
+      if (k1 < i) {
+        const float2 z = cmul(cmul(cscale(x[e], invL), cconj(tb.T4[k1])), up[e]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) W[q][e] = cadd(W[q][e], rot_i(z, -q * s4));
+        up[e] = cmul(up[e], cconj(tb.T5[k1]));
+      }
+    }
+  }
+  // the quarter phases e^{-i pi k2 / 4}, then V+ at m = k + q N and V- at m = (N - k) mod N
+  // + q N (the conjugated mirror bin: -1 for k > 0, as the shift is carried by the bins)
+  const float c4 = 0.70710678118654752440f;
+  const float2 ph[4] = {make_float2(1.f, 0.f), make_float2(c4, -c4), make_float2(0.f, -1.f), make_float2(-c4, -c4)};
+  const long long ringM = A.M;
+  float2* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
+  float2* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
+#pragma unroll
+  for (int e = 0; e < NY; ++e) {
+    const int k1 = tid + nt * e;
+    if (k1 < i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k1 + i * q;
+        const float2 wk = cmul(W[q][e], ph[q]);
+        for (int m = k, qq = 0; m < A.M; m += N, ++qq) Vp[m] = (qq & 1) ? make_float2(-wk.x, -wk.y) : wk;
+        const float2 wr = (k != 0) ? make_float2(-wk.x, wk.y) : cconj(wk);  // -conj(wk) for k > 0
+        for (int m = (N - k) % N, qq = 0; m < A.M; m += N, ++qq) Vm[m] = (qq & 1) ? make_float2(-wr.x, -wr.y) : wr;
+      }
+    }
   }
 }
 
@@ -961,8 +1256,11 @@ int build_fft_tables(jhtx_plan* p) {
   std::vector<int> cls_pairs[kNcls];
   int need[kNcls] = {0}, need64[kNcls] = {0};
   int twL = 16;
-  std::vector<long long> bofs(npairs, -1), cofs(npairs, -1);
-  long long blen = 0, clen = 0;
+  std::vector<long long> bofs(npairs, -1), cofs(npairs, -1), qofs(npairs, -1);
+  std::vector<int> qpairs[2];
+  long long blen = 0, clen = 0, qlen = 0;
+  // the polar chirp-z rings of mixed-precision spin-2 plans with i >= kQuadMinI run split
+  const bool quad = f32 && p->spin == 2 && !ball;
   for (int q = 0; q < npairs; ++q) {
     const int N = ring_len(nside, q);
     int L;
@@ -972,6 +1270,14 @@ int build_fft_tables(jhtx_plan* p) {
       set_error("ring transform length " + std::to_string(L) + " (nside=" + std::to_string(nside) +
                 ") exceeds the single-work-group ring FFT (nside <= 2048)");
       return JHTX_EUNSUPPORTED;
+    }
+    if (kind >= 1) twL = std::max(twL, L);
+    if (quad && kind == 2 && q + 1 < nside && q + 1 >= kQuadMinI) {
+      const int i = q + 1, Lq = quad_len(i);
+      qpairs[Lq == 2048 ? 0 : 1].push_back(q);
+      qofs[q] = qlen;
+      qlen += Lq + 5LL * i;
+      continue;
     }
     const int c = 3 * lc + kind;
     cls_pairs[c].push_back(q);
@@ -983,7 +1289,6 @@ int build_fft_tables(jhtx_plan* p) {
     nd = std::max(nd, N * cs);
     need[c] = std::max(need[c], nd);
     need64[c] = std::max(need64[c], (kind == 0) ? 3 * N * 16 : L * 16);
-    if (kind >= 1) twL = std::max(twL, L);
     if (kind == 1 && ring_shifted(nside, q)) twL = std::max(twL, 2 * N);  // bin phases from the table
     if (kind == 2) {
       bofs[q] = blen;
@@ -1079,6 +1384,49 @@ int build_fft_tables(jhtx_plan* p) {
       return JHTX_EDEVICE;
     }
   }
+  // split polar rings: launch classes (Lq = 2048: 256 threads, 4096: 512 threads, 8 elements
+  // each) and their tables, computed in FP64 and rounded once
+  e = hipMalloc(&p->fft.qofs, sizeof(long long) * npairs);
+  if (e != hipSuccess) return JHTX_ENOMEM;
+  (void)hipMemcpy(p->fft.qofs, qofs.data(), sizeof(long long) * npairs, hipMemcpyHostToDevice);
+  for (int c = 0; c < 2; ++c) {
+    FftClass& fc = p->fft.qcls[c];
+    fc.lc = 2 + c;
+    fc.kind = 3;
+    fc.block = c ? 512 : 256;
+    fc.npairs = (int)qpairs[c].size();
+    fc.lds_bytes = (c ? 4096 : 2048) * (int)sizeof(float2);
+    fc.pairs = nullptr;
+    if (fc.npairs) {
+      e = hipMalloc(&fc.pairs, sizeof(int) * fc.npairs);
+      if (e != hipSuccess) return JHTX_ENOMEM;
+      (void)hipMemcpy(fc.pairs, qpairs[c].data(), sizeof(int) * fc.npairs, hipMemcpyHostToDevice);
+    }
+  }
+  if (qlen > 0) {
+    double2* q64 = nullptr;
+    e = hipMalloc(&q64, sizeof(double2) * qlen);
+    if (e != hipSuccess) return JHTX_ENOMEM;
+    e = hipMalloc(&p->fft.qtab, sizeof(float2) * qlen);
+    if (e != hipSuccess) {
+      (void)hipFree(q64);
+      return JHTX_ENOMEM;
+    }
+    for (int c = 0; c < 2; ++c) {
+      const FftClass& fc = p->fft.qcls[c];
+      if (!fc.npairs) continue;
+      FftArgs a = make_fft_args(p, fc);
+      a.lds_bytes = 4096 * (int)sizeof(double2);  // the FP64 spectrum FFT, Lq <= 4096
+      hipLaunchKernelGGL((k_quad_tables<256, 16>), dim3(fc.npairs), dim3(256), a.lds_bytes, 0, a, q64);
+    }
+    hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, 0, (const double2*)q64, p->fft.qtab, qlen);
+    e = hipDeviceSynchronize();
+    (void)hipFree(q64);
+    if (e != hipSuccess) {
+      set_error(std::string("split ring table kernel failed: ") + hipGetErrorString(e));
+      return JHTX_EDEVICE;
+    }
+  }
   return JHTX_OK;
 }
 
@@ -1093,6 +1441,10 @@ void free_fft_tables(jhtx_plan* p) {
   if (p->fft.chirpsf) (void)hipFree(p->fft.chirpsf);
   for (int c = 0; c < 12; ++c)
     if (p->fft.cls[c].pairs) (void)hipFree(p->fft.cls[c].pairs);
+  for (int c = 0; c < 2; ++c)
+    if (p->fft.qcls[c].pairs) (void)hipFree(p->fft.qcls[c].pairs);
+  if (p->fft.qofs) (void)hipFree(p->fft.qofs);
+  if (p->fft.qtab) (void)hipFree(p->fft.qtab);
   p->fft = FftTables();
 }
 
@@ -1115,6 +1467,26 @@ static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, vo
         launch_class<2, float>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
       else
         launch_class<2, double>(synth, fc.lc, fc.kind, a, grid, sh, s, in, out);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int c = 0; c < 2; ++c) {  // split polar rings (mixed spin-2 plans)
+    const FftClass& fc = p->fft.qcls[c];
+    if (!fc.npairs) continue;
+    const FftArgs a = make_fft_args(p, fc);
+    const dim3 grid(2 * fc.npairs, batch);
+    const size_t sh = fc.lds_bytes;
+    if (synth) {
+      if (c)
+        hipLaunchKernelGGL((k_ring_synth_q<512, 8, 12>), grid, dim3(512), sh, s, a, (const float2*)in, (double*)out);
+      else
+        hipLaunchKernelGGL((k_ring_synth_q<256, 8, 11>), grid, dim3(256), sh, s, a, (const float2*)in, (double*)out);
+    } else {
+      if (c)
+        hipLaunchKernelGGL((k_ring_adjoint_q<512, 8, 12>), grid, dim3(512), sh, s, a, (const double*)in, (float2*)out);
+      else
+        hipLaunchKernelGGL((k_ring_adjoint_q<256, 8, 11>), grid, dim3(256), sh, s, a, (const double*)in, (float2*)out);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -90,6 +90,12 @@ struct FftTables {
   // same tables with the half-sample ring shift folded in, e^{i pi (n^2 + n) / N}
   double2* chirps = nullptr;
   long long chirp_shift = 0;
+  // polar rings of mixed-precision spin-2 plans transformed as four length-N/4 chirp-z
+  // transforms: launch classes for Lq = 2048 and 4096, per ring pair the offset of its tables
+  // in qtab (-1 if it is not split)
+  FftClass qcls[2];
+  long long* qofs = nullptr;
+  float2* qtab = nullptr;
 };
 
 }  // namespace jhtx
