@@ -554,3 +554,29 @@ def test_f64_adjoint_ring_shapes_agree(nside, lmax):
     assert e <= TOL
     if nside <= 64:
         assert _rel(outs[4], O.adjoint_synthesis(v, nside, lmax, 2)) <= TOL
+
+
+@pytest.mark.parametrize("nside,lmax", [(600, 700), (1100, 1200)])
+def test_mixed_split_polar_rings(nside, lmax):
+    """Mixed spin-2 plans transform the polar rings with i >= 513 (N = 4 i > 2048) as four
+    length-i chirp-z transforms (Lq = 2048 up to i = 1024, 4096 above) with a radix-4 combine;
+    the FP64 path keeps one chirp-z transform per ring.  nside 600 is not a power of two (its
+    belt rings are chirp-z rings as well, not split) and splits only in the 2048 class; nside
+    1100 splits in both.  Synthesis and adjoint against FP64, every map <= 1e-5."""
+    j = _jht()
+
+    rng = np.random.default_rng(6000 + nside)
+    batch = 3
+    a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)]) for _ in range(batch)])
+    v = rng.standard_normal((batch, 2, 12 * nside * nside))
+    at, vt = torch.tensor(a, device="cuda"), torch.tensor(v, device="cuda")
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        s64 = j.synthesis_batch(at, nside, lmax, 2, precision="f64").cpu().numpy()
+        smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed").cpu().numpy()
+        b64 = j.adjoint_synthesis_batch(vt, nside, lmax, 2, precision="f64").cpu().numpy()
+        bmx = j.adjoint_synthesis_batch(vt, nside, lmax, 2, precision="mixed").cpu().numpy()
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"split polar rings nside={nside} lmax={lmax}: synth {es:.2e} adjoint {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
