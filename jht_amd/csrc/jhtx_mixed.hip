@@ -727,6 +727,16 @@ __device__ __forceinline__ uint32_t split_f16_rn(float v) {
   return __builtin_bit_cast(uint32_t, halfx2{hi, (_Float16)lo});
 }
 
+// split_f16_rn's pair in two instructions: the conversion writes hi to the lower half, and a
+// mixed-precision FMA, v * 1 - hi (exact in FP32, one rounding to FP16), writes lo to the
+// upper half.  Written out because the compiler converts hi back to FP32, subtracts and
+// converts again (three more instructions per value).
+__device__ __forceinline__ uint32_t split_f16_mix(float v) {
+  uint32_t w = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+  asm("v_fma_mixhi_f16 %0, %1, 1.0, -%0 op_sel_hi:[0,0,1]" : "+v"(w) : "v"(v));
+  return w;
+}
+
 // (hi, hi) and (lo, lo) from a split_f16 pair
 __device__ __forceinline__ uint32_t dup_lo(uint32_t w) { return (w & 0xffffu) | (w << 16); }
 __device__ __forceinline__ uint32_t dup_hi(uint32_t w) { return (w >> 16) | (w & 0xffff0000u); }
@@ -1190,6 +1200,32 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
 constexpr int kQ_CH = 32;  // l steps per chunk: one matrix K block
 constexpr int kQ_PS = 72;  // plane row stride (halves): hi 0..31, lo 32..63, pad; 144 bytes
 
+// f(integral_constant<int, I>) for I = B .. E - 1, unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// alpha_j x + gamma_j of recursion step j (0..31) of a 32-step chunk.  Lane i of every 16-lane
+// row holds (alpha, gamma) of steps i (cf[0]) and 16 + i (cf[1]), so each coefficient is a
+// row broadcast from lane j % 16 (row_newbcast, a 64-bit DPP source operand): one move and one
+// FMA with no LDS read per step (a per-step LDS read of the coefficients, shared by the whole
+// wave, queued behind the operand reads and held up the recursion).  The sources come from an
+// LDS read, never from a vector instruction just before (the DPP read hazard), and the
+// product is the same single-rounding FMA as fma(alpha, x, gamma).
+template <int J>
+__device__ __forceinline__ double ag_step(const double2 (&cf)[2], double x) {
+  double t;
+  asm("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf" : "=v"(t) : "v"(cf[J >> 4].y), "i"(J & 15));
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(t)
+      : "v"(cf[J >> 4].x), "v"(x), "i"(J & 15));
+  return t;
+}
+
 // (hi, hi) and (lo, lo) halves of two FP32 values: hi by mantissa mask (truncation)
 __device__ __forceinline__ void split2_f16(float a, float b, uint32_t& hi, uint32_t& lo) {
   const float ha = __uint_as_float(__float_as_uint(a) & 0xffffe000u);
@@ -1210,7 +1246,7 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
                                                      float2* __restrict__ F, int batch) {
   constexpr int NCB = 8, NW = 8, MG = 32, NCOL = 4 * MG, NT = 64 * NW, RINGS = 32 * NW;
   struct Smem {
-    double2 ag[2][kQ_CH][2];                       // [buffer][step][sign]: (alpha, +-gamma)
+    double2 ag[2][2][kQ_CH];                       // [buffer][sign][step]: (alpha, +-gamma)
     float sc[MG];
     __align__(16) uint16_t bp[2][2][NCOL][kQ_PS];  // [buffer][sign][column][hi | lo]
     __align__(16) uint16_t tp[NW][64][kQ_PS];      // [wave][ring][hi | lo]
@@ -1263,8 +1299,14 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     return (l <= lmax && l != lmin) ? make_double2(A.alpha[cb0 + l], A.gamma[cb0 + l]) : make_double2(0.0, 0.0);
   };
   auto stage_ag = [&](int buf, double2 v) {
-    sm.ag[buf][tid][0] = v;
-    sm.ag[buf][tid][1] = make_double2(v.x, -v.y);
+    sm.ag[buf][0][tid] = v;
+    sm.ag[buf][1][tid] = make_double2(v.x, -v.y);
+  };
+  // this lane's coefficients of the chunk in buffer buf (steps lane % 16 and 16 + lane % 16)
+  double2 cf[2];
+  auto load_cf = [&](int buf) {
+    cf[0] = sm.ag[buf][sg][lane & 15];
+    cf[1] = sm.ag[buf][sg][16 + (lane & 15)];
   };
   auto store_b = [&](int l0c, int buf) {
     const double gsc = (double)sm.sc[sgm];
@@ -1291,24 +1333,24 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     }
   };
   uint16_t* const trow = &sm.tp[wave][lane][0];  // this lane's ring row
-  // 8 recursion steps (group q of the chunk in buffer abuf) into the tile; groups where no lane
-  // of the wave is active (every value below 2^-256) store zeros and only advance the recursion
-  auto rec8 = [&](int abuf, int q, bool& act) {
+  // 8 recursion steps (group q of the chunk whose coefficients are in cf) into the tile; groups
+  // where no lane of the wave is active (every value below 2^-256) store zeros and only advance
+  // the recursion
+  auto rec8 = [&](auto qc, bool& act) {
+    constexpr int q = decltype(qc)::value;
     const bool on = __ballot(valid && ex == 0) != 0ull;
     const bool deep = __ballot(valid && ex < 0) != 0ull;  // exponents never decrease
     act |= on;
     const double f = (ex == 0) ? sA : 0.0;
     uint32_t hw[4], lw[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const double2 a0 = sm.ag[abuf][8 * q + 2 * p][sg];
-      const double2 a1 = sm.ag[abuf][8 * q + 2 * p + 1][sg];
-      const double n0 = fma(fma(a0.x, x, a0.y), mu, -mup);
-      const double n1 = fma(fma(a1.x, x, a1.y), n0, -mu);
+    static_for<0, 4>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      const double n0 = fma(ag_step<8 * q + 2 * p>(cf, x), mu, -mup);
+      const double n1 = fma(ag_step<8 * q + 2 * p + 1>(cf, x), n0, -mu);
       mup = n0;
       mu = n1;
       split2_f16((float)(n0 * f), (float)(n1 * f), hw[p], lw[p]);
-    }
+    });
     *reinterpret_cast<uint4*>(trow + 8 * q) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
     *reinterpret_cast<uint4*>(trow + kQ_CH + 8 * q) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
     if (deep) rescale_pair(mu, mup, ex);
@@ -1326,8 +1368,8 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     store_b(lmin, 0);
     if (nch > 1) prefetch(lmin + kQ_CH);
     __syncthreads();
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) rec8(0, q, act_cur);
+    load_cf(0);
+    static_for<0, 4>([&](auto qc) { rec8(qc, act_cur); });
     wave_sync();
   }
   const uint16_t* const arow = &sm.tp[wave][ii][8 * kk];  // A operands: ring 16 rb + ii, steps 8 kk..
@@ -1347,6 +1389,7 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
       ah[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS);
       al[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS + kQ_CH);
     }
+    if (more) load_cf(b ^ 1);  // chunk jc + 1's coefficients
     const uint16_t* brow = &sm.bp[b][sg][ii][8 * kk];
     bool act_next = false;
     const bool mma = act_cur;
@@ -1358,8 +1401,8 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
       bq[0][0] = *reinterpret_cast<const uint4*>(brow);
       bq[0][1] = *reinterpret_cast<const uint4*>(brow + kQ_CH);
     }
-#pragma unroll
-    for (int c = 0; c < NCB; ++c) {
+    static_for<0, NCB>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
       if (mma) {
         const halfx8 bh = __builtin_bit_cast(halfx8, bq[c & 1][0]);
         const halfx8 bl = __builtin_bit_cast(halfx8, bq[c & 1][1]);
@@ -1379,9 +1422,11 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
         bq[(c + 1) & 1][0] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS);
         bq[(c + 1) & 1][1] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS + kQ_CH);
       }
-      if (more && (c & 1)) rec8(b ^ 1, c >> 1, act_next);
+      if constexpr ((c & 1) != 0) {
+        if (more) rec8(std::integral_constant<int, (c >> 1)>{}, act_next);
+      }
       __builtin_amdgcn_sched_barrier(0);
-    }
+    });
     wave_sync();
     act_cur = act_next;
   }
@@ -2437,9 +2482,9 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
 // and lo halves of the recursion values in separate planes ([row][ring]), and each lane's
 // B operands are the hi and lo halves of its V values packed ring-pairwise, built once per
 // work-group, so the split products come from one instruction each with no per-step operand
-// rebuilding: hi.hi, hi.lo and lo.hi over 32 rings each.  lo.lo is left out: the halves are
-// below 2^-10 and 2^-11 of their values, so the product is below 2^-21 of a.b, and with V's
-// hi rounded to nearest its sign is random (a quarter fewer matrix instructions).  The tiles
+// rebuilding: hi.hi, hi.lo and lo.hi over 32 rings each.  lo.lo is left out: with both hi
+// halves rounded to nearest the lo halves are below 2^-11 of their values, so the product is
+// below 2^-22 of a.b, and its sign is random (a quarter fewer matrix instructions).  The tiles
 // are double-buffered (one barrier per 32 l between the recursion of tile j + 1 and the
 // contraction of tile j), and the two waves of each SIMD (w and w + 4) run the two halves in
 // opposite order: one issues FP64 recursion while the other issues matrix instructions.
@@ -2539,7 +2584,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
       for (int u = 0; u < 8; ++u) {
         uint32_t w[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = split_f16_rn(bf[s][8 * u + k] * sB);
+        for (int k = 0; k < 8; ++k) w[k] = split_f16_mix(bf[s][8 * u + k] * sB);
         // hi halves of rings 2i, 2i + 1 -> dword i; lo halves likewise
         Bq[s][u][0] = make_uint4(__builtin_amdgcn_perm(w[1], w[0], 0x05040100u), __builtin_amdgcn_perm(w[3], w[2], 0x05040100u),
                                  __builtin_amdgcn_perm(w[5], w[4], 0x05040100u), __builtin_amdgcn_perm(w[7], w[6], 0x05040100u));
@@ -2553,34 +2598,39 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   int jf = ntile;  // first active tile (work-group uniform)
   // 32 recursion steps of this wave's sign into its planes of buffer b; odd rows of the
   // sign - tile negated (one B serves both parities, as in k_adj_mx_s2_cbh)
+  // this lane's coefficients of the tile in buffer b (steps lane % 16 and 16 + lane % 16),
+  // broadcast per step by ag_step
+  double2 cf[2];
+  auto load_cf = [&](int b) {
+    cf[0] = s_ag[b][sgw][lane & 15];
+    cf[1] = s_ag[b][sgw][16 + (lane & 15)];
+  };
+  static_assert(TL == 32, "ag_step covers 32 steps");
   auto rec_tile = [&](int b) {
     uint16_t* ph = &tp[b][rbw][sgw][0][0][lane];
     uint16_t* pl = &tp[b][rbw][sgw][1][0][lane];
     int act = 0;
-#pragma unroll 1
-    for (int j0 = 0; j0 < TL; j0 += 8) {
+    static_for<0, TL / 8>([&](auto gc) {
+      constexpr int j0 = 8 * decltype(gc)::value;
       const int anydeep = __ballot(valid && ex < 0) != 0ull;
       act |= __ballot(valid && ex == 0) != 0ull;
       const double f = (ex == 0 && valid) ? sA : 0.0;
       const double fo = sgw ? -f : f;  // odd steps
-      double2 ag[8];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) ag[jj] = s_ag[b][sgw][j0 + jj];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-        const double n = fma(fma(ag[jj].x, x, ag[jj].y), mu, -mup);
+      static_for<0, 8>([&](auto jc) {
+        constexpr int jj = decltype(jc)::value;
+        const double n = fma(ag_step<j0 + jj>(cf, x), mu, -mup);
         mup = mu;
         mu = n;
-        uint32_t w = split_f16((float)(n * ((jj & 1) ? fo : f)));
+        uint32_t w = split_f16_mix((float)(n * ((jj & 1) ? fo : f)));
         // one packed conversion, then its two halves stored (ds_write_b16 / _d16_hi): left
         // visible, the compiler converts each half separately
         asm volatile("" : "+v"(w));
         const int row = s2_tile_row(j0 + jj) * kQ_RS;
         ph[row] = (uint16_t)w;
         pl[row] = (uint16_t)(w >> 16);
-      }
+      });
       if (anydeep) rescale_pair(mu, mup, ex);
-    }
+    });
     if (lane == 0) s_act[b][wave] = act;
   };
   // A operands: lane (row i = lane % 16, K block k4 = lane / 16) reads 8 rings of a plane row
@@ -2664,6 +2714,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   }
   double2 pf = (tid < TL) ? ag_at(2, tid) : make_double2(0.0, 0.0);
   __syncthreads();
+  load_cf(0);
   rec_tile(0);
   __syncthreads();
   const bool rec_first = wave < 4;  // waves w and w + 4 share a SIMD
@@ -2675,6 +2726,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
       pf = ag_at(jt + 3, tid);
     }
     const bool more = jt + 1 < ntile;
+    if (more) load_cf(b ^ 1);  // tile jt + 1's coefficients
     if (rec_first) {
       if (more) rec_tile(b ^ 1);
       contract_tile(jt, b);
