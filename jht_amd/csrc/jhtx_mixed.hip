@@ -1246,7 +1246,6 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
                                                      float2* __restrict__ F, int batch) {
   constexpr int NCB = 8, NW = 8, MG = 32, NCOL = 4 * MG, NT = 64 * NW, RINGS = 32 * NW;
   struct Smem {
-    double2 ag[2][2][kQ_CH];                       // [buffer][sign][step]: (alpha, +-gamma)
     float sc[MG];
     __align__(16) uint16_t bp[2][2][NCOL][kQ_PS];  // [buffer][sign][column][hi | lo]
     __align__(16) uint16_t tp[NW][64][kQ_PS];      // [wave][ring][hi | lo]
@@ -1283,37 +1282,42 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
   const int sjp = tid & 15, sgm = tid >> 4;
   double2 pf_e[2], pf_b[2];
   double pf_g[2];
+  bool pf_ok[2];
+  // loads at clamped (l, map), with no branch around them (a branch leaves the compiler's
+  // wait counting unsure which loads are in flight, and it then waits for all of them); the
+  // coefficients past lmax or the batch are zeroed where they are used
   auto prefetch = [&](int l0n) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int l = l0n + 2 * sjp + e, b = mg * MG + sgm;
-      const bool ok = l <= lmax && b < batch;
-      pf_g[e] = ok ? A.G[cb0 + l] : 0.0;
-      pf_e[e] = ok ? alm[((long long)b * 2 + 0) * A.K + cb0 + l] : make_double2(0.0, 0.0);
-      pf_b[e] = ok ? alm[((long long)b * 2 + 1) * A.K + cb0 + l] : make_double2(0.0, 0.0);
+      pf_ok[e] = l <= lmax && b < batch;
+      const long long lc = cb0 + min(l, lmax), bc = min(b, batch - 1);
+      pf_g[e] = A.G[lc];
+      pf_e[e] = alm[(bc * 2 + 0) * A.K + lc];
+      pf_b[e] = alm[(bc * 2 + 1) * A.K + lc];
     }
   };
-  // (alpha, gamma) of step j of the chunk at l0c (zero past lmax, and at lmin: the seed step)
-  auto ag_at = [&](int l0c, int j) {
-    const int l = l0c + j;
-    return (l <= lmax && l != lmin) ? make_double2(A.alpha[cb0 + l], A.gamma[cb0 + l]) : make_double2(0.0, 0.0);
-  };
-  auto stage_ag = [&](int buf, double2 v) {
-    sm.ag[buf][0][tid] = v;
-    sm.ag[buf][1][tid] = make_double2(v.x, -v.y);
-  };
-  // this lane's coefficients of the chunk in buffer buf (steps lane % 16 and 16 + lane % 16)
+  // this lane's (alpha, gamma) of steps lane % 16 (cf[0]) and 16 + lane % 16 (cf[1]) of a
+  // 32-step chunk, straight from the tables, half a chunk or more before their use (no
+  // branch around the loads); prep_cf then zeroes the steps past lmax and the seed step at
+  // lmin (whose zero coefficients turn the seed into the first value), and negates gamma for
+  // the sign - recursion
   double2 cf[2];
-  auto load_cf = [&](int buf) {
-    cf[0] = sm.ag[buf][sg][lane & 15];
-    cf[1] = sm.ag[buf][sg][16 + (lane & 15)];
+  auto load_cf = [&](int h, int l0c) {
+    const long long lc = cb0 + min(l0c + 16 * h + (lane & 15), lmax);
+    cf[h] = make_double2(A.alpha[lc], A.gamma[lc]);
+  };
+  auto prep_cf = [&](int h, int l0c) {
+    const int l = l0c + 16 * h + (lane & 15);
+    const bool ok = l <= lmax && l != lmin;
+    cf[h] = ok ? make_double2(cf[h].x, sg ? -cf[h].y : cf[h].y) : make_double2(0.0, 0.0);
   };
   auto store_b = [&](int l0c, int buf) {
     const double gsc = (double)sm.sc[sgm];
     float vp[2][4], vm[2][4];  // [step][column]: sign + and sign - B values
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      const double gs = pf_g[e] * gsc;
+      const double gs = pf_ok[e] ? pf_g[e] * gsc : 0.0;
       const double er = pf_e[e].x * gs, ei = pf_e[e].y * gs, br = pf_b[e].x * gs, bi = pf_b[e].y * gs;
       // bp = -(E + iB), bm = -(E - iB); s = (-1)^(l+m) on the folded copies
       const float s = (((l0c + 2 * sjp + e + m) & 1) != 0) ? -1.f : 1.f;
@@ -1360,16 +1364,17 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
   if (nch > 0) {
     // chunk 0's B and (alpha, gamma) and chunk 1's (alpha, gamma); then chunk 1's a_lm in flight
     prefetch(lmin);
-    if (tid < kQ_CH) {
-      stage_ag(0, ag_at(lmin, tid));
-      stage_ag(1, ag_at(lmin + kQ_CH, tid));
-    }
+    load_cf(0, lmin);
+    load_cf(1, lmin);
     __syncthreads();  // sm.sc
     store_b(lmin, 0);
-    if (nch > 1) prefetch(lmin + kQ_CH);
+    prefetch(lmin + kQ_CH);
     __syncthreads();
-    load_cf(0);
+    prep_cf(0, lmin);
+    prep_cf(1, lmin);
     static_for<0, 4>([&](auto qc) { rec8(qc, act_cur); });
+    load_cf(0, lmin + kQ_CH);  // chunk 1's, recursed in the first loop iteration
+    load_cf(1, lmin + kQ_CH);
     wave_sync();
   }
   const uint16_t* const arow = &sm.tp[wave][ii][8 * kk];  // A operands: ring 16 rb + ii, steps 8 kk..
@@ -1378,10 +1383,9 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
     const int b = jc & 1;
     const int l0 = lmin + kQ_CH * jc;
     const bool more = jc + 1 < nch;
-    __syncthreads();  // chunk jc's B and chunk jc + 1's (alpha, gamma) visible; buffer b ^ 1 free
+    __syncthreads();  // chunk jc's B visible; buffer b ^ 1 free
     if (more) store_b(l0 + kQ_CH, b ^ 1);
-    if (jc + 2 < nch) prefetch(l0 + 2 * kQ_CH);
-    if (tid < kQ_CH) stage_ag(b, ag_at(l0 + 2 * kQ_CH, tid));  // chunk jc + 2 (read after the next barrier)
+    prefetch(l0 + 2 * kQ_CH);  // chunk jc + 2's a_lm (clamped loads past lmax)
     // this chunk's A operands, read before the recursion of chunk jc + 1 overwrites the tile
     uint4 ah[4], al[4];
 #pragma unroll
@@ -1389,7 +1393,6 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
       ah[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS);
       al[rb] = *reinterpret_cast<const uint4*>(arow + 16 * rb * kQ_PS + kQ_CH);
     }
-    if (more) load_cf(b ^ 1);  // chunk jc + 1's coefficients
     const uint16_t* brow = &sm.bp[b][sg][ii][8 * kk];
     bool act_next = false;
     const bool mma = act_cur;
@@ -1423,7 +1426,11 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
         bq[(c + 1) & 1][1] = *reinterpret_cast<const uint4*>(brow + 16 * (c + 1) * kQ_PS + kQ_CH);
       }
       if constexpr ((c & 1) != 0) {
+        // chunk jc + 1's coefficients: each half made ready before its first group; reloaded
+        // with chunk jc + 2's once its last group has run
+        if constexpr (c == 1 || c == 5) prep_cf(c >> 2, l0 + kQ_CH);
         if (more) rec8(std::integral_constant<int, (c >> 1)>{}, act_next);
+        if constexpr (c == 3 || c == 7) load_cf(c >> 2, l0 + 2 * kQ_CH);
       }
       __builtin_amdgcn_sched_barrier(0);
     });
