@@ -2549,7 +2549,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   const int g = (lane & 15) >> 2, cc = lane & 3;
   const int bcol = mg * MG + 4 * wave + g;
   uint4 Bq[2][8][2];
-  double inv;  // 1 / (sA sB) of this lane's map
+  int esc;  // 1 / (sA sB) of this lane's map is 2^esc (both scales are powers of two)
   {
     const float par0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
     const bool lo = cc < 2;
@@ -2584,7 +2584,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
     if (mx > 0.f && mx < 3.0e38f) (void)frexpf(4096.f / mx, &e);
     e = min(max(e - 1, -120), 120);
     const float sB = ldexpf(1.f, e);
-    inv = 1.0 / (sA * (double)sB);
+    esc = -(ilogb(sA) + ilogb((double)sB));
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -2690,20 +2690,33 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p
+    // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p.  The four
+    // ring blocks' sums are added in FP32 (rounding 2^-24 of the sum, far below the split's
+    // 2^-22) and scaled by an exact power of two: FP64 here cost about ten vector
+    // instructions per stored value, a third of the recursion's own
     if (bcol < batch && jf <= jt) {
-      const int l0 = lmin + TL * jt;
+      const int l0 = lmin + TL * jt + 8 * (lane >> 4);  // this lane's rows: l0 + 2 r + p
+      float* const pl = pcol + 2 * (cb + l0);
+      float v[2][4];
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int l = l0 + 2 * (4 * (lane >> 4) + r) + p;
-          double v = 0.0;
-#pragma unroll
-          for (int rb = 0; rb < 4; ++rb) v += (double)D[p][rb][r];
-          v *= inv;
-          if (l <= lmax) pcol[2 * (cb + l)] = (p == 1) ? sig_odd * (float)v : (float)v;
+          v[p][r] = ldexpf((D[p][0][r] + D[p][1][r]) + (D[p][2][r] + D[p][3][r]), esc);
+          if (p == 1) v[p][r] *= sig_odd;
         }
+      if (lmin + TL * jt + TL - 1 <= lmax) {  // a full tile: one address, immediate offsets
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) pl[2 * (2 * r + p)] = v[p][r];
+      } else {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (l0 + 2 * r + p <= lmax) pl[2 * (2 * r + p)] = v[p][r];
+      }
     }
   };
   // (alpha, gamma) of step j of tile jt (zero past lmax, and at lmin: the seed step)
