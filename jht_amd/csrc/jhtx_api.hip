@@ -370,7 +370,7 @@ size_t adjoint_scratch_bytes(const jhtx_plan* p, int batch, size_t* part_offset)
   const size_t v_bytes = align256(f_elem * (size_t)batch * p->ncomp * p->nrings * p->M);
   size_t part;
   if (mixed && !single_route(p, batch))
-    part = adj_lfirst_bytes(p) + sizeof(float2) * (size_t)mixed_adj_chunks(p) * batch * p->ncomp * p->K;
+    part = adj_lfirst_bytes(p) + sizeof(float2) * (size_t)mixed_adj_chunks(p) * batch * p->ncomp * adj_part_len(p);
   else
     part = sizeof(double2) * (size_t)adj_rec_chunks(p) * batch * p->ncomp * p->K;
   if (part_offset) *part_offset = v_bytes;
@@ -501,7 +501,7 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   size_t part_bytes;
   if (p->precision == JHTX_PREC_MIXED) {
     part_bytes = jhtx::adj_lfirst_bytes(p) + sizeof(float2) * (size_t)jhtx::mixed_adj_chunks(p) * p->max_batch *
-                                                 p->ncomp * p->K;
+                                                 p->ncomp * jhtx::adj_part_len(p);
     if (p->mx_single) part_bytes = std::max<size_t>(part_bytes, sizeof(double2) * (size_t)p->adj_nchunk * p->ncomp * p->K);
   } else {
     part_bytes = sizeof(double2) * (size_t)p->adj_nchunk * p->max_batch * p->ncomp * p->K;

@@ -33,6 +33,24 @@ __device__ __forceinline__ void rescale_pair(double& mu, double& mup, int& ex) {
 // consecutive orders m per XCD group in the recursion kernels' work-group order
 constexpr int kMGroup = 16;
 
+// Partial layout of the 32-map spin-2 adjoint: order m holds ntile(m) = (lmax - lmin + 32) / 32
+// tiles of 32 consecutive l from lmin = max(m, s), each tile 256-byte aligned (32 complex64),
+// so that a work-group's tile is written as whole lines.  tile_base(m) is the number of tiles
+// of the orders below m, in closed form: sum_{x=lo}^{hi} floor(x / 32) = S(hi) - S(lo - 1)
+// with S(x) = 32 q (q - 1) / 2 + (r + 1) q for x = 32 q + r.
+__host__ __device__ inline long long floor32_prefix(long long x) {
+  if (x < 0) return 0;
+  const long long q = x / 32, r = x % 32;
+  return 32 * (q * (q - 1) / 2) + (r + 1) * q;
+}
+__host__ __device__ inline long long tile_base(int m, int lmax, int s) {
+  long long t = (long long)(m < s ? m : s) * ((lmax - s + 32) / 32);  // orders below s start at l = s
+  if (m > s) t += floor32_prefix(lmax + 32 - s) - floor32_prefix(lmax + 32 - m);
+  return t;
+}
+// complex64 entries per (chunk, map, component) partial vector of that layout
+__host__ __device__ inline long long tile_part_len(int lmax, int s) { return 32 * tile_base(lmax + 1, lmax, s); }
+
 // Per-plan device tables for the recursion kernels.
 struct RecTables {
   // triangle-packed by the healpy index idx(l, m) = m(2 lmax + 1 - m)/2 + l
@@ -159,6 +177,9 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
 // bytes of the mixed adjoint's first-stored-l table (0 where its kernel stores every partial);
 // it sits at the start of the partials region
 size_t adj_lfirst_bytes(const jhtx_plan* p);
+// complex64 entries per (chunk, map, component) of the mixed adjoint's partials: the tile
+// layout's length for the 32-map spin-2 adjoint, K otherwise
+long long adj_part_len(const jhtx_plan* p);
 // ring chunks (partials per map) of the mixed adjoint
 int mixed_adj_chunks(const jhtx_plan* p);
 // F / V: complex128 (FP64 plans) or complex64 (mixed-precision plans)

@@ -1200,6 +1200,11 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
 constexpr int kQ_CH = 32;  // l steps per chunk: one matrix K block
 constexpr int kQ_PS = 72;  // plane row stride (halves): hi 0..31, lo 32..63, pad; 144 bytes
 
+// the value of lane ^ 1 (a quad permutation [1, 0, 3, 2])
+__device__ __forceinline__ float swap_pair(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xf, 0xf, false));
+}
+
 // f(integral_constant<int, I>) for I = B .. E - 1, unrolled at compile time
 template <int B, int E, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -2599,7 +2604,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
                                  __builtin_amdgcn_perm(w[5], w[4], 0x07060302u), __builtin_amdgcn_perm(w[7], w[6], 0x07060302u));
       }
   }
-  float* pcol = reinterpret_cast<float*>(part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * A.K) + (cc & 1);
+  // this lane's partial vector (map bcol, component cc / 2) in the tile layout: tile jt of
+  // order m at 32 (tile_base(m) + jt); the lane's rows 8 k + o of a tile (k = lane / 16)
+  float2* const pvec = part + (((long long)chunk * batch + bcol) * 2 + (cc >> 1)) * tile_part_len(lmax, 2) +
+                       32 * tile_base(m, lmax, 2) + 8 * (lane >> 4);
   const float sig_odd = (cc >= 2) ? -1.0f : 1.0f;
   const int ntile = (lmax - lmin + TL) / TL;
   int jf = ntile;  // first active tile (work-group uniform)
@@ -2690,33 +2698,32 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = l0 + 2 i + p.  The four
-    // ring blocks' sums are added in FP32 (rounding 2^-24 of the sum, far below the split's
-    // 2^-22) and scaled by an exact power of two: FP64 here cost about ten vector
-    // instructions per stored value, a third of the recursion's own
+    // D[i][j] at lane j + 16 (i / 4), register i % 4: row i -> l = lmin + TL jt + 2 i + p, so
+    // this lane holds rows 8 k + o, o = 2 r + p = 0..7, of column j.  The four ring blocks'
+    // sums are added in FP32 (rounding 2^-24 of the sum, far below the split's 2^-22) and
+    // scaled by an exact power of two.  The re lane (cc even) and its im partner (lane ^ 1)
+    // swap halves so that each writes four (re, im) pairs, 32 contiguous bytes: the re lane
+    // rows 0..3, the im lane rows 4..7.  Rows past lmax land in the tile's padding, which the
+    // chunk reduction does not read.
     if (bcol < batch && jf <= jt) {
-      const int l0 = lmin + TL * jt + 8 * (lane >> 4);  // this lane's rows: l0 + 2 r + p
-      float* const pl = pcol + 2 * (cb + l0);
-      float v[2][4];
+      float v[8];
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          v[p][r] = ldexpf((D[p][0][r] + D[p][1][r]) + (D[p][2][r] + D[p][3][r]), esc);
-          if (p == 1) v[p][r] *= sig_odd;
+          const float u = ldexpf((D[p][0][r] + D[p][1][r]) + (D[p][2][r] + D[p][3][r]), esc);
+          v[2 * r + p] = (p == 1) ? sig_odd * u : u;
         }
-      if (lmin + TL * jt + TL - 1 <= lmax) {  // a full tile: one address, immediate offsets
+      const bool odd = (lane & 1) != 0;
+      float w[4];  // the partner's rows o (to the re lane) or 4 + o (to the im lane)
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) pl[2 * (2 * r + p)] = v[p][r];
-      } else {
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (l0 + 2 * r + p <= lmax) pl[2 * (2 * r + p)] = v[p][r];
+      for (int o = 0; o < 4; ++o) {
+        const float lo = swap_pair(v[o]), hi = swap_pair(v[4 + o]);
+        w[o] = odd ? hi : lo;
       }
+      float4* const dst = reinterpret_cast<float4*>(pvec + 32 * jt + (odd ? 4 : 0));
+      dst[0] = odd ? make_float4(w[0], v[4], w[1], v[5]) : make_float4(v[0], w[0], v[1], w[1]);
+      dst[1] = odd ? make_float4(w[2], v[6], w[3], v[7]) : make_float4(v[2], w[2], v[3], w[3]);
     }
   };
   // (alpha, gamma) of step j of tile jt (zero past lmax, and at lmin: the seed step)
@@ -3191,6 +3198,11 @@ int mixed_group(const jhtx_plan* p, bool adjoint);
 // the 32-map spin-2 adjoint k_adj_s2p runs for this plan
 static bool adj_s2p(const jhtx_plan* p) {
   return p->spin == 2 && mixed_group(p, true) == 32 && p->mx_adj_cb && p->mx_adj_f16;
+}
+
+long long adj_part_len(const jhtx_plan* p) {
+  if (p->precision != JHTX_PREC_MIXED || !adj_s2p(p)) return p->K;
+  return std::max<long long>(p->K, tile_part_len(p->lmax, 2));
 }
 
 size_t adj_lfirst_bytes(const jhtx_plan* p) {

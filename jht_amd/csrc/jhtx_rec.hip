@@ -725,16 +725,22 @@ __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int 
   if (idx >= A.K) return;
   const int ncomp = spin ? 2 : 1;
   const double g = A.G[idx];
-  // part[((c * batch + b) * ncomp + comp) * K + idx]
-  const P* p0 = part + ((long long)b * ncomp) * A.K + idx;
-  const long long cstride = (long long)batch * ncomp * A.K;
-  int mi = 0, li = 0;
+  // part[((c * batch + b) * ncomp + comp) * len + pos]: len = K and pos = idx, or, with the
+  // first-stored-l table (the 32-map spin-2 adjoint), its tile layout: pos = 32 tile_base(m)
+  // + l - lmin (nothing stored below lmin)
+  int mi = 0, li = 0, lmin = 0;
+  long long len = A.K, pos = idx;
   if (lfirst) {
     mi = m_of_index(idx, A.lmax);
     li = (int)(idx - col_base(mi, A.lmax));
+    lmin = mi > A.sabs ? mi : A.sabs;
+    len = tile_part_len(A.lmax, A.sabs);
+    pos = 32 * tile_base(mi, A.lmax, A.sabs) + (li - lmin);
   }
+  const P* p0 = part + ((long long)b * ncomp) * len + pos;
+  const long long cstride = (long long)batch * ncomp * len;
   // chunk c holds a stored partial at this index
-  auto stored = [&](int c) { return !lfirst || li >= lfirst[(long long)c * A.M + mi]; };
+  auto stored = [&](int c) { return !lfirst || (li >= lmin && li >= lfirst[(long long)c * A.M + mi]); };
   double2 s0 = make_double2(0.0, 0.0), s1 = s0;
   int c = 0;
   for (; c + 8 <= nchunk; c += 8) {
@@ -743,7 +749,7 @@ __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int 
     for (int k = 0; k < 8; ++k) {
       const bool ok = stored(c + k);
       u[k] = ok ? p0[(c + k) * cstride] : P{};
-      if (spin) v[k] = ok ? p0[(c + k) * cstride + A.K] : P{};
+      if (spin) v[k] = ok ? p0[(c + k) * cstride + len] : P{};
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -761,7 +767,7 @@ __device__ __forceinline__ void adj_finish_body(const RecArgs& A, int spin, int 
     s0.x += (double)u.x;
     s0.y += (double)u.y;
     if (spin) {
-      const P v = p0[c * cstride + A.K];
+      const P v = p0[c * cstride + len];
       s1.x += (double)v.x;
       s1.y += (double)v.y;
     }
