@@ -914,6 +914,9 @@ __host__ __device__ inline int quad_len(int i) {  // Lq: power of two >= 2 i - 1
   return l;
 }
 
+// LDS behind a split ring's FFT exchange: sub-transforms 1..3's inputs, i <= Lq / 2 each
+inline size_t quad_stash_bytes(int Lq) { return 3 * (size_t)(Lq / 2) * sizeof(float2); }
+
 struct QuadTabs {
   const float2 *S, *T1, *T2, *T3, *T4, *T5;
 };
@@ -1059,19 +1062,37 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_synth_q(FftArgs A, const floa
 #pragma unroll
     for (int q = 0; q < 4; ++q) X[q][e] = make_float2(0.f, 0.f);
   }
+  // Every F entry of the ring read once: element k takes bins 4 k .. 4 k + 3 (the four
+  // sub-transforms' inputs, contiguous in F) times T1_k.  Sub-transform 0's stay in registers,
+  // 1..3 go to this thread's own LDS slots behind the FFT exchange (no other thread reads
+  // them).  Read per sub-transform, each pass fetched every 4th entry of the same lines and
+  // the lines came from HBM up to four times.
+  float2* const stash = reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + A.lds_bytes);
+  constexpr int H = L / 2;  // i <= L / 2
+  float2 x0[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = tid + nt * e;
+    x0[e] = make_float2(0.f, 0.f);
+    if (k < i) {
+      const float2 t1 = tb.T1[k];
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const double2 d = fold_bin2(Fp, Fm, 4 * k + r4, N, A.M);
+        const float2 v = cmul(make_float2((float)d.x, (float)d.y), t1);
+        if (r4 == 0) x0[e] = v;
+        else stash[(r4 - 1) * H + k] = v;
+      }
+    }
+    if ((e & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+  }
 #pragma unroll 1
   for (int r4 = 0; r4 < 4; ++r4) {
     float2 x[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = tid + nt * e;
-      if (k < i) {
-        const double2 d = fold_bin2(Fp, Fm, 4 * k + r4, N, A.M);
-        x[e] = cmul(make_float2((float)d.x, (float)d.y), tb.T1[k]);
-      } else {
-        x[e] = make_float2(0.f, 0.f);
-      }
-      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      x[e] = (r4 == 0) ? x0[e] : ((k < i) ? stash[(r4 - 1) * H + k] : make_float2(0.f, 0.f));
     }
     quad_convolve<E, LG>(A, x, tb.S, false, lds, tid);
 #pragma unroll
@@ -1125,19 +1146,39 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_adjoint_q(FftArgs A, const do
 #pragma unroll
     for (int q = 0; q < 4; ++q) W[q][e] = make_float2(0.f, 0.f);
   }
+  // Every sample of the ring read once: element jp takes samples 4 jp .. 4 jp + 3 (the four
+  // sub-transforms' inputs; two 16-byte loads each of Q and U, as a ring starts at an even
+  // pixel) times conj(T2_jp).  Sub-transform 0's stay in registers, 1..3 go to this thread's
+  // own LDS slots behind the FFT exchange (no other thread reads them).  Read per
+  // sub-transform, each pass fetched every 4th sample of the same lines and the lines came
+  // from HBM up to four times.
+  float2* const stash = reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + A.lds_bytes);
+  constexpr int H = L / 2;  // i <= L / 2
+  const double2* Q2 = reinterpret_cast<const double2*>(Q + st);
+  const double2* U2 = reinterpret_cast<const double2*>(U + st);
+  float2 x0[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int jp = tid + nt * e;
+    const long long jc = 2 * (long long)min(jp, i - 1);  // clamped: validity after
+    const double2 q01 = Q2[jc], q23 = Q2[jc + 1], u01 = U2[jc], u23 = U2[jc + 1];
+    x0[e] = make_float2(0.f, 0.f);
+    if (jp < i) {
+      const float2 t2 = cconj(tb.T2[jp]);
+      x0[e] = cmul(make_float2((float)q01.x, (float)u01.x), t2);
+      stash[jp] = cmul(make_float2((float)q01.y, (float)u01.y), t2);
+      stash[H + jp] = cmul(make_float2((float)q23.x, (float)u23.x), t2);
+      stash[2 * H + jp] = cmul(make_float2((float)q23.y, (float)u23.y), t2);
+    }
+    if ((e & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+  }
 #pragma unroll 1
   for (int s4 = 0; s4 < 4; ++s4) {
     float2 x[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int jp = tid + nt * e;
-      // clamped: validity after.  Cached loads: the four sub-transforms read every 4th sample
-      // of the same lines (a streaming load would fetch each line from HBM four times)
-      const long long jc = st + 4 * (long long)min(jp, i - 1) + s4;
-      const double vq = Q[jc];
-      const double vu = U[jc];
-      x[e] = (jp < i) ? cmul(make_float2((float)vq, (float)vu), cconj(tb.T2[jp])) : make_float2(0.f, 0.f);
-      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      x[e] = (s4 == 0) ? x0[e] : ((jp < i) ? stash[(s4 - 1) * H + jp] : make_float2(0.f, 0.f));
     }
     quad_convolve<E, LG>(A, x, tb.S, true, lds, tid);
 #pragma unroll
@@ -1476,7 +1517,7 @@ static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, vo
     if (!fc.npairs) continue;
     const FftArgs a = make_fft_args(p, fc);
     const dim3 grid(2 * fc.npairs, batch);
-    const size_t sh = fc.lds_bytes;
+    const size_t sh = fc.lds_bytes + quad_stash_bytes(c ? 4096 : 2048);
     if (synth) {
       if (c)
         hipLaunchKernelGGL((k_ring_synth_q<512, 8, 12>), grid, dim3(512), sh, s, a, (const float2*)in, (double*)out);
