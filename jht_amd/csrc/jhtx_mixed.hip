@@ -1319,26 +1319,31 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
   };
   auto store_b = [&](int l0c, int buf) {
     const double gsc = (double)sm.sc[sgm];
-    float vp[2][4], vm[2][4];  // [step][column]: sign + and sign - B values
+    float v[4][2];  // [bp.re, bp.im, bm.re, bm.im][step]
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const double gs = pf_ok[e] ? pf_g[e] * gsc : 0.0;
       const double er = pf_e[e].x * gs, ei = pf_e[e].y * gs, br = pf_b[e].x * gs, bi = pf_b[e].y * gs;
-      // bp = -(E + iB), bm = -(E - iB); s = (-1)^(l+m) on the folded copies
-      const float s = (((l0c + 2 * sjp + e + m) & 1) != 0) ? -1.f : 1.f;
-      const float bpr = (float)(bi - er), bpi = (float)(-ei - br), bmr = (float)(-er - bi), bmi = (float)(br - ei);
-      vp[e][0] = bpr, vp[e][1] = bpi, vp[e][2] = s * bmr, vp[e][3] = s * bmi;
-      vm[e][0] = bmr, vm[e][1] = bmi, vm[e][2] = s * bpr, vm[e][3] = s * bpi;
+      // bp = -(E + iB), bm = -(E - iB)
+      v[0][e] = (float)(bi - er), v[1][e] = (float)(-ei - br), v[2][e] = (float)(-er - bi), v[3][e] = (float)(br - ei);
     }
+    uint32_t h[4], l[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) split2_f16_rn(v[c][0], v[c][1], h[c], l[c]);
+    // The folded copies carry s = (-1)^(l+m), opposite for the two steps of a word.  The split
+    // rounds to nearest, so it is odd: s x splits into x's halves with their signs flipped,
+    // one XOR per word instead of a split.
+    const uint32_t sm_ = (((l0c + 2 * sjp + m) & 1) != 0) ? 0x00008000u : 0x80000000u;
+    // sign +: [bp.re, bp.im, s bm.re, s bm.im]; sign -: [bm.re, bm.im, s bp.re, s bp.im]
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      uint32_t h, l;
-      split2_f16_rn(vp[0][c], vp[1][c], h, l);
-      *reinterpret_cast<uint32_t*>(&sm.bp[buf][0][4 * sgm + c][2 * sjp]) = h;
-      *reinterpret_cast<uint32_t*>(&sm.bp[buf][0][4 * sgm + c][kQ_CH + 2 * sjp]) = l;
-      split2_f16_rn(vm[0][c], vm[1][c], h, l);
-      *reinterpret_cast<uint32_t*>(&sm.bp[buf][1][4 * sgm + c][2 * sjp]) = h;
-      *reinterpret_cast<uint32_t*>(&sm.bp[buf][1][4 * sgm + c][kQ_CH + 2 * sjp]) = l;
+      const uint32_t fl = (c >= 2) ? sm_ : 0u;
+      uint16_t* const rp = &sm.bp[buf][0][4 * sgm + c][2 * sjp];
+      uint16_t* const rm = &sm.bp[buf][1][4 * sgm + c][2 * sjp];
+      *reinterpret_cast<uint32_t*>(rp) = h[c] ^ fl;
+      *reinterpret_cast<uint32_t*>(rp + kQ_CH) = l[c] ^ fl;
+      *reinterpret_cast<uint32_t*>(rm) = h[c ^ 2] ^ fl;
+      *reinterpret_cast<uint32_t*>(rm + kQ_CH) = l[c ^ 2] ^ fl;
     }
   };
   uint16_t* const trow = &sm.tp[wave][lane][0];  // this lane's ring row
