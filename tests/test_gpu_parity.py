@@ -458,6 +458,34 @@ def test_mixed_adjoint_32_maps(nside, lmax, batch):
     assert ea <= MIXED_TOL
 
 
+@pytest.mark.parametrize("nside,lmax,batch", [(1, 2, 32), (2, 3, 32), (2, 5, 33), (8, 33, 32), (8, 40, 64)])
+def test_mixed_32_map_kernels_small_lmax(nside, lmax, batch):
+    """The 32-map spin-2 kernels (k_synth_s2q; k_adj_s2p with its tile layout of partials) at
+    tiny band limits, against FP64: columns of a single 32-l tile and of two, tiles running
+    past lmax (their padding rows must not reach the reduction), lmin = lmax (one step), and
+    a ragged second map group."""
+    j = _jht()
+    with _tuned(nside, lmax, 2, "mixed", mx_synth_group=32, mx_adj_group=32):
+        rng = np.random.default_rng(1700 + lmax + batch)
+        a = np.stack([np.stack([O.gaussian_alm(lmax, rng, 2), O.gaussian_alm(lmax, rng, 2, 0.1)])
+                      for _ in range(batch)])
+        at = torch.tensor(a, device="cuda")
+        v = torch.tensor(rng.standard_normal((batch, 2, 12 * nside * nside)), device="cuda")
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s64 = j.synthesis_batch(at, nside, lmax, 2, precision="f64").cpu().numpy()
+            smx = j.synthesis_batch(at, nside, lmax, 2, precision="mixed").cpu().numpy()
+            b64 = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="f64").cpu().numpy()
+            bmx = j.adjoint_synthesis_batch(v, nside, lmax, 2, precision="mixed").cpu().numpy()
+        from jht_amd import healpix as H
+        t = H.get_plan(nside, lmax, 2, batch, H._prec("mixed")).tuning
+        assert t["mx_synth_group"] == 32 and t["mx_adj_group"] == 32 and t["mx_f16"] == 1 and t["mx_synth_fold"] == 1
+    es = max(_rel(smx[i], s64[i]) for i in range(batch))
+    ea = max(_rel(bmx[i], b64[i]) for i in range(batch))
+    print(f"32-map kernels nside={nside} lmax={lmax} B={batch}: synth {es:.2e} adj {ea:.2e}", flush=True)
+    assert es <= MIXED_TOL and ea <= MIXED_TOL
+
+
 @pytest.mark.parametrize("group", [32, 16, 8])
 @pytest.mark.parametrize("nside,lmax", [(64, 128), (256, 300)])
 def test_mixed_spin0_groups(nside, lmax, group):
