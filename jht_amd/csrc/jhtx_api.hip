@@ -324,11 +324,11 @@ bool single_route(const jhtx_plan* p, int batch) {
 // The synthesis recursion kernel (stage 0) into F of the precision's type.  scale_ws: the
 // workspace's scale region, or null where the call has no workspace (then no FP16 split).
 hipError_t launch_synth_stage(const jhtx_plan* p, const void* alm, void* F, int batch, void* scale_ws,
-                              hipStream_t s) {
+                              hipStream_t s, long long ld = 0) {
   const double2* a = reinterpret_cast<const double2*>(alm);
   if (p->precision != JHTX_PREC_MIXED) return launch_synth_rec(p, a, reinterpret_cast<double2*>(F), batch, s);
   if (single_route(p, batch)) return launch_synth_rec(p, a, reinterpret_cast<float2*>(F), batch, s);
-  return launch_synth_mixed(p, a, reinterpret_cast<float2*>(F), batch, scale_ws, s);
+  return launch_synth_mixed(p, a, reinterpret_cast<float2*>(F), batch, scale_ws, s, ld);
 }
 
 // The adjoint recursion kernel (stage 3), then the sum of its ring-chunk partials into
@@ -495,7 +495,9 @@ int configure_plan(jhtx_plan* p, const jhtx_tuning* tuning) {
   shape(p->adj_R, p->adj_block, p->adj_nchunk);
   // ring coefficients F / V: complex128, or complex64 for the mixed path (FP32 ring FFT)
   const size_t f_elem = (p->precision == JHTX_PREC_MIXED) ? sizeof(float2) : sizeof(double2);
-  p->ws_F_bytes = align256(f_elem * (size_t)p->max_batch * p->ncomp * p->nrings * p->M);
+  // the mixed plans' F rows padded for the 32-map synthesis (synth_ring_ld)
+  const long long fld = (p->precision == JHTX_PREC_MIXED) ? jhtx::padded_ld(p->M) : p->M;
+  p->ws_F_bytes = align256(f_elem * (size_t)p->max_batch * p->ncomp * p->nrings * fld);
   // the mixed kernels use one ring per lane and 256-lane work-groups (a mixed plan's one-map
   // calls through the FP64 kernels keep FP64 partials)
   size_t part_bytes;
@@ -705,12 +707,13 @@ int jhtx_synth(jhtx_plan* p, const void* alm, void* map, int batch, void* ws, vo
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   void* F = ws;
   void* scale_ws = reinterpret_cast<char*>(ws) + p->ws_F_bytes + p->ws_part_bytes;
+  const long long ld = synth_ring_ld(p, batch);  // F's row stride, the same for both stages
   StageTimer t0(p, s, 0);
-  hipError_t e = launch_synth_stage(p, alm, F, batch, scale_ws, s);
+  hipError_t e = launch_synth_stage(p, alm, F, batch, scale_ws, s, ld);
   t0.done();
   if (e == hipSuccess) {
     StageTimer t1(p, s, 1);
-    e = launch_ring_synth(p, F, reinterpret_cast<double*>(map), batch, s);
+    e = launch_ring_synth(p, F, reinterpret_cast<double*>(map), batch, s, ld);
     t1.done();
   }
   if (e != hipSuccess) {

@@ -88,6 +88,7 @@ struct FftArgs {
   const float2* qtab;
   int lds_bytes;
   int bluestein_all;
+  long long Mf;  // row stride of F / V (M unless the synthesis's caller pads F's rows)
 };
 
 template <typename T> __device__ __forceinline__ const cx<T>* tw_of(const FftArgs& A);
@@ -582,7 +583,7 @@ __device__ __forceinline__ void ring_synth_body(const FftArgs& A, const cx<T>* _
   ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
   const cx<T>* ch = (A.cofs[p] >= 0) ? chirps_of<T>(A) + A.cofs[p] : nullptr;
-  const long long ringM = A.M;
+  const long long ringM = A.Mf;
   double2* scratch = fold_scratch<T>(lds, N);  // fold partials (only when N <= BLOCK)
   // the bin phase of a shifted ring: in the chirp (chirp-z), the twiddle table (power of
   // two), computed (direct DFT)
@@ -649,7 +650,7 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
   ring_kind(N, A.bluestein_all, &L);
   const long long bofs = A.bofs[p];
   const cx<T>* ch = (A.cofs[p] >= 0) ? chirps_of<T>(A) + A.cofs[p] : nullptr;
-  const long long ringM = A.M;
+  const long long ringM = A.Mf;
   const long long sn = ring_start(A.nside, rn);
   const long long ss = ring_start(A.nside, rs);
   // e^{-i pi k / N} of a shifted ring: in the chirp (chirp-z), the twiddle table (power of
@@ -815,6 +816,7 @@ FftArgs make_fft_args(const jhtx_plan* p, const FftClass& c) {
   a.qtab = p->fft.qtab;
   a.lds_bytes = c.lds_bytes;
   a.bluestein_all = (p->flags & JHTX_FLAG_BLUESTEIN_ALL) ? 1 : 0;
+  a.Mf = p->M;
   return a;
 }
 
@@ -1051,7 +1053,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_synth_q(FftArgs A, const floa
   const int N = ring_len(A.nside, p), i = N >> 2;
   const QuadTabs tb = quad_tabs(A, p, i, L);
   const int r = half ? 4 * A.nside - 2 - p : p;
-  const long long ringM = A.M;
+  const long long ringM = A.Mf;
   const float2* Fp = F + ((long long)b * 2 * A.nrings + r) * ringM;
   const float2* Fm = Fp + (long long)A.nrings * ringM;
   const float invL = 1.f / (float)L;
@@ -1196,7 +1198,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_adjoint_q(FftArgs A, const do
   // + q N (the conjugated mirror bin: -1 for k > 0, as the shift is carried by the bins)
   const float c4 = 0.70710678118654752440f;
   const float2 ph[4] = {make_float2(1.f, 0.f), make_float2(c4, -c4), make_float2(0.f, -1.f), make_float2(-c4, -c4)};
-  const long long ringM = A.M;
+  const long long ringM = A.Mf;
   float2* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
   float2* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
 #pragma unroll
@@ -1490,12 +1492,13 @@ void free_fft_tables(jhtx_plan* p) {
 }
 
 static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, void* out, int batch,
-                              hipStream_t s) {
+                              hipStream_t s, long long ld) {
   const bool f32 = (p->precision == JHTX_PREC_MIXED);
   for (int c = 0; c < 12; ++c) {
     const FftClass& fc = p->fft.cls[c];
     if (!fc.npairs) continue;
-    const FftArgs a = make_fft_args(p, fc);
+    FftArgs a = make_fft_args(p, fc);
+    if (ld) a.Mf = ld;
     const size_t sh = fc.lds_bytes;
     const dim3 grid(p->spin == 2 ? 2 * fc.npairs : fc.npairs, batch);  // spin 2: a work-group per ring
     if (p->spin == 0) {
@@ -1515,7 +1518,8 @@ static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, vo
   for (int c = 0; c < 2; ++c) {  // split polar rings (mixed spin-2 plans)
     const FftClass& fc = p->fft.qcls[c];
     if (!fc.npairs) continue;
-    const FftArgs a = make_fft_args(p, fc);
+    FftArgs a = make_fft_args(p, fc);
+    if (ld) a.Mf = ld;
     const dim3 grid(2 * fc.npairs, batch);
     const size_t sh = fc.lds_bytes + quad_stash_bytes(c ? 4096 : 2048);
     if (synth) {
@@ -1536,13 +1540,13 @@ static hipError_t launch_ring(const jhtx_plan* p, bool synth, const void* in, vo
 }
 
 hipError_t launch_ring_synth(const jhtx_plan* p, const void* F, double* map, int batch,
-                             hipStream_t s) {
-  return launch_ring(p, true, F, map, batch, s);
+                             hipStream_t s, long long ld) {
+  return launch_ring(p, true, F, map, batch, s, ld);
 }
 
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, int batch,
                                hipStream_t s) {
-  return launch_ring(p, false, map, V, batch, s);
+  return launch_ring(p, false, map, V, batch, s, 0);
 }
 
 

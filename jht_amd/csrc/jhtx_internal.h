@@ -79,6 +79,7 @@ struct RecArgs {
   long long K;
   int sabs;      // |spin| (the spin-weighted recursions start at l = max(m, |spin|))
   double ssign;  // (-1)^spin: the -spin channel's factor, -(-1)^s (E - iB)
+  long long Mf;  // row stride of F: M, or synth_ring_ld() on the 32-map synthesis's path
 };
 
 // Ring-FFT launch class: ring pairs whose transform length fits one work-group shape.
@@ -169,11 +170,18 @@ hipError_t launch_adj_finish(const jhtx_plan* p, const double2* part, double2* a
 // lfirst: the mixed adjoint's first-stored-l table (or null), adj_lfirst_bytes() before the partials
 hipError_t launch_adj_finish(const jhtx_plan* p, const float2* part, const int* lfirst, double2* alm, int batch,
                              int nchunk, hipStream_t s);
+// ld: F's row stride (0: M)
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
-                              void* scale_ws, hipStream_t s);
+                              void* scale_ws, hipStream_t s, long long ld = 0);
 constexpr int kScaleBlocks = 256;  // partial maxima per map of the FP16 split's scale pass
 hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int* lfirst, int batch,
                             hipStream_t s);
+// Row stride of the ring coefficients F in the workspace of a jhtx_synth call.  The 32-map
+// spin-2 synthesis writes F one order m per work-group, 16 consecutive m per XCD: with rows
+// rounded up to 16 entries (128 bytes) those 16 columns fill whole cache lines instead of
+// straddling two.  M everywhere else (jhtx_legendre_synth keeps the documented stride M).
+long long synth_ring_ld(const jhtx_plan* p, int batch);
+inline long long padded_ld(long long M) { return (M + 15) / 16 * 16; }
 // bytes of the mixed adjoint's first-stored-l table (0 where its kernel stores every partial);
 // it sits at the start of the partials region
 size_t adj_lfirst_bytes(const jhtx_plan* p);
@@ -183,8 +191,9 @@ long long adj_part_len(const jhtx_plan* p);
 // ring chunks (partials per map) of the mixed adjoint
 int mixed_adj_chunks(const jhtx_plan* p);
 // F / V: complex128 (FP64 plans) or complex64 (mixed-precision plans)
+// ld: the row stride of F (0: M)
 hipError_t launch_ring_synth(const jhtx_plan* p, const void* F, double* map, int batch,
-                             hipStream_t s);
+                             hipStream_t s, long long ld = 0);
 hipError_t launch_ring_adjoint(const jhtx_plan* p, const double* map, void* V, int batch,
                                hipStream_t s);
 

@@ -1451,7 +1451,7 @@ __global__ __launch_bounds__(512, 1) void k_synth_s2q(RecArgs A, const double2* 
   // are undone in FP64
   __syncthreads();
   float* ep = reinterpret_cast<float*>(&sm.tp[0][0][0]) + wave * 64 * 16;
-  const long long ringM = A.M;
+  const long long ringM = A.Mf;  // F's row stride
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
 #pragma unroll
@@ -3190,6 +3190,7 @@ RecArgs make_args_mixed(const jhtx_plan* p) {
   a.nrings = p->nrings;
   a.nchunk = (p->T + 255) / 256;
   a.K = p->K;
+  a.Mf = p->M;
   a.sabs = p->spin;  // 0 or 2 (HEALPix plans)
   a.ssign = 1.0;
   return a;
@@ -3203,6 +3204,15 @@ int mixed_group(const jhtx_plan* p, bool adjoint);
 // the 32-map spin-2 adjoint k_adj_s2p runs for this plan
 static bool adj_s2p(const jhtx_plan* p) {
   return p->spin == 2 && mixed_group(p, true) == 32 && p->mx_adj_cb && p->mx_adj_f16;
+}
+
+// padded rows where the synthesis runs k_synth_s2q (mixed spin 2, 32 maps per recursion,
+// FP16 split, parity fold; a jhtx_synth call always has the scale workspace).  Padding the
+// adjoint's V the same way (read by k_adj_s2p) measured slower, 69.1 -> 69.8 ms.
+long long synth_ring_ld(const jhtx_plan* p, int batch) {
+  const bool s2q = p->precision == JHTX_PREC_MIXED && !(p->mx_single && batch == 1) && p->spin == 2 &&
+                   mixed_group(p, false) == 32 && p->mx_f16 && p->mx_synth_fold;
+  return s2q ? padded_ld(p->M) : p->M;
 }
 
 long long adj_part_len(const jhtx_plan* p) {
@@ -3224,7 +3234,7 @@ int mixed_group(const jhtx_plan* p, bool adjoint) {
 }
 
 hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F, int batch,
-                              void* scale_ws, hipStream_t s) {
+                              void* scale_ws, hipStream_t s, long long ld) {
   const RecArgs a = make_args_mixed(p);
   const int Mpad = (p->M + 8 * kMGroup - 1) / (8 * kMGroup) * (8 * kMGroup);
   int MG = mixed_group(p, false);
@@ -3244,10 +3254,12 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     at.nchunk = (p->T + rings - 1) / rings;
     constexpr int MB = kMGroup, CB = 1;  // the decode_tile shape of both forms
     dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
-    if (MG == 32)
+    if (MG == 32) {
+      if (ld) at.Mf = ld;
       hipLaunchKernelGGL((k_synth_s2q<MB, CB>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
-    else
+    } else {
       hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, MB, CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
+    }
   } else if (p->spin == 2 && MG == 16) {
     RecArgs at = a;
     at.nchunk = (p->T + 127) / 128;

@@ -836,6 +836,7 @@ RecArgs make_args(const jhtx_plan* p, bool adjoint) {
   a.nrings = p->nrings;
   a.nchunk = adjoint ? p->adj_nchunk : p->nchunk;
   a.K = p->K;
+  a.Mf = p->M;
   a.sabs = p->spin < 0 ? -p->spin : p->spin;
   a.ssign = (p->spin & 1) ? -1.0 : 1.0;
   return a;
