@@ -562,6 +562,70 @@ __device__ __forceinline__ double2* fold_scratch(void* lds, int N) {
   return reinterpret_cast<double2*>(reinterpret_cast<char*>(lds) + off);
 }
 
+// A spin-2 power-of-two ring of N = BLOCK E samples with M <= 2 N, from registers to
+// registers: thread tid folds bins k = tid + BLOCK e straight into x[e] (the cyclic layout
+// the transform takes), and after the transform x[e] holds sample tid + BLOCK e, stored from
+// there.  Only the transform's exchanges use LDS.  The fold is ring_fold's M <= 2 N case,
+// term for term (bin k sums Fp at k and k + N, conj(Fm) at N - k and 2 N - k), so the
+// samples are those of the staged path bit for bit.
+template <int BLOCK, int E, int LG, typename T>
+__device__ __forceinline__ void ring_synth_direct(const FftArgs& A, const cx<T>* __restrict__ Fp,
+                                                  const cx<T>* __restrict__ Fm, int N, bool sh, int phase,
+                                                  int tstep, void* lds, double* __restrict__ Qr,
+                                                  double* __restrict__ Ur) {
+  constexpr int G = 8;  // bins per load group
+  static_assert(E % G == 0, "load groups of 8 bins");
+  const int tid = threadIdx.x;
+  const int M = A.M;
+  const bool wide = M > N;  // the terms at k + N and 2 N - k exist for some bins
+  const double s1 = sh ? -1.0 : 1.0;  // the q = 1 sign
+  cx<T> x[E];
+#pragma unroll
+  for (int e0 = 0; e0 < E; e0 += G) {
+    cx<T> p0[G], p1[G], q0[G], q1[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int k = tid + BLOCK * (e0 + u);
+      p0[u] = Fp[min(k, M - 1)];
+      q0[u] = Fm[min(N - k, M - 1)];
+      if (wide) {
+        p1[u] = Fp[min(k + N, M - 1)];
+        q1[u] = Fm[min(2 * N - k, M - 1)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int k = tid + BLOCK * (e0 + u);
+      double2 acc = make_double2(0.0, 0.0);
+      if (k < M) {  // m = k (q = 0)
+        acc.x += (double)p0[u].x;
+        acc.y += (double)p0[u].y;
+      }
+      if (wide && k + N < M) {  // m = k + N (q = 1)
+        acc.x += s1 * (double)p1[u].x;
+        acc.y += s1 * (double)p1[u].y;
+      }
+      if (N - k < M) {  // m = N - k (q = 1)
+        acc.x += s1 * (double)q0[u].x;
+        acc.y -= s1 * (double)q0[u].y;
+      }
+      if (wide && 2 * N - k < M) {  // m = 2 N - k (q = 2)
+        acc.x += (double)q1[u].x;
+        acc.y -= (double)q1[u].y;
+      }
+      if (phase) acc = cmul(acc, bin_phase(phase, A.tw, tstep, k, N));
+      x[e0 + u] = to_cx<T>(acc);
+    }
+  }
+  fft_pow2<E, LG, LG, T>(x, LG, tid, +1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = tid + BLOCK * e;
+    __builtin_nontemporal_store((double)x[e].x, &Qr[j]);
+    __builtin_nontemporal_store((double)x[e].y, &Ur[j]);
+  }
+}
+
 // Work-group per mirror ring pair; template on the transform kind, spin and scalar type so
 // each instantiation holds one code path (register pressure).
 // Ring pair p, map b; spin 2 runs one ring per work-group (half 0 the north ring, 1 the
@@ -613,6 +677,13 @@ __device__ __forceinline__ void ring_synth_body(const FftArgs& A, const cx<T>* _
     double* Q = map + (long long)b * 2 * A.npix;
     double* U = Q + A.npix;
     const int r = half ? rs : rn;
+    if constexpr (KIND == 1 && LGLO == LGHI) {
+      if (N == BLOCK * E && A.M <= 2 * N) {
+        ring_synth_direct<BLOCK, E, LGLO, T>(A, Fpb + r * ringM, Fmb + r * ringM, N, sh, phase, tstep, lds,
+                                             Q + ring_start(A.nside, r), U + ring_start(A.nside, r));
+        return;
+      }
+    }
     ring_fold<T>(Fpb + r * ringM, Fmb + r * ringM, A.M, N, sh, phase, A.tw, tstep, false, false, l2, scratch, tid,
                  BLOCK);
     const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, +1, bofs, ch, sh, lds, tid, BLOCK);
@@ -700,20 +771,48 @@ __device__ __forceinline__ void ring_adjoint_body(const FftArgs& A, const double
     const bool h = half;
     const int r = h ? rs : rn;
     const long long st = h ? ss : sn;
-    for (int j0 = tid; j0 < N; j0 += 8 * BLOCK) {  // 8 loads per field in flight per thread
-      double vq[8], vu[8];
+    int o = 0;
+    bool direct = false;
+    if constexpr (KIND == 1 && LGLO == LGHI) {
+      // a power-of-two ring of N = BLOCK E samples: thread tid loads samples tid + BLOCK e
+      // straight into x[e], the cyclic layout the transform takes (no LDS staging)
+      if (N == BLOCK * E) {
+        direct = true;
+        cx<T> x[E];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int jc = min(j0 + u * BLOCK, N - 1);
-        vq[u] = __builtin_nontemporal_load(&Q[st + jc]);
-        vu[u] = __builtin_nontemporal_load(&U[st + jc]);
+        for (int e0 = 0; e0 < E; e0 += 8) {  // 8 loads per field in flight per thread
+          double vq[8], vu[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int j = tid + BLOCK * (e0 + u);
+            vq[u] = __builtin_nontemporal_load(&Q[st + j]);
+            vu[u] = __builtin_nontemporal_load(&U[st + j]);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[e0 + u] = mk2((T)vq[u], (T)vu[u]);
+        }
+        fft_pow2<E, LGLO, LGHI, T>(x, LGLO, tid, -1, tw_of<T>(A), A.twL, lds, A.lds_bytes);
+#pragma unroll
+        for (int e = 0; e < E; ++e) l2[tid + BLOCK * e] = x[e];
+        __syncthreads();
       }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vq[u], (T)vu[u]);
     }
-    __syncthreads();
-    const int o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, sh, lds, tid, BLOCK);
+    if (!direct) {
+      for (int j0 = tid; j0 < N; j0 += 8 * BLOCK) {  // 8 loads per field in flight per thread
+        double vq[8], vu[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int jc = min(j0 + u * BLOCK, N - 1);
+          vq[u] = __builtin_nontemporal_load(&Q[st + jc]);
+          vu[u] = __builtin_nontemporal_load(&U[st + jc]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (j0 + u * BLOCK < N) l2[j0 + u * BLOCK] = mk2((T)vq[u], (T)vu[u]);
+      }
+      __syncthreads();
+      o = ring_dft<E, LGLO, LGHI, KIND, T>(A, N, L, -1, bofs, ch, sh, lds, tid, BLOCK);
+    }
     const cx<T>* W = l2 + o;
     cx<T>* Vp = V + (((long long)b * 2 + 0) * A.nrings + r) * ringM;
     cx<T>* Vm = V + (((long long)b * 2 + 1) * A.nrings + r) * ringM;
