@@ -177,9 +177,10 @@ constexpr int kScaleBlocks = 256;  // partial maxima per map of the FP16 split's
 hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, int* lfirst, int batch,
                             hipStream_t s);
 // Row stride of the ring coefficients F in the workspace of a jhtx_synth call.  The 32-map
-// spin-2 synthesis writes F one order m per work-group, 16 consecutive m per XCD: with rows
-// rounded up to 16 entries (128 bytes) those 16 columns fill whole cache lines instead of
-// straddling two.  M everywhere else (jhtx_legendre_synth keeps the documented stride M).
+// spin-2 synthesis writes F one order m per work-group, groups of 8 consecutive m per XCD:
+// with rows rounded up to 16 entries (128 bytes) each group's columns are one aligned
+// 64-byte run of a row instead of straddling two lines.  M everywhere else
+// (jhtx_legendre_synth keeps the documented stride M).
 long long synth_ring_ld(const jhtx_plan* p, int batch);
 inline long long padded_ld(long long M) { return (M + 15) / 16 * 16; }
 // bytes of the mixed adjoint's first-stored-l table (0 where its kernel stores every partial);

@@ -1195,8 +1195,13 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
 // rounded to nearest so the dropped term has no sign bias).  The B planes are
 // double-buffered, one barrier per chunk; the matrix instructions of chunk j and the
 // recursion of chunk j + 1 (into the tile the wave has just read into registers) form one
-// block, interleaved by column block.
+// block, interleaved by column block.  Work order decode_tile<kQ_MB, kQ_CB>: each XCD runs
+// 8 consecutive orders m x 2 ring chunks at a time.  A work-group writes its 256 KB of F as
+// 8-byte stores F[ring][m] at the end of its l loop; with 8 orders per group the lines that
+// one XCD has partly written at a time fit its L2 until the group's other orders fill them
+// (64-byte runs), where 16 orders per group overran it (F writes 2.9x the bytes of F).
 // ---------------------------------------------------------------------------------- //
+constexpr int kQ_MB = 8, kQ_CB = 2;  // work order of k_synth_s2q: orders m x ring chunks
 constexpr int kQ_CH = 32;  // l steps per chunk: one matrix K block
 constexpr int kQ_PS = 72;  // plane row stride (halves): hi 0..31, lo 32..63, pad; 144 bytes
 
@@ -3252,12 +3257,15 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     RecArgs at = a;
     const int rings = (MG == 32) ? 256 : 128;
     at.nchunk = (p->T + rings - 1) / rings;
-    constexpr int MB = kMGroup, CB = 1;  // the decode_tile shape of both forms
-    dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
     if (MG == 32) {
+      // Mpad is a multiple of 8 kQ_MB; the grid holds Mpad x ceil(nchunk / kQ_CB) x kQ_CB slots
+      static_assert((8 * kMGroup) % (8 * kQ_MB) == 0, "Mpad covers whole order groups");
       if (ld) at.Mf = ld;
-      hipLaunchKernelGGL((k_synth_s2q<MB, CB>), gt, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
+      const dim3 gq(Mpad * ((at.nchunk + kQ_CB - 1) / kQ_CB) * kQ_CB, (batch + MG - 1) / MG);
+      hipLaunchKernelGGL((k_synth_s2q<kQ_MB, kQ_CB>), gq, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
     } else {
+      constexpr int MB = kMGroup, CB = 1;
+      const dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
       hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, MB, CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
     }
   } else if (p->spin == 2 && MG == 16) {
