@@ -2521,6 +2521,10 @@ __device__ __forceinline__ int q_ring(int t, int lane) {
 // Tiles before the first tile in which any of the work-group's recursions is active (values
 // at or above 2^-256: activity only grows with l) contribute exactly zero: their partials are
 // not stored, and lfirst[chunk * M + m] records the first stored l for the chunk reduction.
+// Work order decode_tile<kA_MB, kA_CB>: each XCD runs 8 consecutive orders m x 2 ring chunks
+// at a time (the orders' V reads share cache lines; 16 x 1, 8 x 1, 8 x 4 and 16 x 2 measured
+// slower).
+constexpr int kA_MB = 8, kA_CB = 2;
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
                                                    float2* __restrict__ part, int* __restrict__ lfirst, int batch) {
   constexpr int NW = 8, MG = 32, RINGS = 256;
@@ -2535,7 +2539,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __r
   auto& s_act = sm.act;
   auto& tp = sm.tp;
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
+  if (!decode_tile<kA_MB, kA_CB>(A, (int)blockIdx.x, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int sgw = wave & 1, rbw = wave >> 1;  // this wave's recursion: sign, ring block
@@ -3313,9 +3317,10 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   // 32 maps per adjoint recursion: the FP16-split column-block kernel only
   if (MG == 32 && !adj_s2p(p)) MG = 16;
   dim3 grid(Mpad * a.nchunk, (batch + MG - 1) / MG);
+  static_assert((8 * kMGroup) % (8 * kA_MB) == 0, "Mpad covers whole order groups");
   if (MG == 32)
-    hipLaunchKernelGGL(k_adj_s2p, dim3(Mpad * a.nchunk, (batch + 31) / 32), dim3(512), 0, s, a, V, p->mx_sA, part,
-                       lfirst, batch);
+    hipLaunchKernelGGL(k_adj_s2p, dim3(Mpad * ((a.nchunk + kA_CB - 1) / kA_CB) * kA_CB, (batch + 31) / 32), dim3(512), 0,
+                       s, a, V, p->mx_sA, part, lfirst, batch);
   else if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
     hipLaunchKernelGGL(k_adj_mx_s0_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
   else if (p->spin == 0 && MG == 16)
