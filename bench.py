@@ -48,8 +48,8 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # a kernel change needs a new profile.
 TRAFFIC_PROFILE = "profiles/kernel_stats_c3_spin2_mixed_batch32.md"
 MEASURED_TRAFFIC = {
-    ("mixed", 2048, 4096, 2, 32, "adj_rec"): 91.60e9,
-    ("mixed", 2048, 4096, 2, 32, "synth_rec"): 185.16e9,
+    ("mixed", 2048, 4096, 2, 32, "adj_rec"): 107.34e9,
+    ("mixed", 2048, 4096, 2, 32, "synth_rec"): 156.73e9,
 }
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X dense FP16 matrix

@@ -179,7 +179,7 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
 // Row stride of the ring coefficients F in the workspace of a jhtx_synth call.  The 32-map
 // spin-2 synthesis writes F one order m per work-group, groups of 8 consecutive m per XCD:
 // with rows rounded up to 16 entries (128 bytes) each group's columns are one aligned
-// 64-byte run of a row instead of straddling two lines.  M everywhere else
+// 64-byte run of a row and never straddle two lines.  M everywhere else
 // (jhtx_legendre_synth keeps the documented stride M).
 long long synth_ring_ld(const jhtx_plan* p, int batch);
 inline long long padded_ld(long long M) { return (M + 15) / 16 * 16; }

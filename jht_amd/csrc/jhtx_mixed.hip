@@ -1195,11 +1195,11 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_synth_s2p(RecArgs A, const do
 // rounded to nearest so the dropped term has no sign bias).  The B planes are
 // double-buffered, one barrier per chunk; the matrix instructions of chunk j and the
 // recursion of chunk j + 1 (into the tile the wave has just read into registers) form one
-// block, interleaved by column block.  Work order decode_tile<kQ_MB, kQ_CB>: each XCD runs
-// 8 consecutive orders m x 2 ring chunks at a time.  A work-group writes its 256 KB of F as
-// 8-byte stores F[ring][m] at the end of its l loop; with 8 orders per group the lines that
-// one XCD has partly written at a time fit its L2 until the group's other orders fill them
-// (64-byte runs), where 16 orders per group overran it (F writes 2.9x the bytes of F).
+// block, interleaved by column block.  Work order decode_tile<kQ_MB, kQ_CB>: each XCD takes
+// groups of 8 consecutive orders m x 2 ring chunks, so its 32 resident work-groups are 8
+// orders x 4 chunks and each order's a_lm column is streamed by four work-groups at once
+// (two with 16 orders per group): fetch 136 -> 109 GB per call at nside 2048, 32 maps.  The
+// 8-byte stores F[ring][m] of a group are 64-byte runs of the padded rows.
 // ---------------------------------------------------------------------------------- //
 constexpr int kQ_MB = 8, kQ_CB = 2;  // work order of k_synth_s2q: orders m x ring chunks
 constexpr int kQ_CH = 32;  // l steps per chunk: one matrix K block
@@ -2521,9 +2521,9 @@ __device__ __forceinline__ int q_ring(int t, int lane) {
 // Tiles before the first tile in which any of the work-group's recursions is active (values
 // at or above 2^-256: activity only grows with l) contribute exactly zero: their partials are
 // not stored, and lfirst[chunk * M + m] records the first stored l for the chunk reduction.
-// Work order decode_tile<kA_MB, kA_CB>: each XCD runs 8 consecutive orders m x 2 ring chunks
-// at a time (the orders' V reads share cache lines; 16 x 1, 8 x 1, 8 x 4 and 16 x 2 measured
-// slower).
+// Work order decode_tile<kA_MB, kA_CB>: each XCD takes groups of 8 consecutive orders m x 2
+// ring chunks (adjacent orders read adjacent V entries).  16 x 1, 8 x 1, 8 x 4 and 16 x 2
+// measured slower, although 8 x 2 fetches more (V lines shared by 8 orders instead of 16).
 constexpr int kA_MB = 8, kA_CB = 2;
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
                                                    float2* __restrict__ part, int* __restrict__ lfirst, int batch) {
