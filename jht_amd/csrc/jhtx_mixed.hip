@@ -98,6 +98,9 @@ __device__ __forceinline__ bool decode_tile(const RecArgs& A, int P, int& m, int
   return m < A.M && chunk < A.nchunk;
 }
 
+// work order (decode_tile) of the 32-map spin-2 adjoint and the 16-map column-block one
+constexpr int kA_MB = 8, kA_CB = 2;
+
 // ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
 __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
 
@@ -963,9 +966,9 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s2h(RecArgs A, const d
 // v_mfma_f32_16x16x32_f16 contracts 8 consecutive steps.  Operands are split into FP16
 // hi + lo halves after power-of-two scales, as in k_synth_tile_s2h: A halves (hi, lo, hi,
 // lo) of two steps from the wave's tile, B halves (hi, hi, lo, lo) of the same steps,
-// staged in LDS so that one 16-byte read is the B operand.  Work-groups take the grouped
-// order (decode_block<true>): the 16 consecutive orders m of one ring chunk run back to back
-// on one XCD, so their 8-byte stores F[ring][m] meet in that XCD's L2 as 128-byte runs.
+// staged in LDS so that one 16-byte read is the B operand.  Work order decode_tile<MB, CB>,
+// launched as <8, 2> like k_synth_s2q: groups of 8 consecutive orders m x 2 ring chunks per
+// XCD (16 x 1 before: 53.3 -> 51.5 ms for 16 maps at nside 2048).
 // Instantiated for 16 maps (4 waves, 128 rings, 3 work-groups per CU); 32 maps per
 // recursion run k_synth_s2q below.
 // ---------------------------------------------------------------------------------- //
@@ -2315,7 +2318,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s2_cbh(RecArgs A, const float
   __shared__ __align__(16) uint32_t tiles[NW][2][TL][TS2];  // [ring block][sign][row][col]: (hi, lo)
   __shared__ int s_act[NW];
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
+  if (!decode_tile<kA_MB, kA_CB>(A, (int)blockIdx.x, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -2524,7 +2527,6 @@ __device__ __forceinline__ int q_ring(int t, int lane) {
 // Work order decode_tile<kA_MB, kA_CB>: each XCD takes groups of 8 consecutive orders m x 2
 // ring chunks (adjacent orders read adjacent V entries).  16 x 1, 8 x 1, 8 x 4 and 16 x 2
 // measured slower, although 8 x 2 fetches more (V lines shared by 8 orders instead of 16).
-constexpr int kA_MB = 8, kA_CB = 2;
 __global__ __launch_bounds__(512, 1) void k_adj_s2p(RecArgs A, const float2* __restrict__ V, double sA,
                                                    float2* __restrict__ part, int* __restrict__ lfirst, int batch) {
   constexpr int NW = 8, MG = 32, RINGS = 256;
@@ -3268,9 +3270,8 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
       const dim3 gq(Mpad * ((at.nchunk + kQ_CB - 1) / kQ_CB) * kQ_CB, (batch + MG - 1) / MG);
       hipLaunchKernelGGL((k_synth_s2q<kQ_MB, kQ_CB>), gq, dim3(512), 0, s, at, alm, sB, p->mx_sA, F, batch);
     } else {
-      constexpr int MB = kMGroup, CB = 1;
-      const dim3 gt(Mpad * ((at.nchunk + CB - 1) / CB) * CB, (batch + MG - 1) / MG);
-      hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, MB, CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
+      const dim3 gt(Mpad * ((at.nchunk + kQ_CB - 1) / kQ_CB) * kQ_CB, (batch + MG - 1) / MG);
+      hipLaunchKernelGGL((k_synth_s2p<4, 4, 3, kQ_MB, kQ_CB>), gt, dim3(256), 0, s, at, alm, sB, p->mx_sA, F, batch);
     }
   } else if (p->spin == 2 && MG == 16) {
     RecArgs at = a;
@@ -3328,7 +3329,8 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
   else if (p->spin == 0)
     hipLaunchKernelGGL(k_adj_mx_s0, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16 && p->mx_adj_cb && p->mx_adj_f16)
-    hipLaunchKernelGGL(k_adj_mx_s2_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
+    hipLaunchKernelGGL(k_adj_mx_s2_cbh, dim3(Mpad * ((a.nchunk + kA_CB - 1) / kA_CB) * kA_CB, grid.y), dim3(256), 0, s, a,
+                       V, p->mx_sA, part, batch);
   else if (MG == 16 && p->mx_adj_cb)
     hipLaunchKernelGGL(k_adj_mx_s2_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (MG == 16)
