@@ -98,7 +98,8 @@ __device__ __forceinline__ bool decode_tile(const RecArgs& A, int P, int& m, int
   return m < A.M && chunk < A.nchunk;
 }
 
-// work order (decode_tile) of the 32-map spin-2 adjoint and the 16-map column-block one
+// work order (decode_tile) of the 32-map spin-2 adjoint, the 16-map column-block adjoints
+// and the 16-map spin-0 FP16-split synthesis
 constexpr int kA_MB = 8, kA_CB = 2;
 
 // ring coefficients of the mixed path are complex64 (the FP32 ring FFT consumes them)
@@ -1697,7 +1698,7 @@ __global__ __launch_bounds__(256, MINB) void k_synth_tile_s0h(RecArgs A, const d
   uint32_t* s_b = s_buf;               // [l][2 column + {hi hi, lo lo}]
   uint32_t* tiles = s_buf + CH * ASH;  // [wave][buffer][step][lane]: (hi, lo)
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
+  if (!decode_tile<kA_MB, kA_CB>(A, (int)blockIdx.x, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ii = lane & 15, kk = lane >> 4;
@@ -3014,7 +3015,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_mx_s0_cbh(RecArgs A, const float
   __shared__ __align__(16) uint32_t tiles[NW][TL][TS2];  // [ring block][row][col]: (hi, lo)
   __shared__ int s_act[NW];
   int m, chunk;
-  if (!decode_block<true>(A, m, chunk)) return;
+  if (!decode_tile<kA_MB, kA_CB>(A, (int)blockIdx.x, m, chunk)) return;
   const int mg = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lmax = A.lmax;
@@ -3293,7 +3294,8 @@ hipError_t launch_synth_mixed(const jhtx_plan* p, const double2* alm, float2* F,
     float* sB = reinterpret_cast<float*>(part + (size_t)kScaleBlocks * batch);
     hipLaunchKernelGGL(k_alm_absmax, dim3(kScaleBlocks, batch), dim3(256), 0, s, alm, p->K, part);
     hipLaunchKernelGGL(k_alm_scale, dim3(batch), dim3(256), 0, s, part, p->mx_gmax, sB);
-    hipLaunchKernelGGL((k_synth_tile_s0h<2, 2>), grid, dim3(256), 0, s, a, alm, sB, p->mx_sA, F, batch);
+    hipLaunchKernelGGL((k_synth_tile_s0h<2, 2>), dim3(Mpad * ((a.nchunk + kA_CB - 1) / kA_CB) * kA_CB, grid.y), dim3(256), 0,
+                       s, a, alm, sB, p->mx_sA, F, batch);
   } else if (p->spin == 0 && MG == 16)
     hipLaunchKernelGGL((k_synth_tile_s0<2, 2>), grid, dim3(256), 0, s, a, alm, F, batch);
   else if (p->spin == 0)
@@ -3323,7 +3325,8 @@ hipError_t launch_adj_mixed(const jhtx_plan* p, const float2* V, float2* part, i
     hipLaunchKernelGGL(k_adj_s2p, dim3(Mpad * ((a.nchunk + kA_CB - 1) / kA_CB) * kA_CB, (batch + 31) / 32), dim3(512), 0,
                        s, a, V, p->mx_sA, part, lfirst, batch);
   else if (p->spin == 0 && MG == 16 && p->mx_adj_f16)
-    hipLaunchKernelGGL(k_adj_mx_s0_cbh, grid, dim3(256), 0, s, a, V, p->mx_sA, part, batch);
+    hipLaunchKernelGGL(k_adj_mx_s0_cbh, dim3(Mpad * ((a.nchunk + kA_CB - 1) / kA_CB) * kA_CB, grid.y), dim3(256), 0, s, a,
+                       V, p->mx_sA, part, batch);
   else if (p->spin == 0 && MG == 16)
     hipLaunchKernelGGL(k_adj_mx_s0_cb, grid, dim3(256), 0, s, a, V, part, batch);
   else if (p->spin == 0)
