@@ -580,8 +580,34 @@ __device__ __forceinline__ void ring_synth_direct(const FftArgs& A, const cx<T>*
   const bool wide = M > N;  // the terms at k + N and 2 N - k exist for some bins
   const double s1 = sh ? -1.0 : 1.0;  // the q = 1 sign
   cx<T> x[E];
+  if (!wide) {
+    // M <= N: bin k takes Fp at k (k < M) and conj(Fm) at N - k (N - k < M) only; all E
+    // bins' loads are issued before the first is used
+    cx<T> p0[E], q0[E];
 #pragma unroll
-  for (int e0 = 0; e0 < E; e0 += G) {
+    for (int e = 0; e < E; ++e) {
+      const int k = tid + BLOCK * e;
+      p0[e] = Fp[min(k, M - 1)];
+      q0[e] = Fm[min(N - k, M - 1)];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = tid + BLOCK * e;
+      double2 acc = make_double2(0.0, 0.0);
+      if (k < M) {
+        acc.x += (double)p0[e].x;
+        acc.y += (double)p0[e].y;
+      }
+      if (N - k < M) {
+        acc.x += s1 * (double)q0[e].x;
+        acc.y -= s1 * (double)q0[e].y;
+      }
+      if (phase) acc = cmul(acc, bin_phase(phase, A.tw, tstep, k, N));
+      x[e] = to_cx<T>(acc);
+    }
+  }
+#pragma unroll
+  for (int e0 = 0; e0 < E && wide; e0 += G) {
     cx<T> p0[G], p1[G], q0[G], q1[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
