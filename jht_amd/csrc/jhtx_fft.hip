@@ -1200,13 +1200,18 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_synth_q(FftArgs A, const floa
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = tid + nt * e;
+    // loads at a clamped element with no branch around them (validity after): a branch
+    // would make the element's loads wait apart from the next element's
+    const int kc = min(k, i - 1);
+    const float2 t1 = tb.T1[kc];
+    double2 d[4];
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) d[r4] = fold_bin2(Fp, Fm, 4 * kc + r4, N, A.M);
     x0[e] = make_float2(0.f, 0.f);
     if (k < i) {
-      const float2 t1 = tb.T1[k];
 #pragma unroll
       for (int r4 = 0; r4 < 4; ++r4) {
-        const double2 d = fold_bin2(Fp, Fm, 4 * k + r4, N, A.M);
-        const float2 v = cmul(make_float2((float)d.x, (float)d.y), t1);
+        const float2 v = cmul(make_float2((float)d[r4].x, (float)d[r4].y), t1);
         if (r4 == 0) x0[e] = v;
         else stash[(r4 - 1) * H + k] = v;
       }
@@ -1289,9 +1294,10 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_adjoint_q(FftArgs A, const do
     const int jp = tid + nt * e;
     const long long jc = 2 * (long long)min(jp, i - 1);  // clamped: validity after
     const double2 q01 = Q2[jc], q23 = Q2[jc + 1], u01 = U2[jc], u23 = U2[jc + 1];
+    const float2 t2c = tb.T2[min(jp, i - 1)];
     x0[e] = make_float2(0.f, 0.f);
     if (jp < i) {
-      const float2 t2 = cconj(tb.T2[jp]);
+      const float2 t2 = cconj(t2c);
       x0[e] = cmul(make_float2((float)q01.x, (float)u01.x), t2);
       stash[jp] = cmul(make_float2((float)q01.y, (float)u01.y), t2);
       stash[H + jp] = cmul(make_float2((float)q23.x, (float)u23.x), t2);
