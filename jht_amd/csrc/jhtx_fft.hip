@@ -1230,11 +1230,13 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_synth_q(FftArgs A, const floa
 #pragma unroll
     for (int e = 0; e < NY; ++e) {
       const int j1 = tid + nt * e;
+      const int jc = min(j1, i - 1);  // table loads with no branch around them
+      const float2 t2 = tb.T2[jc], t3 = tb.T3[jc];
       if (j1 < i) {
-        const float2 y = cmul(cmul(cscale(x[e], invL), tb.T2[j1]), up[e]);
+        const float2 y = cmul(cmul(cscale(x[e], invL), t2), up[e]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) X[q][e] = cadd(X[q][e], rot_i(y, q * r4));
-        up[e] = cmul(up[e], tb.T3[j1]);
+        up[e] = cmul(up[e], t3);
       }
     }
   }
@@ -1317,11 +1319,13 @@ __global__ __launch_bounds__(BLOCK, 4) void k_ring_adjoint_q(FftArgs A, const do
 #pragma unroll
     for (int e = 0; e < NY; ++e) {
       const int k1 = tid + nt * e;
+      const int kc = min(k1, i - 1);  // table loads with no branch around them
+      const float2 t4 = tb.T4[kc], t5 = tb.T5[kc];
       if (k1 < i) {
-        const float2 z = cmul(cmul(cscale(x[e], invL), cconj(tb.T4[k1])), up[e]);
+        const float2 z = cmul(cmul(cscale(x[e], invL), cconj(t4)), up[e]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) W[q][e] = cadd(W[q][e], rot_i(z, -q * s4));
-        up[e] = cmul(up[e], cconj(tb.T5[k1]));
+        up[e] = cmul(up[e], cconj(t5));
       }
     }
   }
